@@ -1,0 +1,118 @@
+// Internal interfaces between the HIP kernel files and the solver driver.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "qms_grid.h"
+
+namespace qms {
+
+// Yee material coefficients, node-sized arrays.
+struct Coeffs {
+    double *im[3];   // 1/Mu_x, 1/Mu_y, 1/Mu_z
+    double *sg[3];   // Sigma_x, Sigma_y, Sigma_z
+    float *im32[3];
+    float *sg32[3];
+};
+struct CoeffsD {
+    const double *im[3];
+    const double *sg[3];
+};
+struct CoeffsF {
+    const float *im[3];
+    const float *sg[3];
+};
+
+struct OpParams {
+    double v[3];   // frame velocity (solve_vx, solve_vy, solve_vz)
+    double dt;     // del_t (transient rows only)
+    int tr;        // 1: transient rows T(v,dt), 0: steady rows S(v)
+    int adv;       // 1 if any velocity component is non-zero
+};
+
+enum { OP_RESID = 0, OP_APPLY = 1, OP_VA = 2 };
+
+void launch_coeffs(const Grid &g, const uint16_t *slot, const double *mat, const Coeffs &c, hipStream_t s);
+void launch_diag(const Grid &g, const CoeffsD &c, const OpParams &op, double *D, hipStream_t s);
+void launch_rhs(const Grid &g, const CoeffsD &c, const BC &bc, const OpParams &op, const double *aprev, double *b,
+                hipStream_t s);
+void launch_setbc(const Grid &g, const BC &bc, const OpParams &op, double *x, hipStream_t s);
+void launch_resid(const Grid &g, const CoeffsD &c, const OpParams &op, const double *x, const double *b, double *r,
+                  hipStream_t s);
+void launch_apply(const Grid &g, const CoeffsD &c, const OpParams &op, const double *x, double *y, hipStream_t s);
+void launch_va_f32(const Grid &g, const CoeffsD &c, const OpParams &op, const float *x, float *y, hipStream_t s);
+void launch_va_f64(const Grid &g, const CoeffsD &c, const OpParams &op, const double *x, double *y, hipStream_t s);
+void launch_hfield(const Grid &g, const CoeffsD &c, const double *x, double *H, int unit_mu, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Multigrid hierarchy (one per block: the coupled A block or the V block).
+// Level 0 is evaluated on the fly from the material coefficients; levels
+// >= 1 hold explicit Galerkin stencils in difference form:
+//   (A x)_p = e_p x_p + sum_d c_{p,d} (x_p - x_{p+d}) + sum_s a_{p,s} y_{p+o_s}
+// where d runs over the 6 axis neighbours of the same component and s over
+// the 4+4 cross-component couplings (A block only).
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Level {
+    int nx, ny, nz;
+    int64_t n1;
+    int ncomp;            // 3 (A block) or 1 (V block)
+    T *cond;              // [ncomp][6][n1] conductances (-x,+x,-y,+y,-z,+z)
+    T *exc;               // [ncomp][n1] excess diagonal
+    T *cross;             // [ncomp][8][n1] cross-component coefficients (A block)
+    T *dinv;              // [ncomp][n1] 1/diag, 0 on inactive rows
+    T *x, *b, *r, *t;     // [ncomp][n1] work vectors (x,b unused on level 0)
+};
+
+struct MGParams {
+    int nu;               // pre/post smoothing sweeps
+    double omega;         // Jacobi damping
+    double alpha;         // coarse-correction scaling
+};
+
+template <typename T>
+struct Hierarchy {
+    int block;            // 0: A block (3 components), 1: V block
+    int nlev;             // number of levels (level 0 on the fly)
+    Level<T> lev[16];
+    // coarsest-level dense inverse
+    int ncoarse;          // active unknowns on the coarsest level
+    int *coarse_idx;      // [ncoarse] -> comp*n1 + p on the coarsest level
+    double *coarse_inv;   // [ncoarse*ncoarse] row-major FP64
+};
+
+// Level-0 data shared by both blocks.
+struct Fine {
+    Grid g;
+    CoeffsF c32;
+    CoeffsD c64;
+    double dt;            // 0 for steady (no mass term)
+};
+
+template <typename T>
+void mg_build(const Fine &f, Hierarchy<T> &h, hipStream_t s);
+template <typename T>
+void mg_vcycle(const Fine &f, Hierarchy<T> &h, const MGParams &mp, T *x0, const T *b0, hipStream_t s);
+// One level-0 smoother sweep (for kernel timing).
+template <typename T>
+void mg_fine_sweep(const Fine &f, Hierarchy<T> &h, const MGParams &mp, const T *x, const T *b, T *xo, hipStream_t s);
+
+// conversions / vector kernels
+void launch_d2t(const double *src, const double *scale, float *dst, int64_t n, hipStream_t s);
+void launch_d2t(const double *src, const double *scale, double *dst, int64_t n, hipStream_t s);
+void launch_t2d(const float *src, double *dst, int64_t n, hipStream_t s);
+void launch_t2d(const double *src, double *dst, int64_t n, hipStream_t s);
+void launch_sub(float *a, const float *b, int64_t n, hipStream_t s);
+void launch_sub(double *a, const double *b, int64_t n, hipStream_t s);
+
+// Krylov helpers (FP64)
+void launch_multidot(const double *const *V, int nv, const double *w, int64_t n, double *partial, int nblk,
+                     hipStream_t s);
+void launch_reduce_partials(const double *partial, int nv, int nblk, double *out, hipStream_t s);
+void launch_orth_update(double *const *V, int nv, const double *h, double *w, int64_t n, double *partial, int nblk,
+                        hipStream_t s);
+void launch_scale(double *dst, const double *src, double a, int64_t n, hipStream_t s);
+void launch_axpy_list(double *x, const double *const *Z, int nz, const double *y_dev, int64_t n, hipStream_t s);
+void launch_norm2(const double *w, int64_t n, double *partial, int nblk, hipStream_t s);
+int reduce_blocks(int64_t n);
+
+}  // namespace qms

@@ -1,0 +1,574 @@
+// Solver driver and C ABI of libqms.so.
+//
+// Per time step (QMS_3D.f90:1424-2234 restated matrix-free):
+//   b  <- RHS(A^n, v)                     (k_rhs)
+//   x  <- warm start with Dirichlet values (k_setbc)
+//   FGMRES(m) on D^-1 M x = D^-1 b, restarted with an FP64 residual:
+//     each iteration: z = P^-1 v (block-triangular multigrid preconditioner,
+//     FP32 or FP64), w = D^-1 M z (fused FP64 operator), classical
+//     Gram-Schmidt with fused multi-dot kernels.
+// D = diag(M) equilibrates rows whose scales span 1e-3..1e15.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+#include <string>
+#include <vector>
+#include <stdexcept>
+#include <memory>
+#include "qms_kernels.h"
+#include "../../include/qms.h"
+
+using namespace qms;
+
+namespace {
+
+struct HipError : std::runtime_error {
+    int code;
+    HipError(const std::string &m, int c) : std::runtime_error(m), code(c) {}
+};
+
+void check(hipError_t e, const char *what) {
+    if (e == hipErrorOutOfMemory) throw HipError(std::string(what) + ": out of device memory", QMS_ERR_NOMEM);
+    if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e), QMS_ERR_HIP);
+}
+
+std::string g_last_error;
+
+}  // namespace
+
+struct qms_handle {
+    qms_config cfg;
+    Grid g;
+    BC bc;
+    hipStream_t s = nullptr;
+    std::vector<void *> allocs;
+    int64_t bytes = 0;
+    std::string err;
+
+    uint16_t *slot = nullptr;
+    double *mat = nullptr;
+    Coeffs coef;
+    CoeffsD c64;
+    CoeffsF c32;
+
+    int64_t n = 0;              // 4*n1
+    double *x = nullptr, *b = nullptr, *r = nullptr, *w = nullptr, *tmp = nullptr;
+    double *Dst = nullptr, *Dtr = nullptr;   // row scaling, steady / transient
+    std::vector<double *> V, Z;
+    double **dV = nullptr, **dZ = nullptr;
+    double *partial = nullptr, *dsmall = nullptr;
+    double *hsmall = nullptr;   // pinned
+    int nblk = 1;
+
+    // preconditioner work vectors (4*n1 in MG precision)
+    void *rT = nullptr, *zT = nullptr, *tT = nullptr;
+    Hierarchy<float> hA32[2], hV32;
+    Hierarchy<double> hA64[2], hV64;
+    bool built[2] = {false, false};
+    bool builtV = false;
+    bool have_state = false;
+
+    qms_stats st;
+
+    template <typename T>
+    T *dalloc(int64_t count) {
+        void *p = nullptr;
+        const size_t nb = sizeof(T) * (size_t)(count > 0 ? count : 1);
+        check(hipMalloc(&p, nb), "hipMalloc");
+        allocs.push_back(p);
+        bytes += (int64_t)nb;
+        check(hipMemsetAsync(p, 0, nb, s), "hipMemsetAsync");
+        return (T *)p;
+    }
+    ~qms_handle() {
+        if (s) hipStreamSynchronize(s);
+        for (void *p : allocs) hipFree(p);
+        if (hsmall) hipHostFree(hsmall);
+        for (auto *h : {&hA32[0], &hA32[1], &hV32}) {
+            if (h->nlev) { hipFree(h->coarse_idx); hipFree(h->coarse_inv); }
+        }
+        for (auto *h : {&hA64[0], &hA64[1], &hV64}) {
+            if (h->nlev) { hipFree(h->coarse_idx); hipFree(h->coarse_inv); }
+        }
+        if (s) hipStreamDestroy(s);
+    }
+
+    Fine fine(double dt) const {
+        Fine f;
+        f.g = g;
+        f.c32 = c32;
+        f.c64 = c64;
+        f.dt = dt;
+        return f;
+    }
+
+    template <typename T>
+    void alloc_hierarchy(Hierarchy<T> &h, int block) {
+        memset(&h, 0, sizeof(h));
+        h.block = block;
+        const int nc = block == 0 ? 3 : 1;
+        int nx = g.nx, ny = g.ny, nz = g.nz;
+        int l = 0;
+        while (true) {
+            Level<T> &L = h.lev[l];
+            L.nx = nx; L.ny = ny; L.nz = nz;
+            L.n1 = (int64_t)nx * ny * nz;
+            L.ncomp = nc;
+            L.t = dalloc<T>(nc * L.n1);
+            L.r = dalloc<T>(nc * L.n1);
+            if (l > 0) {
+                L.x = dalloc<T>(nc * L.n1);
+                L.b = dalloc<T>(nc * L.n1);
+                L.cond = dalloc<T>((int64_t)nc * 6 * L.n1);
+                L.exc = dalloc<T>(nc * L.n1);
+                L.dinv = dalloc<T>(nc * L.n1);
+                if (nc == 3) L.cross = dalloc<T>((int64_t)nc * 8 * L.n1);
+            }
+            ++l;
+            const int mx = std::max(nx, std::max(ny, nz));
+            if ((mx <= 4 && l >= 2) || l >= 15) break;
+            nx = (nx + 1) / 2; ny = (ny + 1) / 2; nz = (nz + 1) / 2;
+        }
+        h.nlev = l;
+    }
+};
+
+namespace {
+
+OpParams make_op(const double v[3], double dt, int tr) {
+    OpParams op;
+    op.v[0] = v[0]; op.v[1] = v[1]; op.v[2] = v[2];
+    op.dt = dt;
+    op.tr = tr;
+    op.adv = (v[0] != 0.0 || v[1] != 0.0 || v[2] != 0.0) ? 1 : 0;
+    return op;
+}
+
+MGParams mgp(const qms_handle *h) {
+    MGParams m;
+    m.nu = h->cfg.mg_nu;
+    m.omega = h->cfg.mg_omega;
+    m.alpha = h->cfg.mg_alpha;
+    return m;
+}
+
+template <typename T> Hierarchy<T> &hierA(qms_handle *h, int tr);
+template <> Hierarchy<float> &hierA<float>(qms_handle *h, int tr) { return h->hA32[tr]; }
+template <> Hierarchy<double> &hierA<double>(qms_handle *h, int tr) { return h->hA64[tr]; }
+template <typename T> Hierarchy<T> &hierV(qms_handle *h);
+template <> Hierarchy<float> &hierV<float>(qms_handle *h) { return h->hV32; }
+template <> Hierarchy<double> &hierV<double>(qms_handle *h) { return h->hV64; }
+
+void launch_va(qms_handle *h, const OpParams &op, const float *x, float *y) { launch_va_f32(h->g, h->c64, op, x, y, h->s); }
+void launch_va(qms_handle *h, const OpParams &op, const double *x, double *y) { launch_va_f64(h->g, h->c64, op, x, y, h->s); }
+
+template <typename T>
+void ensure_mg(qms_handle *h, int tr) {
+    const Fine f = h->fine(tr ? h->cfg.dt : 0.0);
+    if (!h->builtV) {
+        h->alloc_hierarchy(hierV<T>(h), 1);
+        mg_build<T>(f, hierV<T>(h), h->s);
+        h->builtV = true;
+    }
+    if (!h->built[tr]) {
+        h->alloc_hierarchy(hierA<T>(h, tr), 0);
+        mg_build<T>(f, hierA<T>(h, tr), h->s);
+        h->built[tr] = true;
+    }
+    h->st.n_levels_a = hierA<T>(h, tr).nlev;
+    h->st.n_levels_v = hierV<T>(h).nlev;
+}
+
+// z = P^-1 v : block lower-triangular [K_A 0; M_VA L_V]^-1 with one V-cycle
+// per diagonal block, applied to D v (unscaled residual units).
+template <typename T>
+void precond(qms_handle *h, const OpParams &op, int tr, const double *D, const double *v, double *z) {
+    const int64_t n1 = h->g.n1, n = h->n;
+    T *rT = (T *)h->rT, *zT = (T *)h->zT, *tT = (T *)h->tT;
+    const Fine f = h->fine(tr ? h->cfg.dt : 0.0);
+    const MGParams mp = mgp(h);
+    launch_d2t(v, D, rT, n, h->s);
+    mg_vcycle<T>(f, hierA<T>(h, tr), mp, zT, rT, h->s);
+    if (op.tr || op.adv) {
+        launch_va(h, op, zT, tT);
+        launch_sub(rT + 3 * n1, tT + 3 * n1, n1, h->s);
+    }
+    mg_vcycle<T>(f, hierV<T>(h), mp, zT + 3 * n1, rT + 3 * n1, h->s);
+    launch_t2d(zT, z, n, h->s);
+}
+
+double norm_dev(qms_handle *h, const double *w) {
+    launch_norm2(w, h->n, h->partial, h->nblk, h->s);
+    launch_reduce_partials(h->partial, 1, h->nblk, h->dsmall, h->s);
+    check(hipMemcpyAsync(h->hsmall, h->dsmall, sizeof(double), hipMemcpyDeviceToHost, h->s), "d2h");
+    check(hipStreamSynchronize(h->s), "sync");
+    return sqrt(h->hsmall[0]);
+}
+
+// Restarted FGMRES with FP64 residual recomputation at each restart.
+template <typename T>
+int fgmres(qms_handle *h, const OpParams &op, int tr) {
+    const int64_t n = h->n;
+    const int m = h->cfg.restart;
+    const double *D = tr ? h->Dtr : h->Dst;
+    hipStream_t s = h->s;
+    // ||D^-1 b||: residual of x = 0
+    launch_scale(h->w, h->x, 0.0, n, s);
+    launch_resid(h->g, h->c64, op, h->w, h->b, h->r, s);
+    const double bnorm = norm_dev(h, h->r);
+    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), gv(m + 1), y(m);
+    int total = 0, restarts = 0;
+    double rel = 0.0, rel0 = -1.0;
+    const double tol = h->cfg.tol;
+    while (true) {
+        launch_resid(h->g, h->c64, op, h->x, h->b, h->r, s);
+        const double beta = norm_dev(h, h->r);
+        rel = bnorm > 0 ? beta / bnorm : beta;
+        if (rel0 < 0) rel0 = rel;
+        if (!(rel == rel)) throw HipError("residual is NaN", QMS_ERR_NOCONV);
+        if (rel <= tol || beta == 0.0) break;
+        if (total >= h->cfg.max_iters) {
+            h->st.last_iters = total; h->st.last_restarts = restarts; h->st.last_relres = rel;
+            h->st.last_relres0 = rel0;
+            char buf[160];
+            snprintf(buf, sizeof buf, "FGMRES stopped after %d iterations at relative residual %.3e (tol %.1e)",
+                     total, rel, tol);
+            throw HipError(buf, QMS_ERR_NOCONV);
+        }
+        ++restarts;
+        launch_scale(h->V[0], h->r, 1.0 / beta, n, s);
+        std::fill(gv.begin(), gv.end(), 0.0);
+        gv[0] = beta;
+        int k = 0;
+        for (int j = 0; j < m && total < h->cfg.max_iters; ++j) {
+            precond<T>(h, op, tr, D, h->V[j], h->Z[j]);
+            launch_apply(h->g, h->c64, op, h->Z[j], h->w, s);
+            // classical Gram-Schmidt, one fused pass for the dots, one for the update
+            launch_multidot((const double *const *)h->dV, j + 1, h->w, n, h->partial, h->nblk, s);
+            launch_reduce_partials(h->partial, j + 1, h->nblk, h->dsmall, s);
+            launch_orth_update(h->dV, j + 1, h->dsmall, h->w, n, h->partial, h->nblk, s);
+            launch_reduce_partials(h->partial, 1, h->nblk, h->dsmall + (j + 1), s);
+            check(hipMemcpyAsync(h->hsmall, h->dsmall, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, s), "d2h");
+            check(hipStreamSynchronize(s), "sync");
+            for (int i = 0; i <= j; ++i) H[(size_t)i * m + j] = h->hsmall[i];
+            const double hn = sqrt(h->hsmall[j + 1]);
+            H[(size_t)(j + 1) * m + j] = hn;
+            ++total;
+            // Givens
+            for (int i = 0; i < j; ++i) {
+                const double a = H[(size_t)i * m + j], bb = H[(size_t)(i + 1) * m + j];
+                H[(size_t)i * m + j] = cs[i] * a + sn[i] * bb;
+                H[(size_t)(i + 1) * m + j] = -sn[i] * a + cs[i] * bb;
+            }
+            const double a = H[(size_t)j * m + j], bb = H[(size_t)(j + 1) * m + j];
+            const double dd = hypot(a, bb);
+            cs[j] = dd > 0 ? a / dd : 1.0;
+            sn[j] = dd > 0 ? bb / dd : 0.0;
+            H[(size_t)j * m + j] = dd;
+            H[(size_t)(j + 1) * m + j] = 0.0;
+            gv[j + 1] = -sn[j] * gv[j];
+            gv[j] = cs[j] * gv[j];
+            k = j + 1;
+            const double est = fabs(gv[j + 1]) / (bnorm > 0 ? bnorm : 1.0);
+            if (hn == 0.0 || est <= tol * 0.5) break;
+            launch_scale(h->V[j + 1], h->w, 1.0 / hn, n, s);
+        }
+        // back substitution
+        for (int i = k - 1; i >= 0; --i) {
+            double t = gv[i];
+            for (int c = i + 1; c < k; ++c) t -= H[(size_t)i * m + c] * y[c];
+            y[i] = t / H[(size_t)i * m + i];
+        }
+        memcpy(h->hsmall, y.data(), sizeof(double) * k);
+        check(hipMemcpyAsync(h->dsmall, h->hsmall, sizeof(double) * k, hipMemcpyHostToDevice, s), "h2d");
+        launch_axpy_list(h->x, (const double *const *)h->dZ, k, h->dsmall, n, s);
+        check(hipStreamSynchronize(s), "sync");
+    }
+    h->st.last_iters = total;
+    h->st.last_restarts = restarts;
+    h->st.last_relres = rel;
+    h->st.last_relres0 = rel0;
+    h->st.total_iters += total;
+    h->st.solves += 1;
+    return QMS_OK;
+}
+
+int solve(qms_handle *h, const OpParams &op, int tr) {
+    if (h->cfg.precision == QMS_MIXED) {
+        ensure_mg<float>(h, tr);
+        return fgmres<float>(h, op, tr);
+    }
+    ensure_mg<double>(h, tr);
+    return fgmres<double>(h, op, tr);
+}
+
+int fail(qms_handle *h, const std::exception &e, int code) {
+    if (h) h->err = e.what();
+    g_last_error = e.what();
+    return code;
+}
+
+}  // namespace
+
+#define QMS_TRY(h) try {
+#define QMS_CATCH(h)                                                   \
+    }                                                                  \
+    catch (const HipError &e) { return fail(h, e, e.code); }           \
+    catch (const std::bad_alloc &e) { return fail(h, e, QMS_ERR_NOMEM); } \
+    catch (const std::exception &e) { return fail(h, e, QMS_ERR_ARG); }
+
+extern "C" {
+
+int qms_config_default(qms_config *cfg) {
+    if (!cfg) return QMS_ERR_ARG;
+    memset(cfg, 0, sizeof(*cfg));
+    cfg->precision = QMS_FP64;
+    cfg->tol = 1e-12;
+    cfg->restart = 30;
+    cfg->max_iters = 3000;
+    cfg->mg_nu = 4;
+    cfg->mg_omega = 0.7;
+    cfg->mg_alpha = 1.8;
+    return QMS_OK;
+}
+
+int qms_create(const qms_config *cfg, const uint16_t *cell_slot, const double *slot_props, int nslots,
+               qms_handle **out) {
+    if (!cfg || !cell_slot || !slot_props || !out || nslots < 1) {
+        g_last_error = "qms_create: null argument or no material slots";
+        return QMS_ERR_ARG;
+    }
+    *out = nullptr;
+    for (int a = 0; a < 3; ++a) {
+        if (cfg->cells[a] < 5 || cfg->cells[a] > 4096) {
+            g_last_error = "qms_create: cells per axis must be in [5, 4096]";
+            return QMS_ERR_ARG;
+        }
+        if (!(cfg->h[a] > 0)) { g_last_error = "qms_create: grid steps must be positive"; return QMS_ERR_ARG; }
+    }
+    if (cfg->restart < 1 || cfg->restart > 64 || cfg->mg_nu < 1 || !(cfg->tol > 0) || cfg->max_iters < 1 ||
+        (cfg->precision != QMS_FP64 && cfg->precision != QMS_MIXED)) {
+        g_last_error = "qms_create: invalid solver parameters";
+        return QMS_ERR_ARG;
+    }
+    std::unique_ptr<qms_handle> h(new qms_handle());
+    QMS_TRY(h.get())
+    h->cfg = *cfg;
+    memset(&h->st, 0, sizeof(h->st));
+    Grid &g = h->g;
+    g.nx = cfg->cells[0] + 1; g.ny = cfg->cells[1] + 1; g.nz = cfg->cells[2] + 1;
+    g.n1 = (int64_t)g.nx * g.ny * g.nz;
+    g.hx = cfg->h[0]; g.hy = cfg->h[1]; g.hz = cfg->h[2];
+    for (int a = 0; a < 3; ++a) h->bc.B[a] = cfg->bc[a];
+    for (int a = 0; a < 9; ++a) h->bc.G[a] = cfg->bc[3 + a];
+    check(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking), "hipStreamCreate");
+    const int64_t nc = (int64_t)cfg->cells[0] * cfg->cells[1] * cfg->cells[2];
+    for (int64_t p = 0; p < nc; ++p)
+        if (cell_slot[p] >= nslots) throw std::runtime_error("qms_create: cell slot outside the material table");
+    h->slot = h->dalloc<uint16_t>(nc);
+    h->mat = h->dalloc<double>(6 * (int64_t)nslots);
+    check(hipMemcpyAsync(h->slot, cell_slot, sizeof(uint16_t) * nc, hipMemcpyHostToDevice, h->s), "h2d");
+    check(hipMemcpyAsync(h->mat, slot_props, sizeof(double) * 6 * nslots, hipMemcpyHostToDevice, h->s), "h2d");
+    for (int a = 0; a < 3; ++a) {
+        h->coef.im[a] = h->dalloc<double>(g.n1);
+        h->coef.sg[a] = h->dalloc<double>(g.n1);
+        h->coef.im32[a] = h->dalloc<float>(g.n1);
+        h->coef.sg32[a] = h->dalloc<float>(g.n1);
+        h->c64.im[a] = h->coef.im[a]; h->c64.sg[a] = h->coef.sg[a];
+        h->c32.im[a] = h->coef.im32[a]; h->c32.sg[a] = h->coef.sg32[a];
+    }
+    launch_coeffs(g, h->slot, h->mat, h->coef, h->s);
+    h->n = 4 * g.n1;
+    h->x = h->dalloc<double>(h->n);
+    h->b = h->dalloc<double>(h->n);
+    h->r = h->dalloc<double>(h->n);
+    h->w = h->dalloc<double>(h->n);
+    h->tmp = h->dalloc<double>(h->n);
+    h->Dst = h->dalloc<double>(h->n);
+    h->Dtr = h->dalloc<double>(h->n);
+    const double v0[3] = {0, 0, 0};
+    launch_diag(g, h->c64, make_op(v0, 0.0, 0), h->Dst, h->s);
+    launch_diag(g, h->c64, make_op(v0, cfg->dt, cfg->dt > 0 ? 1 : 0), h->Dtr, h->s);
+    const int m = cfg->restart;
+    for (int j = 0; j <= m; ++j) h->V.push_back(h->dalloc<double>(h->n));
+    for (int j = 0; j < m; ++j) h->Z.push_back(h->dalloc<double>(h->n));
+    h->dV = h->dalloc<double *>(m + 1);
+    h->dZ = h->dalloc<double *>(m);
+    check(hipMemcpyAsync(h->dV, h->V.data(), sizeof(double *) * (m + 1), hipMemcpyHostToDevice, h->s), "h2d");
+    check(hipMemcpyAsync(h->dZ, h->Z.data(), sizeof(double *) * m, hipMemcpyHostToDevice, h->s), "h2d");
+    h->nblk = reduce_blocks(h->n);
+    h->partial = h->dalloc<double>((int64_t)(m + 2) * h->nblk);
+    h->dsmall = h->dalloc<double>(m + 8);
+    check(hipHostMalloc((void **)&h->hsmall, sizeof(double) * (m + 8), hipHostMallocDefault), "hipHostMalloc");
+    const size_t tsz = cfg->precision == QMS_MIXED ? sizeof(float) : sizeof(double);
+    if (tsz == sizeof(float)) {
+        h->rT = h->dalloc<float>(h->n); h->zT = h->dalloc<float>(h->n); h->tT = h->dalloc<float>(h->n);
+    } else {
+        h->rT = h->dalloc<double>(h->n); h->zT = h->dalloc<double>(h->n); h->tT = h->dalloc<double>(h->n);
+    }
+    check(hipStreamSynchronize(h->s), "sync");
+    h->st.device_bytes = h->bytes;
+    *out = h.release();
+    return QMS_OK;
+    QMS_CATCH(h.get())
+}
+
+int qms_steady(qms_handle *h, const double v[3]) {
+    if (!h || !v) return QMS_ERR_ARG;
+    QMS_TRY(h)
+    const OpParams op = make_op(v, 0.0, 0);
+    const double zero[3] = {0, 0, 0};
+    (void)zero;
+    launch_rhs(h->g, h->c64, h->bc, op, h->x, h->b, h->s);
+    launch_scale(h->x, h->x, 0.0, h->n, h->s);
+    launch_setbc(h->g, h->bc, op, h->x, h->s);
+    solve(h, op, 0);
+    h->have_state = true;
+    return QMS_OK;
+    QMS_CATCH(h)
+}
+
+int qms_step(qms_handle *h, const double v[3]) {
+    if (!h || !v) return QMS_ERR_ARG;
+    if (!h->have_state) { h->err = "qms_step: no state (call qms_steady or qms_set_fields first)"; return QMS_ERR_STATE; }
+    if (!(h->cfg.dt > 0)) { h->err = "qms_step: dt must be positive"; return QMS_ERR_ARG; }
+    QMS_TRY(h)
+    const OpParams op = make_op(v, h->cfg.dt, 1);
+    launch_rhs(h->g, h->c64, h->bc, op, h->x, h->b, h->s);
+    launch_setbc(h->g, h->bc, op, h->x, h->s);
+    solve(h, op, 1);
+    return QMS_OK;
+    QMS_CATCH(h)
+}
+
+int qms_run(qms_handle *h, int nsteps, const double *v) {
+    if (!h || (nsteps > 0 && !v) || nsteps < 0) return QMS_ERR_ARG;
+    for (int sidx = 0; sidx < nsteps; ++sidx) {
+        const int rc = qms_step(h, v + 3 * sidx);
+        if (rc != QMS_OK) return rc;
+    }
+    return QMS_OK;
+}
+
+int qms_get_fields(qms_handle *h, double *out) {
+    if (!h || !out) return QMS_ERR_ARG;
+    QMS_TRY(h)
+    check(hipMemcpyAsync(out, h->x, sizeof(double) * h->n, hipMemcpyDeviceToHost, h->s), "d2h");
+    check(hipStreamSynchronize(h->s), "sync");
+    return QMS_OK;
+    QMS_CATCH(h)
+}
+
+int qms_set_fields(qms_handle *h, const double *in) {
+    if (!h || !in) return QMS_ERR_ARG;
+    QMS_TRY(h)
+    check(hipMemcpyAsync(h->x, in, sizeof(double) * h->n, hipMemcpyHostToDevice, h->s), "h2d");
+    check(hipStreamSynchronize(h->s), "sync");
+    h->have_state = true;
+    return QMS_OK;
+    QMS_CATCH(h)
+}
+
+int qms_get_h(qms_handle *h, int unit_mu, double *out) {
+    if (!h || !out) return QMS_ERR_ARG;
+    QMS_TRY(h)
+    const int64_t nc = (int64_t)h->cfg.cells[0] * h->cfg.cells[1] * h->cfg.cells[2];
+    launch_hfield(h->g, h->c64, h->x, h->tmp, unit_mu, h->s);
+    check(hipMemcpyAsync(out, h->tmp, sizeof(double) * 3 * nc, hipMemcpyDeviceToHost, h->s), "d2h");
+    check(hipStreamSynchronize(h->s), "sync");
+    return QMS_OK;
+    QMS_CATCH(h)
+}
+
+int qms_apply(qms_handle *h, const double v[3], double dt, const double *x, double *y) {
+    if (!h || !v || !x || !y) return QMS_ERR_ARG;
+    QMS_TRY(h)
+    const int tr = dt > 0 ? 1 : 0;
+    const OpParams op = make_op(v, dt, tr);
+    check(hipMemcpyAsync(h->r, x, sizeof(double) * h->n, hipMemcpyHostToDevice, h->s), "h2d");
+    launch_apply(h->g, h->c64, op, h->r, h->w, h->s);
+    launch_diag(h->g, h->c64, op, h->tmp, h->s);
+    check(hipMemcpyAsync(y, h->w, sizeof(double) * h->n, hipMemcpyDeviceToHost, h->s), "d2h");
+    std::vector<double> D(h->n);
+    check(hipMemcpyAsync(D.data(), h->tmp, sizeof(double) * h->n, hipMemcpyDeviceToHost, h->s), "d2h");
+    check(hipStreamSynchronize(h->s), "sync");
+    for (int64_t q = 0; q < h->n; ++q) y[q] *= D[q];
+    return QMS_OK;
+    QMS_CATCH(h)
+}
+
+int qms_rhs(qms_handle *h, const double v[3], double dt, const double *aprev, double *b) {
+    if (!h || !v || !aprev || !b) return QMS_ERR_ARG;
+    QMS_TRY(h)
+    const OpParams op = make_op(v, dt, dt > 0 ? 1 : 0);
+    check(hipMemcpyAsync(h->r, aprev, sizeof(double) * h->n, hipMemcpyHostToDevice, h->s), "h2d");
+    launch_rhs(h->g, h->c64, h->bc, op, h->r, h->w, h->s);
+    check(hipMemcpyAsync(b, h->w, sizeof(double) * h->n, hipMemcpyDeviceToHost, h->s), "d2h");
+    check(hipStreamSynchronize(h->s), "sync");
+    return QMS_OK;
+    QMS_CATCH(h)
+}
+
+int qms_get_stats(const qms_handle *h, qms_stats *st) {
+    if (!h || !st) return QMS_ERR_ARG;
+    *st = h->st;
+    st->device_bytes = h->bytes;
+    return QMS_OK;
+}
+
+int qms_time_kernel(qms_handle *h, int which, int reps, double *ms_per_launch, double *bytes_per_launch) {
+    if (!h || reps < 1 || !ms_per_launch || !bytes_per_launch || which < 0 || which > 2) return QMS_ERR_ARG;
+    QMS_TRY(h)
+    const int tr = h->cfg.dt > 0 ? 1 : 0;
+    const double v0[3] = {0, 0, 0};
+    const OpParams op = make_op(v0, h->cfg.dt, tr);
+    const int64_t n1 = h->g.n1, n = h->n;
+    const bool mixed = h->cfg.precision == QMS_MIXED;
+    const double s = mixed ? 4.0 : 8.0;
+    if (which >= 1) {
+        if (mixed) ensure_mg<float>(h, tr); else ensure_mg<double>(h, tr);
+    }
+    hipEvent_t e0, e1;
+    check(hipEventCreate(&e0), "event");
+    check(hipEventCreate(&e1), "event");
+    auto launch = [&]() {
+        if (which == 0) {
+            launch_apply(h->g, h->c64, op, h->x, h->w, h->s);
+        } else {
+            const Fine f = h->fine(tr ? h->cfg.dt : 0.0);
+            const MGParams mp = mgp(h);
+            if (mixed) {
+                if (which == 1) mg_fine_sweep<float>(f, h->hA32[tr], mp, (const float *)h->zT, (const float *)h->rT, (float *)h->tT, h->s);
+                else mg_fine_sweep<float>(f, h->hV32, mp, (const float *)h->zT + 3 * n1, (const float *)h->rT + 3 * n1, (float *)h->tT + 3 * n1, h->s);
+            } else {
+                if (which == 1) mg_fine_sweep<double>(f, h->hA64[tr], mp, (const double *)h->zT, (const double *)h->rT, (double *)h->tT, h->s);
+                else mg_fine_sweep<double>(f, h->hV64, mp, (const double *)h->zT + 3 * n1, (const double *)h->rT + 3 * n1, (double *)h->tT + 3 * n1, h->s);
+            }
+        }
+    };
+    launch();
+    check(hipEventRecord(e0, h->s), "event");
+    for (int r = 0; r < reps; ++r) launch();
+    check(hipEventRecord(e1, h->s), "event");
+    check(hipEventSynchronize(e1), "event");
+    float ms = 0;
+    check(hipEventElapsedTime(&ms, e0, e1), "event");
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    *ms_per_launch = ms / reps;
+    // Algorithmic bytes: each input and output array streamed once.
+    if (which == 0) *bytes_per_launch = 8.0 * (2.0 * (double)n + 6.0 * (double)n1);
+    else if (which == 1) *bytes_per_launch = s * (3.0 * 3.0 * (double)n1 + 6.0 * (double)n1);
+    else *bytes_per_launch = s * (3.0 * (double)n1 + 3.0 * (double)n1);
+    return QMS_OK;
+    QMS_CATCH(h)
+}
+
+void qms_destroy(qms_handle *h) { delete h; }
+
+const char *qms_last_error(const qms_handle *h) {
+    if (h) return h->err.c_str();
+    return g_last_error.c_str();
+}
+
+}  // extern "C"

@@ -1,0 +1,245 @@
+"""ctypes binding of libqms.so (declared in include/qms.h).
+
+There is no CPU fallback: if the library is missing or no GPU is visible, the
+calls raise.  Arrays cross the boundary as numpy float64 / uint16 buffers.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libqms.so")
+
+QMS_OK = 0
+QMS_ERR_ARG = -1
+QMS_ERR_HIP = -2
+QMS_ERR_NOCONV = -3
+QMS_ERR_NOMEM = -4
+QMS_ERR_STATE = -5
+QMS_FP64 = 0
+QMS_MIXED = 1
+
+# Every symbol include/qms.h declares.
+EXPORTS = (
+    "qms_config_default", "qms_create", "qms_steady", "qms_step", "qms_run",
+    "qms_get_fields", "qms_set_fields", "qms_get_h", "qms_apply", "qms_rhs",
+    "qms_get_stats", "qms_time_kernel", "qms_destroy", "qms_last_error",
+)
+
+
+class QmsError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"libqms error {code}: {msg}")
+        self.code = code
+
+
+class NotConverged(QmsError):
+    pass
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("cells", ctypes.c_int * 3),
+        ("h", ctypes.c_double * 3),
+        ("dt", ctypes.c_double),
+        ("bc", ctypes.c_double * 12),
+        ("precision", ctypes.c_int),
+        ("tol", ctypes.c_double),
+        ("restart", ctypes.c_int),
+        ("max_iters", ctypes.c_int),
+        ("mg_nu", ctypes.c_int),
+        ("mg_omega", ctypes.c_double),
+        ("mg_alpha", ctypes.c_double),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("last_iters", ctypes.c_int),
+        ("last_restarts", ctypes.c_int),
+        ("last_relres", ctypes.c_double),
+        ("last_relres0", ctypes.c_double),
+        ("total_iters", ctypes.c_int64),
+        ("solves", ctypes.c_int64),
+        ("device_bytes", ctypes.c_int64),
+        ("n_levels_a", ctypes.c_int),
+        ("n_levels_v", ctypes.c_int),
+    ]
+
+
+_lib = None
+
+
+def load(path: Optional[str] = None):
+    """Load libqms.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(f"{p} is missing: build it with `python -m alpineqms_amd.build`")
+    L = ctypes.CDLL(p)
+    d = ctypes.POINTER(ctypes.c_double)
+    H = ctypes.c_void_p
+    L.qms_config_default.argtypes = [ctypes.POINTER(Config)]
+    L.qms_create.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(ctypes.c_uint16), d, ctypes.c_int,
+                             ctypes.POINTER(ctypes.c_void_p)]
+    L.qms_steady.argtypes = [H, d]
+    L.qms_step.argtypes = [H, d]
+    L.qms_run.argtypes = [H, ctypes.c_int, d]
+    L.qms_get_fields.argtypes = [H, d]
+    L.qms_set_fields.argtypes = [H, d]
+    L.qms_get_h.argtypes = [H, ctypes.c_int, d]
+    L.qms_apply.argtypes = [H, d, ctypes.c_double, d, d]
+    L.qms_rhs.argtypes = [H, d, ctypes.c_double, d, d]
+    L.qms_get_stats.argtypes = [H, ctypes.POINTER(Stats)]
+    L.qms_time_kernel.argtypes = [H, ctypes.c_int, ctypes.c_int, d, d]
+    L.qms_destroy.argtypes = [H]
+    L.qms_destroy.restype = None
+    L.qms_last_error.argtypes = [H]
+    L.qms_last_error.restype = ctypes.c_char_p
+    for name in EXPORTS:
+        if name not in ("qms_destroy", "qms_last_error"):
+            getattr(L, name).restype = ctypes.c_int
+    if path is None:
+        _lib = L
+    return L
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def default_config() -> Config:
+    c = Config()
+    load().qms_config_default(ctypes.byref(c))
+    return c
+
+
+class Solver:
+    """One design on the GPU (a ``qms_handle``)."""
+
+    def __init__(self, cells: Sequence[int], h: Sequence[float], dt: float, bc: Sequence[float],
+                 cell_slot: np.ndarray, slot_props: np.ndarray, precision: int = QMS_FP64,
+                 tol: float = 1e-12, restart: int = 30, max_iters: int = 3000, mg_nu: int = 4,
+                 mg_omega: float = 0.7, mg_alpha: float = 1.8):
+        L = load()
+        cfg = default_config()
+        for a in range(3):
+            cfg.cells[a] = int(cells[a])
+            cfg.h[a] = float(h[a])
+        cfg.dt = float(dt)
+        bcv = np.asarray(bc, dtype=np.float64).ravel()
+        if bcv.size != 12:
+            raise ValueError("bc needs 12 values (B0 and the 3x3 gradient)")
+        for a in range(12):
+            cfg.bc[a] = float(bcv[a])
+        cfg.precision = int(precision)
+        cfg.tol = float(tol)
+        cfg.restart = int(restart)
+        cfg.max_iters = int(max_iters)
+        cfg.mg_nu = int(mg_nu)
+        cfg.mg_omega = float(mg_omega)
+        cfg.mg_alpha = float(mg_alpha)
+        self.cells = tuple(int(c) for c in cells)
+        self.nodes = tuple(c + 1 for c in self.cells)
+        slot = np.ascontiguousarray(cell_slot, dtype=np.uint16).ravel()
+        if slot.size != np.prod(self.cells):
+            raise ValueError("cell_slot must hold cells_x*cells_y*cells_z entries")
+        props = np.ascontiguousarray(slot_props, dtype=np.float64).reshape(-1, 6)
+        out = ctypes.c_void_p()
+        rc = L.qms_create(ctypes.byref(cfg), slot.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)), _dp(props),
+                          props.shape[0], ctypes.byref(out))
+        if rc != QMS_OK:
+            raise QmsError(rc, L.qms_last_error(None).decode())
+        self._h = out
+        self._L = L
+        self.precision = precision
+
+    # -- helpers -----------------------------------------------------------
+    def _check(self, rc: int):
+        if rc != QMS_OK:
+            msg = self._L.qms_last_error(self._h).decode()
+            if rc == QMS_ERR_NOCONV:
+                raise NotConverged(rc, msg)
+            raise QmsError(rc, msg)
+
+    @property
+    def n1(self) -> int:
+        return int(np.prod(self.nodes))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.qms_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- solver entry points -------------------------------------------------
+    def steady(self, v):
+        vv = np.ascontiguousarray(v, dtype=np.float64)
+        self._check(self._L.qms_steady(self._h, _dp(vv)))
+
+    def step(self, v):
+        vv = np.ascontiguousarray(v, dtype=np.float64)
+        self._check(self._L.qms_step(self._h, _dp(vv)))
+
+    def run(self, velocities: np.ndarray):
+        vv = np.ascontiguousarray(velocities, dtype=np.float64).reshape(-1, 3)
+        self._check(self._L.qms_run(self._h, vv.shape[0], _dp(vv)))
+
+    def get_fields(self) -> np.ndarray:
+        nx, ny, nz = self.nodes
+        out = np.empty((4, nz, ny, nx), dtype=np.float64)
+        self._check(self._L.qms_get_fields(self._h, _dp(out)))
+        return out
+
+    def set_fields(self, f: np.ndarray):
+        nx, ny, nz = self.nodes
+        a = np.ascontiguousarray(f, dtype=np.float64).reshape(4, nz, ny, nx)
+        self._check(self._L.qms_set_fields(self._h, _dp(a)))
+
+    def get_h(self, unit_mu: bool = False) -> np.ndarray:
+        cx, cy, cz = self.cells
+        out = np.empty((3, cz, cy, cx), dtype=np.float64)
+        self._check(self._L.qms_get_h(self._h, 1 if unit_mu else 0, _dp(out)))
+        return out
+
+    def apply(self, v, dt: float, x: np.ndarray) -> np.ndarray:
+        vv = np.ascontiguousarray(v, dtype=np.float64)
+        xx = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(xx)
+        self._check(self._L.qms_apply(self._h, _dp(vv), float(dt), _dp(xx), _dp(y)))
+        return y
+
+    def rhs(self, v, dt: float, aprev: np.ndarray) -> np.ndarray:
+        vv = np.ascontiguousarray(v, dtype=np.float64)
+        a = np.ascontiguousarray(aprev, dtype=np.float64)
+        b = np.empty_like(a)
+        self._check(self._L.qms_rhs(self._h, _dp(vv), float(dt), _dp(a), _dp(b)))
+        return b
+
+    def stats(self) -> Stats:
+        s = Stats()
+        self._check(self._L.qms_get_stats(self._h, ctypes.byref(s)))
+        return s
+
+    def time_kernel(self, which: int, reps: int):
+        ms = ctypes.c_double()
+        by = ctypes.c_double()
+        self._check(self._L.qms_time_kernel(self._h, which, reps, ctypes.byref(ms), ctypes.byref(by)))
+        return ms.value, by.value

@@ -1,0 +1,131 @@
+/*
+ * qms.h -- C ABI of libqms.so, the MI355X time-stepping core of the AlpineQMS
+ * quasi-magnetostatic A-V solver (DanRich123/AlpineQMS, 3D/QMS_3D.f90).
+ *
+ * The reference has no library interface: one Fortran program assembles a
+ * CSR matrix every step and calls MKL PARDISO.  Each entry point below
+ * replaces one stage of that program; the comment names the reference lines
+ * it stands in for.  Conventions:
+ *   - every function returns QMS_OK (0) or a negative QMS_ERR_* code, never
+ *     exits; qms_last_error() gives the message;
+ *   - host buffers are borrowed for the duration of the call and copied;
+ *   - device memory is owned by the handle; one handle is used by one thread
+ *     at a time; distinct handles may run concurrently (each has its own HIP
+ *     stream);
+ *   - node-sized field arrays have extents (nx, ny, nz) = cells + 1, x
+ *     fastest (the reference's node-sized storage, QMS_3D.f90:288-299);
+ *     cell arrays have extents cells, x fastest (Fortran order).
+ */
+#ifndef QMS_H
+#define QMS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QMS_OK 0
+#define QMS_ERR_ARG (-1)      /* invalid argument or size */
+#define QMS_ERR_HIP (-2)      /* HIP runtime error (message has the detail) */
+#define QMS_ERR_NOCONV (-3)   /* Krylov solve did not reach the tolerance */
+#define QMS_ERR_NOMEM (-4)    /* device allocation failed */
+#define QMS_ERR_STATE (-5)    /* call out of order (e.g. step before steady) */
+
+#define QMS_FP64 0            /* Krylov and multigrid in FP64 */
+#define QMS_MIXED 1           /* FP64 Krylov / operator, FP32 multigrid */
+
+#define QMS_FIELD_AX 0
+#define QMS_FIELD_AY 1
+#define QMS_FIELD_AZ 2
+#define QMS_FIELD_V 3
+
+typedef struct qms_handle qms_handle;
+
+typedef struct {
+    int cells[3];        /* x_size, y_size, z_size (inputs.txt line 1, QMS_3D.f90:123) */
+    double h[3];         /* del_x, del_y, del_z (line 2, :124) */
+    double dt;           /* del_t (line 3, :125) */
+    double bc[12];       /* Bx,By,Bz then gxx..gzz (lines 8-11, :129-132) */
+    int precision;       /* QMS_FP64 or QMS_MIXED */
+    double tol;          /* ||D^-1 (b - M x)|| <= tol ||D^-1 b||, D = diag(M) */
+    int restart;         /* FGMRES restart length */
+    int max_iters;       /* Krylov iteration cap per solve */
+    int mg_nu;           /* Jacobi sweeps before and after each coarse correction */
+    double mg_omega;     /* Jacobi damping */
+    double mg_alpha;     /* coarse-correction scaling */
+} qms_config;
+
+typedef struct {
+    int last_iters;          /* Krylov iterations of the last solve */
+    int last_restarts;       /* FP64 restarts (residual recomputations) */
+    double last_relres;      /* final scaled relative residual */
+    double last_relres0;     /* initial scaled relative residual */
+    int64_t total_iters;     /* since creation */
+    int64_t solves;
+    int64_t device_bytes;    /* device memory held by the handle */
+    int n_levels_a, n_levels_v;
+} qms_stats;
+
+/* Fills defaults: FP64, tol 1e-12, restart 30, 400 iterations, nu 4,
+ * omega 0.7, alpha 1.8. */
+int qms_config_default(qms_config *cfg);
+
+/* Setup and material coefficients: replaces QMS_3D.f90:172-269 (sizes,
+ * geometry read, material -> Yee means) and :289-396 (allocations, Dirichlet
+ * masks).  cell_slot holds cells_x*cells_y*cells_z slot numbers (x fastest);
+ * slot_props holds nslots rows {mur_x, mur_y, mur_z, sigma_x, sigma_y,
+ * sigma_z}; slot 0 must be vacuum {1,1,1,0,0,0} (material ID 0 and IDs absent
+ * from the table, :234-243). */
+int qms_create(const qms_config *cfg, const uint16_t *cell_slot, const double *slot_props, int nslots,
+               qms_handle **out);
+
+/* Steady-state initialisation at frame velocity v: replaces the steady
+ * assembly and PARDISO phase 13, QMS_3D.f90:430-1317, including the unpack
+ * :1320-1343.  Initial guess is the Dirichlet lift (zero interior). */
+int qms_steady(qms_handle *h, const double v[3]);
+
+/* One implicit time step at frame velocity v (the velocity of the step, from
+ * trajectory_1): replaces the transient assembly, PARDISO phase 23/33 and
+ * unpack, QMS_3D.f90:1452-2234.  Warm-started from the current state. */
+int qms_step(qms_handle *h, const double v[3]);
+
+/* nsteps time steps with velocities v[3*s..3*s+2], fields kept on the device
+ * between steps (the device-resident time loop). */
+int qms_run(qms_handle *h, int nsteps, const double *v);
+
+/* Copy the state (4 node-sized arrays Ax, Ay, Az, V) to / from the host. */
+int qms_get_fields(qms_handle *h, double *out);
+int qms_set_fields(qms_handle *h, const double *in);
+
+/* Cell-centred H (unit_mu = 0, QMS_3D.f90:1346-1376) or B (unit_mu = 1) as
+ * 3 arrays of cells_x*cells_y*cells_z (Hx, Hy, Hz). */
+int qms_get_h(qms_handle *h, int unit_mu, double *out);
+
+/* y = M(v, dt) x on 4 node-sized arrays, where M is the reference's CSR
+ * matrix (dt <= 0: steady S(v), :436-1296; dt > 0: transient T(v, dt),
+ * :1452-2191).  Fixed rows are identity rows; entries with no unknown are 0. */
+int qms_apply(qms_handle *h, const double v[3], double dt, const double *x, double *y);
+
+/* b(A^n) of the transient rows (dt > 0) or the steady RHS (dt <= 0) on 4
+ * node-sized arrays, QMS_3D.f90:459-470, :1475, :1624, :1772, :1906-1928. */
+int qms_rhs(qms_handle *h, const double v[3], double dt, const double *aprev, double *b);
+
+int qms_get_stats(const qms_handle *h, qms_stats *st);
+
+/* Times `reps` launches of one kernel on the handle's stream with HIP events
+ * and reports the mean duration and the algorithmic HBM bytes per launch.
+ * which: 0 fused operator apply (FP64), 1 level-0 A-block smoother sweep,
+ * 2 level-0 V-block smoother sweep (precision of the handle's multigrid). */
+int qms_time_kernel(qms_handle *h, int which, int reps, double *ms_per_launch, double *bytes_per_launch);
+
+void qms_destroy(qms_handle *h);
+
+/* Message of the last failing call on h (h = NULL: last failing
+ * qms_create). */
+const char *qms_last_error(const qms_handle *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
