@@ -10,6 +10,7 @@
 // D = diag(M) equilibrates rows whose scales span 1e-3..1e15.
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 #include <string>
 #include <vector>
@@ -302,6 +303,19 @@ int solve(qms_handle *h, const OpParams &op, int tr) {
     return fgmres<double>(h, op, tr);
 }
 
+// The multigrid-preconditioned solve faulted the GPU in its first hierarchy
+// build on the two GPU runs made so far, with no cause found in this code (a
+// CPU build of the same sources runs the same tests clean under
+// AddressSanitizer).  Until the cause is shown, the solve entry points refuse
+// to run unless QMS_ENABLE_SOLVE=1 is set explicitly.
+bool solve_enabled(qms_handle *h) {
+    const char *e = getenv("QMS_ENABLE_SOLVE");
+    if (e && e[0] == '1') return true;
+    h->err = "solve path disabled: it faulted the GPU during the first multigrid hierarchy build "
+             "(cause not identified); set QMS_ENABLE_SOLVE=1 to run it";
+    return false;
+}
+
 int fail(qms_handle *h, const std::exception &e, int code) {
     if (h) h->err = e.what();
     g_last_error = e.what();
@@ -415,6 +429,7 @@ int qms_create(const qms_config *cfg, const uint16_t *cell_slot, const double *s
 
 int qms_steady(qms_handle *h, const double v[3]) {
     if (!h || !v) return QMS_ERR_ARG;
+    if (!solve_enabled(h)) return QMS_ERR_STATE;
     QMS_TRY(h)
     const OpParams op = make_op(v, 0.0, 0);
     const double zero[3] = {0, 0, 0};
@@ -432,6 +447,7 @@ int qms_step(qms_handle *h, const double v[3]) {
     if (!h || !v) return QMS_ERR_ARG;
     if (!h->have_state) { h->err = "qms_step: no state (call qms_steady or qms_set_fields first)"; return QMS_ERR_STATE; }
     if (!(h->cfg.dt > 0)) { h->err = "qms_step: dt must be positive"; return QMS_ERR_ARG; }
+    if (!solve_enabled(h)) return QMS_ERR_STATE;
     QMS_TRY(h)
     const OpParams op = make_op(v, h->cfg.dt, 1);
     launch_rhs(h->g, h->c64, h->bc, op, h->x, h->b, h->s);
@@ -526,6 +542,7 @@ int qms_time_kernel(qms_handle *h, int which, int reps, double *ms_per_launch, d
     const bool mixed = h->cfg.precision == QMS_MIXED;
     const double s = mixed ? 4.0 : 8.0;
     if (which >= 1) {
+        if (!solve_enabled(h)) return QMS_ERR_STATE;
         if (mixed) ensure_mg<float>(h, tr); else ensure_mg<double>(h, tr);
     }
     hipEvent_t e0, e1;
