@@ -114,5 +114,6 @@ void launch_scale(double *dst, const double *src, double a, int64_t n, hipStream
 void launch_axpy_list(double *x, const double *const *Z, int nz, const double *y_dev, int64_t n, hipStream_t s);
 void launch_norm2(const double *w, int64_t n, double *partial, int nblk, hipStream_t s);
 int reduce_blocks(int64_t n);
+void launch_copy(double *dst, const double *src, int64_t n, hipStream_t s);
 
 }  // namespace qms

@@ -123,6 +123,15 @@ __global__ void k_axpy_list(double *__restrict__ x, const double *const *__restr
     x[e] = s;
 }
 
+// Plain streaming copy (calibration reference for HBM byte counters).
+__global__ void k_copy(double *__restrict__ dst, const double *__restrict__ src, int64_t n) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) dst[e] = src[e];
+}
+void launch_copy(double *dst, const double *src, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dst, src, n);
+}
+
 void launch_multidot(const double *const *V, int nv, const double *w, int64_t n, double *partial, int nblk,
                      hipStream_t s) {
     for (int c0 = 0; c0 < nv; c0 += CH)

@@ -533,7 +533,7 @@ int qms_get_stats(const qms_handle *h, qms_stats *st) {
 }
 
 int qms_time_kernel(qms_handle *h, int which, int reps, double *ms_per_launch, double *bytes_per_launch) {
-    if (!h || reps < 1 || !ms_per_launch || !bytes_per_launch || which < 0 || which > 2) return QMS_ERR_ARG;
+    if (!h || reps < 1 || !ms_per_launch || !bytes_per_launch || which < 0 || which > 3) return QMS_ERR_ARG;
     QMS_TRY(h)
     const int tr = h->cfg.dt > 0 ? 1 : 0;
     const double v0[3] = {0, 0, 0};
@@ -541,7 +541,7 @@ int qms_time_kernel(qms_handle *h, int which, int reps, double *ms_per_launch, d
     const int64_t n1 = h->g.n1, n = h->n;
     const bool mixed = h->cfg.precision == QMS_MIXED;
     const double s = mixed ? 4.0 : 8.0;
-    if (which >= 1) {
+    if (which == 1 || which == 2) {
         if (!solve_enabled(h)) return QMS_ERR_STATE;
         if (mixed) ensure_mg<float>(h, tr); else ensure_mg<double>(h, tr);
     }
@@ -551,6 +551,8 @@ int qms_time_kernel(qms_handle *h, int which, int reps, double *ms_per_launch, d
     auto launch = [&]() {
         if (which == 0) {
             launch_apply(h->g, h->c64, op, h->x, h->w, h->s);
+        } else if (which == 3) {
+            launch_copy(h->w, h->x, n, h->s);
         } else {
             const Fine f = h->fine(tr ? h->cfg.dt : 0.0);
             const MGParams mp = mgp(h);
@@ -575,6 +577,7 @@ int qms_time_kernel(qms_handle *h, int which, int reps, double *ms_per_launch, d
     *ms_per_launch = ms / reps;
     // Algorithmic bytes: each input and output array streamed once.
     if (which == 0) *bytes_per_launch = 8.0 * (2.0 * (double)n + 6.0 * (double)n1);
+    else if (which == 3) *bytes_per_launch = 16.0 * (double)n;
     else if (which == 1) *bytes_per_launch = s * (3.0 * 3.0 * (double)n1 + 6.0 * (double)n1);
     else *bytes_per_launch = s * (3.0 * (double)n1 + 3.0 * (double)n1);
     return QMS_OK;

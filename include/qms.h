@@ -116,7 +116,8 @@ int qms_get_stats(const qms_handle *h, qms_stats *st);
 /* Times `reps` launches of one kernel on the handle's stream with HIP events
  * and reports the mean duration and the algorithmic HBM bytes per launch.
  * which: 0 fused operator apply (FP64), 1 level-0 A-block smoother sweep,
- * 2 level-0 V-block smoother sweep (precision of the handle's multigrid). */
+ * 2 level-0 V-block smoother sweep (precision of the handle's multigrid),
+ * 3 plain copy of the 4-field state (calibration of byte counters). */
 int qms_time_kernel(qms_handle *h, int which, int reps, double *ms_per_launch, double *bytes_per_launch);
 
 void qms_destroy(qms_handle *h);
