@@ -160,25 +160,29 @@ def config5_stream(count: int, rate: float, seed: int = 5000, nmin: int = 32, nm
 
 
 def small_box_design(N: int, seed: int, nbox: int = 3, h: float = 5e-4, dt: float = 5e-6, steps: int = 4,
-                     v: float = 0.0, ramp: bool = True, gradient: bool = False) -> Design:
+                     v: float = 0.0, ramp: bool = True, gradient: bool = False, cells=None) -> Design:
     """Small designs for parity tests: random boxes, optional frame velocity
-    ramp and an optional traceless symmetric background gradient."""
+    ramp and an optional traceless symmetric background gradient.  `cells`
+    (x, y, z) overrides the cubic N-1 cells per axis."""
     rng = np.random.default_rng(seed)
-    c = N - 1
-    geom = np.ones((c, c, c))
+    cz = tuple(cells) if cells is not None else (N - 1,) * 3
+    geom = np.ones((cz[2], cz[1], cz[0]))
     mats = [Material(1, (1.0, 1.0, 1.0), (1e-3, 1e-3, 1e-3))]
     for b in range(nbox):
-        e = rng.integers(2, max(3, c // 3), size=3)
-        lo = [int(rng.integers(3, max(4, c - 3 - e[a]))) for a in range(3)]
+        if cells is None:
+            e = rng.integers(2, max(3, cz[0] // 3), size=3)
+        else:
+            e = [int(rng.integers(2, max(3, cz[a] // 3))) for a in range(3)]
+        lo = [int(rng.integers(3, max(4, cz[a] - 3 - e[a]))) for a in range(3)]
         mur = float(10 ** rng.uniform(0, 3))
         sig = float(10 ** rng.uniform(5, np.log10(SIG_MAX)))
         mats.append(Material(b + 2, (mur, mur, mur), (sig, sig, sig)))
         geom[lo[2]:lo[2] + e[2], lo[1]:lo[1] + e[1], lo[0]:lo[0] + e[0]] = b + 2
     B0 = 0.05 * _unit(rng)
-    inp = _inputs((c, c, c), h, dt, steps, B0, v_axis_x=v, ramp=ramp, mats=mats)
+    inp = _inputs(cz, h, dt, steps, B0, v_axis_x=v, ramp=ramp, mats=mats)
     if gradient:
         g = rng.normal(size=(3, 3)) * 0.5
         g = (g + g.T) / 2
         g -= np.eye(3) * np.trace(g) / 3
         inp.G = tuple(tuple(float(x) for x in row) for row in g)
-    return Design(f"small_N{N}_s{seed}", inp, geom)
+    return Design(f"small_{'x'.join(str(c + 1) for c in cz)}_s{seed}", inp, geom)

@@ -27,7 +27,25 @@ struct OpParams {
     double dt;     // del_t (transient rows only)
     int tr;        // 1: transient rows T(v,dt), 0: steady rows S(v)
     int adv;       // 1 if any velocity component is non-zero
+    int tile_x;    // operator tile width in nodes (32, 64, 128 or 256; 256 threads per tile)
 };
+
+// Per-launch constants of the operator rows, computed once on the host so
+// that the kernels multiply instead of dividing (an FP64 division costs
+// about ten dependent VALU instructions).
+struct OpK {
+    int sy, sz;                  // node strides along y and z (nx, nx*ny)
+    int tr, adv;
+    double ih2[3];               // 1/h_a^2
+    double ihh[3][3];            // 1/(h_a h_b)
+    double ih[3];                // 1/h_a
+    double idt;                  // 1/dt (transient rows)
+    double iht[3];               // 1/(h_a dt)
+    double v1[3][3];             // v_a / (2 h_b)
+    double v2[3][3][3];          // v_a / (2 h_b h_c)
+    double v[3];                 // frame velocity (Dirichlet values of V)
+};
+OpK make_opk(const Grid &g, const OpParams &op);
 
 enum { OP_RESID = 0, OP_APPLY = 1, OP_VA = 2 };
 

@@ -60,6 +60,7 @@ struct qms_handle {
     double *partial = nullptr, *dsmall = nullptr;
     double *hsmall = nullptr;   // pinned
     int nblk = 1;
+    int tile_x = 64;            // operator tile width in nodes
 
     // preconditioner work vectors (4*n1 in MG precision)
     void *rT = nullptr, *zT = nullptr, *tT = nullptr;
@@ -136,12 +137,13 @@ struct qms_handle {
 
 namespace {
 
-OpParams make_op(const double v[3], double dt, int tr) {
+OpParams make_op(const qms_handle *h, const double v[3], double dt, int tr) {
     OpParams op;
     op.v[0] = v[0]; op.v[1] = v[1]; op.v[2] = v[2];
     op.dt = dt;
     op.tr = tr;
     op.adv = (v[0] != 0.0 || v[1] != 0.0 || v[2] != 0.0) ? 1 : 0;
+    op.tile_x = h->tile_x;
     return op;
 }
 
@@ -401,8 +403,8 @@ int qms_create(const qms_config *cfg, const uint16_t *cell_slot, const double *s
     h->Dst = h->dalloc<double>(h->n);
     h->Dtr = h->dalloc<double>(h->n);
     const double v0[3] = {0, 0, 0};
-    launch_diag(g, h->c64, make_op(v0, 0.0, 0), h->Dst, h->s);
-    launch_diag(g, h->c64, make_op(v0, cfg->dt, cfg->dt > 0 ? 1 : 0), h->Dtr, h->s);
+    launch_diag(g, h->c64, make_op(h.get(), v0, 0.0, 0), h->Dst, h->s);
+    launch_diag(g, h->c64, make_op(h.get(), v0, cfg->dt, cfg->dt > 0 ? 1 : 0), h->Dtr, h->s);
     const int m = cfg->restart;
     for (int j = 0; j <= m; ++j) h->V.push_back(h->dalloc<double>(h->n));
     for (int j = 0; j < m; ++j) h->Z.push_back(h->dalloc<double>(h->n));
@@ -431,7 +433,7 @@ int qms_steady(qms_handle *h, const double v[3]) {
     if (!h || !v) return QMS_ERR_ARG;
     if (!solve_enabled(h)) return QMS_ERR_STATE;
     QMS_TRY(h)
-    const OpParams op = make_op(v, 0.0, 0);
+    const OpParams op = make_op(h, v, 0.0, 0);
     const double zero[3] = {0, 0, 0};
     (void)zero;
     launch_rhs(h->g, h->c64, h->bc, op, h->x, h->b, h->s);
@@ -449,7 +451,7 @@ int qms_step(qms_handle *h, const double v[3]) {
     if (!(h->cfg.dt > 0)) { h->err = "qms_step: dt must be positive"; return QMS_ERR_ARG; }
     if (!solve_enabled(h)) return QMS_ERR_STATE;
     QMS_TRY(h)
-    const OpParams op = make_op(v, h->cfg.dt, 1);
+    const OpParams op = make_op(h, v, h->cfg.dt, 1);
     launch_rhs(h->g, h->c64, h->bc, op, h->x, h->b, h->s);
     launch_setbc(h->g, h->bc, op, h->x, h->s);
     solve(h, op, 1);
@@ -500,7 +502,7 @@ int qms_apply(qms_handle *h, const double v[3], double dt, const double *x, doub
     if (!h || !v || !x || !y) return QMS_ERR_ARG;
     QMS_TRY(h)
     const int tr = dt > 0 ? 1 : 0;
-    const OpParams op = make_op(v, dt, tr);
+    const OpParams op = make_op(h, v, dt, tr);
     check(hipMemcpyAsync(h->r, x, sizeof(double) * h->n, hipMemcpyHostToDevice, h->s), "h2d");
     launch_apply(h->g, h->c64, op, h->r, h->w, h->s);
     launch_diag(h->g, h->c64, op, h->tmp, h->s);
@@ -516,7 +518,7 @@ int qms_apply(qms_handle *h, const double v[3], double dt, const double *x, doub
 int qms_rhs(qms_handle *h, const double v[3], double dt, const double *aprev, double *b) {
     if (!h || !v || !aprev || !b) return QMS_ERR_ARG;
     QMS_TRY(h)
-    const OpParams op = make_op(v, dt, dt > 0 ? 1 : 0);
+    const OpParams op = make_op(h, v, dt, dt > 0 ? 1 : 0);
     check(hipMemcpyAsync(h->r, aprev, sizeof(double) * h->n, hipMemcpyHostToDevice, h->s), "h2d");
     launch_rhs(h->g, h->c64, h->bc, op, h->r, h->w, h->s);
     check(hipMemcpyAsync(b, h->w, sizeof(double) * h->n, hipMemcpyDeviceToHost, h->s), "d2h");
@@ -532,13 +534,25 @@ int qms_get_stats(const qms_handle *h, qms_stats *st) {
     return QMS_OK;
 }
 
-int qms_time_kernel(qms_handle *h, int which, int reps, double *ms_per_launch, double *bytes_per_launch) {
-    if (!h || reps < 1 || !ms_per_launch || !bytes_per_launch || which < 0 || which > 3) return QMS_ERR_ARG;
+int qms_set_tile(qms_handle *h, int tile_x) {
+    if (!h) return QMS_ERR_ARG;
+    if (tile_x != 32 && tile_x != 64 && tile_x != 128 && tile_x != 256) {
+        h->err = "qms_set_tile: tile width must be 32, 64, 128 or 256";
+        return QMS_ERR_ARG;
+    }
+    h->tile_x = tile_x;
+    return QMS_OK;
+}
+
+int qms_time_kernel(qms_handle *h, int which, const double v[3], int reps, double *ms_per_launch,
+                    double *bytes_per_launch) {
+    if (!h || reps < 1 || !ms_per_launch || !bytes_per_launch || which < 0 || which > 6) return QMS_ERR_ARG;
     QMS_TRY(h)
     const int tr = h->cfg.dt > 0 ? 1 : 0;
     const double v0[3] = {0, 0, 0};
-    const OpParams op = make_op(v0, h->cfg.dt, tr);
+    const OpParams op = make_op(h, v ? v : v0, h->cfg.dt, tr);
     const int64_t n1 = h->g.n1, n = h->n;
+    const int64_t nc = (int64_t)h->cfg.cells[0] * h->cfg.cells[1] * h->cfg.cells[2];
     const bool mixed = h->cfg.precision == QMS_MIXED;
     const double s = mixed ? 4.0 : 8.0;
     if (which == 1 || which == 2) {
@@ -553,6 +567,12 @@ int qms_time_kernel(qms_handle *h, int which, int reps, double *ms_per_launch, d
             launch_apply(h->g, h->c64, op, h->x, h->w, h->s);
         } else if (which == 3) {
             launch_copy(h->w, h->x, n, h->s);
+        } else if (which == 4) {
+            launch_rhs(h->g, h->c64, h->bc, op, h->x, h->w, h->s);
+        } else if (which == 5) {
+            launch_hfield(h->g, h->c64, h->x, h->tmp, 0, h->s);
+        } else if (which == 6) {
+            launch_resid(h->g, h->c64, op, h->x, h->b, h->w, h->s);
         } else {
             const Fine f = h->fine(tr ? h->cfg.dt : 0.0);
             const MGParams mp = mgp(h);
@@ -576,10 +596,16 @@ int qms_time_kernel(qms_handle *h, int which, int reps, double *ms_per_launch, d
     hipEventDestroy(e1);
     *ms_per_launch = ms / reps;
     // Algorithmic bytes: each input and output array streamed once.
-    if (which == 0) *bytes_per_launch = 8.0 * (2.0 * (double)n + 6.0 * (double)n1);
-    else if (which == 3) *bytes_per_launch = 16.0 * (double)n;
-    else if (which == 1) *bytes_per_launch = s * (3.0 * 3.0 * (double)n1 + 6.0 * (double)n1);
-    else *bytes_per_launch = s * (3.0 * (double)n1 + 3.0 * (double)n1);
+    const double N = (double)n, N1 = (double)n1, NC = (double)nc;
+    switch (which) {
+    case 0: *bytes_per_launch = 8.0 * (2.0 * N + 6.0 * N1); break;           // x, y, 6 coefficient arrays
+    case 6: *bytes_per_launch = 8.0 * (3.0 * N + 6.0 * N1); break;           // x, b, r, 6 coefficient arrays
+    case 3: *bytes_per_launch = 16.0 * N; break;                             // read + write of the state
+    case 4: *bytes_per_launch = 8.0 * (3.0 * N1 + 3.0 * N1 + 4.0 * N1); break; // A^n, sigma, b
+    case 5: *bytes_per_launch = 8.0 * (6.0 * N1 + 3.0 * NC); break;          // A, 1/Mu, H
+    case 1: *bytes_per_launch = s * (3.0 * 3.0 * N1 + 6.0 * N1); break;
+    default: *bytes_per_launch = s * (3.0 * N1 + 3.0 * N1); break;
+    }
     return QMS_OK;
     QMS_CATCH(h)
 }

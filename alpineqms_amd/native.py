@@ -27,8 +27,11 @@ QMS_MIXED = 1
 EXPORTS = (
     "qms_config_default", "qms_create", "qms_steady", "qms_step", "qms_run",
     "qms_get_fields", "qms_set_fields", "qms_get_h", "qms_apply", "qms_rhs",
-    "qms_get_stats", "qms_time_kernel", "qms_destroy", "qms_last_error",
+    "qms_get_stats", "qms_set_tile", "qms_time_kernel", "qms_destroy", "qms_last_error",
 )
+
+# qms_time_kernel selectors
+K_APPLY, K_SMOOTH_A, K_SMOOTH_V, K_COPY, K_RHS, K_HFIELD, K_RESID = range(7)
 
 
 class QmsError(RuntimeError):
@@ -97,7 +100,8 @@ def load(path: Optional[str] = None):
     L.qms_apply.argtypes = [H, d, ctypes.c_double, d, d]
     L.qms_rhs.argtypes = [H, d, ctypes.c_double, d, d]
     L.qms_get_stats.argtypes = [H, ctypes.POINTER(Stats)]
-    L.qms_time_kernel.argtypes = [H, ctypes.c_int, ctypes.c_int, d, d]
+    L.qms_set_tile.argtypes = [H, ctypes.c_int]
+    L.qms_time_kernel.argtypes = [H, ctypes.c_int, d, ctypes.c_int, d, d]
     L.qms_destroy.argtypes = [H]
     L.qms_destroy.restype = None
     L.qms_last_error.argtypes = [H]
@@ -238,8 +242,17 @@ class Solver:
         self._check(self._L.qms_get_stats(self._h, ctypes.byref(s)))
         return s
 
-    def time_kernel(self, which: int, reps: int):
+    def set_tile(self, tile_x: int):
+        self._check(self._L.qms_set_tile(self._h, int(tile_x)))
+
+    def time_kernel(self, which: int, reps: int, v=None):
+        """Mean milliseconds and algorithmic bytes per launch of one kernel
+        (selectors K_*), timed with HIP events on the handle's stream."""
         ms = ctypes.c_double()
         by = ctypes.c_double()
-        self._check(self._L.qms_time_kernel(self._h, which, reps, ctypes.byref(ms), ctypes.byref(by)))
+        vp = None
+        if v is not None:
+            vv = np.ascontiguousarray(v, dtype=np.float64)
+            vp = _dp(vv)
+        self._check(self._L.qms_time_kernel(self._h, which, vp, reps, ctypes.byref(ms), ctypes.byref(by)))
         return ms.value, by.value

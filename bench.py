@@ -11,14 +11,16 @@ which is disabled in this build (it faulted the GPU during its first
 hierarchy build, cause not identified), so `value` is null and says so.
 What is measured on the GPU: the fused A-V operator kernel at 128^3 (the
 matrix-free replacement of the reference's per-step CSR assembly and the
-operator inside every Krylov iteration) against the HBM roofline, with HIP
-events on the stream it is launched on, and a plain copy kernel of the same
-state for calibration.  The CPU baseline is the oracle (CSR assembly +
+operator inside every Krylov iteration) with the rows of the last ramp step
+(transient, v = 5 m/s), against the HBM roofline, with HIP events on the
+stream it is launched on; its tile width is picked from a sweep over
+32..256 nodes.  Beside it: the FP64 residual, the step right-hand side, the
+cell-centred H and a plain copy kernel of the same state (calibration).  The CPU baseline is the oracle (CSR assembly +
 equilibrated SuperLU factor/solve, the restated reference algorithm) timed
 on one core over one ramp step of a 24^3-node design drawn from the same
 box distribution.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--tile T] [--no-cpu-baseline]
 """
 import argparse
 import json
@@ -54,12 +56,35 @@ def cpu_baseline():
             "dof_steps_per_s": d.dof / t}
 
 
+TILES = (32, 64, 128, 256)
+
+
+def traffic_from_profile(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by what the
+    counter reads for the plain copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}.txt")
+    if not os.path.exists(path):
+        return None, None
+    kv = {}
+    for line in open(path):
+        if ":" in line and not line.startswith("#"):
+            k, v = line.split(":", 1)
+            kv[k.strip()] = v.strip()
+    try:
+        t = float(kv["fetch_kib"]) * float(kv["fetch_scale"]) + float(kv["write_kib"]) * float(kv["write_scale"])
+    except (KeyError, ValueError):
+        return None, None
+    return t * 1024.0, os.path.relpath(path, ROOT)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--nodes", type=int, default=128)
+    ap.add_argument("--tile", type=int, default=0, help="operator tile width (0: pick the fastest of 32..256)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -67,7 +92,7 @@ def main():
               "path (whole designs per GPU, no inter-GPU traffic) is not part of this build", file=sys.stderr)
         return 2
     import numpy as np
-    from alpineqms_amd import configs, native
+    from alpineqms_amd import configs, native, program
     from alpineqms_amd.qms_io import material_slots
 
     d = configs.config3(N=a.nodes, ramp=True)
@@ -78,17 +103,38 @@ def main():
     nx, ny, nz = d.nodes
     rng = np.random.default_rng(0)
     s.set_fields(rng.normal(size=(4, nz, ny, nx)))
-    s.time_kernel(0, max(a.warmup, 1))
-    ms, nbytes = s.time_kernel(0, a.steps)
-    s.time_kernel(3, max(a.warmup, 1))
-    ms_copy, bytes_copy = s.time_kernel(3, a.steps)
+    # rows of the last ramp step: transient, frame velocity 5 m/s (advection terms on)
+    v = program.step_velocity(inp.traj, inp.dt, inp.steps)
+    w = max(a.warmup, 1)
+
+    def timed(which):
+        s.time_kernel(which, w, v=v)
+        return s.time_kernel(which, a.steps, v=v)
+
+    sweep = {}
+    tiles = (a.tile,) if a.tile else TILES
+    for t in tiles:
+        s.set_tile(t)
+        sweep[t] = timed(native.K_APPLY)[0]
+        print(f"bench: operator tile {t}: {sweep[t] * 1e3:.1f} us", file=sys.stderr, flush=True)
+    tile = min(sweep, key=sweep.get)
+    s.set_tile(tile)
+    ms, nbytes = timed(native.K_APPLY)
     achieved = nbytes / (ms * 1e-3) / 1e9
+    others = {}
+    for name, which in (("k_operator_resid", native.K_RESID), ("k_rhs", native.K_RHS),
+                        ("k_hfield", native.K_HFIELD), ("k_copy", native.K_COPY)):
+        mk, bk = timed(which)
+        gbs = bk / (mk * 1e-3) / 1e9
+        others[name] = {"ms_per_launch": mk, "algorithmic_bytes": bk, "achieved_GBs": gbs,
+                        "frac": gbs / HBM_PEAK_GBS}
     status = None
     try:
         s.steady(np.zeros(3))
     except native.QmsError as e:
         status = str(e)
     s.close()
+    traffic, tsrc = traffic_from_profile("k_operator_apply_config3_128")
     line = {
         "metric": "A-V time steps/s at 128^3, mixed vs FP64 @1e-8 rel err; served p50/p99 latency",
         "value": None,
@@ -102,16 +148,16 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"config 3: {nx}x{ny}x{nz} nodes, 0.25 mm, 24 boxes + slab, v ramp 0->5 m/s",
+        "config": {"workload": f"config 3: {nx}x{ny}x{nz} nodes, 0.25 mm, 24 boxes + slab, v ramp 0->5 m/s "
+                               f"(rows of the last ramp step: transient, v = {v[0]:g} m/s)",
                    "dof": d.dof},
         "status": "time-step rate not measured: " + (status or "solve path unavailable"),
-        "roofline": {"kernel": "k_operator (fused A-V operator apply, FP64)", "bound": "hbm",
+        "roofline": {"kernel": f"k_operator (fused A-V operator apply, FP64, tile {tile})", "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "algorithmic_bytes": nbytes, "ms_per_launch": ms,
-                     "launches": a.steps},
-        "copy_reference": {"achieved": bytes_copy / (ms_copy * 1e-3) / 1e9, "unit": "GB/s",
-                           "bytes": bytes_copy, "ms_per_launch": ms_copy},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": tsrc,
+                     "algorithmic_bytes": nbytes, "ms_per_launch": ms, "launches": a.steps},
+        "tile_sweep_ms": {str(t): m for t, m in sweep.items()},
+        "kernels": others,
         "cpu_baseline": None if a.no_cpu_baseline else cpu_baseline(),
     }
     print(json.dumps(line), flush=True)

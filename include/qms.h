@@ -67,7 +67,7 @@ typedef struct {
     int n_levels_a, n_levels_v;
 } qms_stats;
 
-/* Fills defaults: FP64, tol 1e-12, restart 30, 400 iterations, nu 4,
+/* Fills defaults: FP64, tol 1e-12, restart 30, 3000 iterations, nu 4,
  * omega 0.7, alpha 1.8. */
 int qms_config_default(qms_config *cfg);
 
@@ -113,12 +113,22 @@ int qms_rhs(qms_handle *h, const double v[3], double dt, const double *aprev, do
 
 int qms_get_stats(const qms_handle *h, qms_stats *st);
 
+/* Operator tile width in nodes (32, 64, 128 or 256; default 64): each
+ * 256-thread workgroup of the operator and RHS kernels covers tile_x x
+ * (256/tile_x) nodes of one z-plane.  A tuning knob with no counterpart in
+ * the reference (whose assembly loop is serial, QMS_3D.f90:448-1293). */
+int qms_set_tile(qms_handle *h, int tile_x);
+
 /* Times `reps` launches of one kernel on the handle's stream with HIP events
  * and reports the mean duration and the algorithmic HBM bytes per launch.
+ * v: frame velocity of the rows (NULL: 0); rows are transient when dt > 0.
  * which: 0 fused operator apply (FP64), 1 level-0 A-block smoother sweep,
  * 2 level-0 V-block smoother sweep (precision of the handle's multigrid),
- * 3 plain copy of the 4-field state (calibration of byte counters). */
-int qms_time_kernel(qms_handle *h, int which, int reps, double *ms_per_launch, double *bytes_per_launch);
+ * 3 plain copy of the 4-field state (calibration of byte counters),
+ * 4 step right-hand side, 5 cell-centred H, 6 FP64 residual b - M x.
+ * No reference counterpart (measurement only). */
+int qms_time_kernel(qms_handle *h, int which, const double v[3], int reps, double *ms_per_launch,
+                    double *bytes_per_launch);
 
 void qms_destroy(qms_handle *h);
 
