@@ -27,7 +27,7 @@ struct OpParams {
     double dt;     // del_t (transient rows only)
     int tr;        // 1: transient rows T(v,dt), 0: steady rows S(v)
     int adv;       // 1 if any velocity component is non-zero
-    int tile_x;    // operator tile width in nodes (32, 64, 128 or 256; 256 threads per tile)
+    int zchunk;    // planes per operator workgroup (z-march depth)
 };
 
 // Per-launch constants of the operator rows, computed once on the host so
@@ -37,12 +37,12 @@ struct OpK {
     int sy, sz;                  // node strides along y and z (nx, nx*ny)
     int tr, adv;
     double ih2[3];               // 1/h_a^2
-    double ihh[3][3];            // 1/(h_a h_b)
+    double ixy, ixz, iyz;        // 1/(h_a h_b)
     double ih[3];                // 1/h_a
     double idt;                  // 1/dt (transient rows)
     double iht[3];               // 1/(h_a dt)
     double v1[3][3];             // v_a / (2 h_b)
-    double v2[3][3][3];          // v_a / (2 h_b h_c)
+    double w[12];                // v_a / (2 h_b h_c) for the (a,b,c) of the V row's advection terms
     double v[3];                 // frame velocity (Dirichlet values of V)
 };
 OpK make_opk(const Grid &g, const OpParams &op);

@@ -1,13 +1,16 @@
-// Fused matrix-free operator of the coupled A-V system on the Yee grid.
+// Fused matrix-free operator of the coupled A-V system on the Yee grid, and
+// the per-step kernels around it (coefficients, diagonal, RHS, Dirichlet
+// values, cell-centred H).
 //
-// One thread owns one node p = (i,j,k) and evaluates the four rows (Ax, Ay,
-// Az, V) that live there, exactly the rows the reference assembles into CSR
-// every step (steady QMS_3D.f90:448-1293, transient :1457-2189).  Arithmetic
-// is FP64 whatever the storage type of the input vector: the V rows mix
-// conductivities 1e-3..6e7 S/m, and only FP64 evaluation keeps the weak
-// background couplings that pin each conductor's floating potential.
-// Self couplings are evaluated in difference form c*(x_p - x_q) so that a
-// constant on a conductor produces its true (tiny) residual.
+// In the operator one thread owns one node p = (i,j,k) and evaluates the
+// four rows (Ax, Ay, Az, V) that live there, exactly the rows the reference
+// assembles into CSR every step (steady QMS_3D.f90:448-1293, transient
+// :1457-2189).  Arithmetic is FP64 whatever the storage type of the input
+// vector: the V rows mix conductivities 1e-3..6e7 S/m, and only FP64
+// evaluation keeps the weak background couplings that pin each conductor's
+// floating potential.  Self couplings are evaluated in difference form
+// c*(x_p - x_q) so that a constant on a conductor produces its true (tiny)
+// residual.
 #include <hip/hip_runtime.h>
 #include "qms_kernels.h"
 
@@ -61,55 +64,60 @@ __global__ void k_coeffs(Grid g, const uint16_t *__restrict__ slot, const double
 template <typename T>
 struct View {
     const T *ax, *ay, *az, *v;
-    __device__ __forceinline__ double AX(int o) const { return (double)ax[o]; }
-    __device__ __forceinline__ double AY(int o) const { return (double)ay[o]; }
-    __device__ __forceinline__ double AZ(int o) const { return (double)az[o]; }
-    __device__ __forceinline__ double VV(int o) const { return (double)v[o]; }
+    int sy, sz;
+    __device__ __forceinline__ int off(int di, int dj, int dk) const { return di + dj * sy + dk * sz; }
+    __device__ __forceinline__ double AX(int di, int dj, int dk) const { return (double)ax[off(di, dj, dk)]; }
+    __device__ __forceinline__ double AY(int di, int dj, int dk) const { return (double)ay[off(di, dj, dk)]; }
+    __device__ __forceinline__ double AZ(int di, int dj, int dk) const { return (double)az[off(di, dj, dk)]; }
+    __device__ __forceinline__ double VV(int di, int dj, int dk) const { return (double)v[off(di, dj, dk)]; }
 };
 
 struct CView {
-    const double *imx, *imy, *imz, *sx, *sy, *sz;
-    __device__ __forceinline__ double IMX(int o) const { return imx[o]; }
-    __device__ __forceinline__ double IMY(int o) const { return imy[o]; }
-    __device__ __forceinline__ double IMZ(int o) const { return imz[o]; }
-    __device__ __forceinline__ double SX(int o) const { return sx[o]; }
-    __device__ __forceinline__ double SY(int o) const { return sy[o]; }
-    __device__ __forceinline__ double SZ(int o) const { return sz[o]; }
+    const double *imx, *imy, *imz, *sx, *sy_, *sz_;
+    int sy, sz;
+    __device__ __forceinline__ int off(int di, int dj, int dk) const { return di + dj * sy + dk * sz; }
+    __device__ __forceinline__ double IMX(int di, int dj, int dk) const { return imx[off(di, dj, dk)]; }
+    __device__ __forceinline__ double IMY(int di, int dj, int dk) const { return imy[off(di, dj, dk)]; }
+    __device__ __forceinline__ double IMZ(int di, int dj, int dk) const { return imz[off(di, dj, dk)]; }
+    __device__ __forceinline__ double SX(int di, int dj, int dk) const { return sx[off(di, dj, dk)]; }
+    __device__ __forceinline__ double SY(int di, int dj, int dk) const { return sy_[off(di, dj, dk)]; }
+    __device__ __forceinline__ double SZ(int di, int dj, int dk) const { return sz_[off(di, dj, dk)]; }
 };
 
+// Row-uniform views: one wave covers 64 consecutive nodes of one grid row.
+// The field pointers are wave-uniform; each lane adds its own byte offset.
 template <typename T>
-__device__ __forceinline__ View<T> view_at(const T *x, int64_t n1, int64_t p) {
-    return View<T>{x + p, x + n1 + p, x + 2 * n1 + p, x + 3 * n1 + p};
+__device__ __forceinline__ View<T> view_at(const T *x, int64_t n1, int64_t p, const OpK &k) {
+    return View<T>{x + p, x + n1 + p, x + 2 * n1 + p, x + 3 * n1 + p, k.sy, k.sz};
 }
-__device__ __forceinline__ CView cview_at(const CoeffsD &c, int64_t p) {
-    return CView{c.im[0] + p, c.im[1] + p, c.im[2] + p, c.sg[0] + p, c.sg[1] + p, c.sg[2] + p};
+__device__ __forceinline__ CView cview_at(const CoeffsD &c, int64_t p, const OpK &k) {
+    return CView{c.im[0] + p, c.im[1] + p, c.im[2] + p, c.sg[0] + p, c.sg[1] + p, c.sg[2] + p, k.sy, k.sz};
 }
 
 // Which blocks of the row to include.
 enum : int { B_AA = 1, B_AV = 2, B_VA = 4, B_VV = 8, B_ALL = 15 };
 
 // Ax row, QMS_3D.f90:476-607 (+ :1504, :1475)
-template <typename T>
-__device__ __forceinline__ double row_ax(const View<T> &x, const CView &c, const OpK &k, int blocks, double &diag) {
-    const int X = 1, Y = k.sy, Z = k.sz;
-    const double s = c.SX(0);
-    const double imy_km = c.IMY(-Z), imy_k = c.IMY(0);
-    const double imz_jm = c.IMZ(-Y), imz_j = c.IMZ(0);
-    const double imx_im = c.IMX(-X), imx_i = c.IMX(0);
+template <typename XV, typename CV>
+__device__ __forceinline__ double row_ax(const XV &x, const CV &c, const OpK &k, int blocks, double &diag) {
+    const double s = c.SX(0, 0, 0);
+    const double imy_km = c.IMY(0, 0, -1), imy_k = c.IMY(0, 0, 0);
+    const double imz_jm = c.IMZ(0, -1, 0), imz_j = c.IMZ(0, 0, 0);
+    const double imx_im = c.IMX(-1, 0, 0), imx_i = c.IMX(0, 0, 0);
     const double ihx2 = k.ih2[0], ihy2 = k.ih2[1], ihz2 = k.ih2[2];
     double d = (imz_j + imz_jm) * ihy2 + (imy_k + imy_km) * ihz2 + (imx_i + imx_im) * ihx2;
     if (k.tr) d = d + s * k.idt;
     diag = d;
     double y = 0.0;
     if (blocks & B_AA) {
-        const double x0 = x.AX(0);
-        const double xzm = x.AX(-Z), xym = x.AX(-Y), xxm = x.AX(-X), xxp = x.AX(X), xyp = x.AX(Y), xzp = x.AX(Z);
+        const double x0 = x.AX(0, 0, 0);
+        const double xzm = x.AX(0, 0, -1), xym = x.AX(0, -1, 0), xxm = x.AX(-1, 0, 0), xxp = x.AX(1, 0, 0), xyp = x.AX(0, 1, 0), xzp = x.AX(0, 0, 1);
         y += imy_km * ihz2 * (x0 - xzm) + imz_jm * ihy2 * (x0 - xym) + imx_im * ihx2 * (x0 - xxm)
            + imx_i * ihx2 * (x0 - xxp) + imz_j * ihy2 * (x0 - xyp) + imy_k * ihz2 * (x0 - xzp);
         if (k.tr) y += (s * k.idt) * x0;
-        const double ay_m0 = x.AY(-Y), ay_p0 = x.AY(X - Y), ay_00 = x.AY(0), ay_10 = x.AY(X);
-        const double az_m0 = x.AZ(-Z), az_p0 = x.AZ(X - Z), az_00 = x.AZ(0), az_10 = x.AZ(X);
-        const double ixy = k.ihh[0][1], ixz = k.ihh[0][2];
+        const double ay_m0 = x.AY(0, -1, 0), ay_p0 = x.AY(1, -1, 0), ay_00 = x.AY(0, 0, 0), ay_10 = x.AY(1, 0, 0);
+        const double az_m0 = x.AZ(0, 0, -1), az_p0 = x.AZ(1, 0, -1), az_00 = x.AZ(0, 0, 0), az_10 = x.AZ(1, 0, 0);
+        const double ixy = k.ixy, ixz = k.ixz;
         y += imz_jm * ixy * (ay_m0 - ay_p0) + imz_j * ixy * (ay_10 - ay_00);
         y += imy_km * ixz * (az_m0 - az_p0) + imy_k * ixz * (az_10 - az_00);
         if (k.adv) {
@@ -118,32 +126,31 @@ __device__ __forceinline__ double row_ax(const View<T> &x, const CView &c, const
             y += s * k.v1[2][0] * (-az_m0 + az_p0 - az_00 + az_10);
         }
     }
-    if (blocks & B_AV) y += s * k.ih[0] * (x.VV(X) - x.VV(0));
+    if (blocks & B_AV) y += s * k.ih[0] * (x.VV(1, 0, 0) - x.VV(0, 0, 0));
     return y;
 }
 
 // Ay row, QMS_3D.f90:650-781 (+ :1681, :1624)
-template <typename T>
-__device__ __forceinline__ double row_ay(const View<T> &x, const CView &c, const OpK &k, int blocks, double &diag) {
-    const int X = 1, Y = k.sy, Z = k.sz;
-    const double s = c.SY(0);
-    const double imx_km = c.IMX(-Z), imx_k = c.IMX(0);
-    const double imy_jm = c.IMY(-Y), imy_j = c.IMY(0);
-    const double imz_im = c.IMZ(-X), imz_i = c.IMZ(0);
+template <typename XV, typename CV>
+__device__ __forceinline__ double row_ay(const XV &x, const CV &c, const OpK &k, int blocks, double &diag) {
+    const double s = c.SY(0, 0, 0);
+    const double imx_km = c.IMX(0, 0, -1), imx_k = c.IMX(0, 0, 0);
+    const double imy_jm = c.IMY(0, -1, 0), imy_j = c.IMY(0, 0, 0);
+    const double imz_im = c.IMZ(-1, 0, 0), imz_i = c.IMZ(0, 0, 0);
     const double ihx2 = k.ih2[0], ihy2 = k.ih2[1], ihz2 = k.ih2[2];
     double d = (imz_i + imz_im) * ihx2 + (imx_k + imx_km) * ihz2 + (imy_j + imy_jm) * ihy2;
     if (k.tr) d = d + s * k.idt;
     diag = d;
     double y = 0.0;
     if (blocks & B_AA) {
-        const double x0 = x.AY(0);
-        const double xzm = x.AY(-Z), xym = x.AY(-Y), xxm = x.AY(-X), xxp = x.AY(X), xyp = x.AY(Y), xzp = x.AY(Z);
+        const double x0 = x.AY(0, 0, 0);
+        const double xzm = x.AY(0, 0, -1), xym = x.AY(0, -1, 0), xxm = x.AY(-1, 0, 0), xxp = x.AY(1, 0, 0), xyp = x.AY(0, 1, 0), xzp = x.AY(0, 0, 1);
         y += imx_km * ihz2 * (x0 - xzm) + imy_jm * ihy2 * (x0 - xym) + imz_im * ihx2 * (x0 - xxm)
            + imz_i * ihx2 * (x0 - xxp) + imy_j * ihy2 * (x0 - xyp) + imx_k * ihz2 * (x0 - xzp);
         if (k.tr) y += (s * k.idt) * x0;
-        const double ax_m0 = x.AX(-X), ax_00 = x.AX(0), ax_m1 = x.AX(Y - X), ax_01 = x.AX(Y);
-        const double az_m0 = x.AZ(-Z), az_m1 = x.AZ(Y - Z), az_00 = x.AZ(0), az_01 = x.AZ(Y);
-        const double ixy = k.ihh[0][1], iyz = k.ihh[1][2];
+        const double ax_m0 = x.AX(-1, 0, 0), ax_00 = x.AX(0, 0, 0), ax_m1 = x.AX(-1, 1, 0), ax_01 = x.AX(0, 1, 0);
+        const double az_m0 = x.AZ(0, 0, -1), az_m1 = x.AZ(0, 1, -1), az_00 = x.AZ(0, 0, 0), az_01 = x.AZ(0, 1, 0);
+        const double ixy = k.ixy, iyz = k.iyz;
         y += imz_im * ixy * (ax_m0 - ax_m1) + imz_i * ixy * (ax_01 - ax_00);
         y += imx_km * iyz * (az_m0 - az_m1) + imx_k * iyz * (az_01 - az_00);
         if (k.adv) {
@@ -152,32 +159,31 @@ __device__ __forceinline__ double row_ay(const View<T> &x, const CView &c, const
             y += s * k.v1[2][1] * (-az_m0 + az_m1 - az_00 + az_01);
         }
     }
-    if (blocks & B_AV) y += s * k.ih[1] * (x.VV(Y) - x.VV(0));
+    if (blocks & B_AV) y += s * k.ih[1] * (x.VV(0, 1, 0) - x.VV(0, 0, 0));
     return y;
 }
 
 // Az row, QMS_3D.f90:824-955 (+ :1857, :1772)
-template <typename T>
-__device__ __forceinline__ double row_az(const View<T> &x, const CView &c, const OpK &k, int blocks, double &diag) {
-    const int X = 1, Y = k.sy, Z = k.sz;
-    const double s = c.SZ(0);
-    const double imz_km = c.IMZ(-Z), imz_k = c.IMZ(0);
-    const double imx_jm = c.IMX(-Y), imx_j = c.IMX(0);
-    const double imy_im = c.IMY(-X), imy_i = c.IMY(0);
+template <typename XV, typename CV>
+__device__ __forceinline__ double row_az(const XV &x, const CV &c, const OpK &k, int blocks, double &diag) {
+    const double s = c.SZ(0, 0, 0);
+    const double imz_km = c.IMZ(0, 0, -1), imz_k = c.IMZ(0, 0, 0);
+    const double imx_jm = c.IMX(0, -1, 0), imx_j = c.IMX(0, 0, 0);
+    const double imy_im = c.IMY(-1, 0, 0), imy_i = c.IMY(0, 0, 0);
     const double ihx2 = k.ih2[0], ihy2 = k.ih2[1], ihz2 = k.ih2[2];
     double d = (imy_i + imy_im) * ihx2 + (imx_j + imx_jm) * ihy2 + (imz_k + imz_km) * ihz2;
     if (k.tr) d = d + s * k.idt;
     diag = d;
     double y = 0.0;
     if (blocks & B_AA) {
-        const double x0 = x.AZ(0);
-        const double xzm = x.AZ(-Z), xym = x.AZ(-Y), xxm = x.AZ(-X), xxp = x.AZ(X), xyp = x.AZ(Y), xzp = x.AZ(Z);
+        const double x0 = x.AZ(0, 0, 0);
+        const double xzm = x.AZ(0, 0, -1), xym = x.AZ(0, -1, 0), xxm = x.AZ(-1, 0, 0), xxp = x.AZ(1, 0, 0), xyp = x.AZ(0, 1, 0), xzp = x.AZ(0, 0, 1);
         y += imz_km * ihz2 * (x0 - xzm) + imx_jm * ihy2 * (x0 - xym) + imy_im * ihx2 * (x0 - xxm)
            + imy_i * ihx2 * (x0 - xxp) + imx_j * ihy2 * (x0 - xyp) + imz_k * ihz2 * (x0 - xzp);
         if (k.tr) y += (s * k.idt) * x0;
-        const double ax_m0 = x.AX(-X), ax_00 = x.AX(0), ax_m1 = x.AX(Z - X), ax_01 = x.AX(Z);
-        const double ay_m0 = x.AY(-Y), ay_00 = x.AY(0), ay_m1 = x.AY(Z - Y), ay_01 = x.AY(Z);
-        const double ixz = k.ihh[0][2], iyz = k.ihh[1][2];
+        const double ax_m0 = x.AX(-1, 0, 0), ax_00 = x.AX(0, 0, 0), ax_m1 = x.AX(-1, 0, 1), ax_01 = x.AX(0, 0, 1);
+        const double ay_m0 = x.AY(0, -1, 0), ay_00 = x.AY(0, 0, 0), ay_m1 = x.AY(0, -1, 1), ay_01 = x.AY(0, 0, 1);
+        const double ixz = k.ixz, iyz = k.iyz;
         y += imy_im * ixz * (ax_m0 - ax_m1) + imy_i * ixz * (ax_01 - ax_00);
         y += imx_jm * iyz * (ay_m0 - ay_m1) + imx_j * iyz * (ay_01 - ay_00);
         if (k.adv) {
@@ -186,72 +192,69 @@ __device__ __forceinline__ double row_az(const View<T> &x, const CView &c, const
             y += s * k.v1[1][2] * (-ay_m0 - ay_00 + ay_m1 + ay_01);
         }
     }
-    if (blocks & B_AV) y += s * k.ih[2] * (x.VV(Z) - x.VV(0));
+    if (blocks & B_AV) y += s * k.ih[2] * (x.VV(0, 0, 1) - x.VV(0, 0, 0));
     return y;
 }
 
 // V row, QMS_3D.f90:1003-1285 (+ :1975-2126 transient terms)
-template <typename T>
-__device__ __forceinline__ double row_v(const View<T> &x, const CView &c, const OpK &k, int blocks, double &diag) {
-    const int X = 1, Y = k.sy, Z = k.sz;
+template <typename XV, typename CV>
+__device__ __forceinline__ double row_v(const XV &x, const CV &c, const OpK &k, int blocks, double &diag) {
     const double ihx2 = k.ih2[0], ihy2 = k.ih2[1], ihz2 = k.ih2[2];
-    const double x0 = c.SX(0), x1 = c.SX(-X);
-    const double y0 = c.SY(0), y1 = c.SY(-Y);
-    const double z0 = c.SZ(0), z1 = c.SZ(-Z);
+    const double x0 = c.SX(0, 0, 0), x1 = c.SX(-1, 0, 0);
+    const double y0 = c.SY(0, 0, 0), y1 = c.SY(0, -1, 0);
+    const double z0 = c.SZ(0, 0, 0), z1 = c.SZ(0, 0, -1);
     const double fx0 = fmax(x0, QMS_SIGFLOOR), fx1 = fmax(x1, QMS_SIGFLOOR);
     const double fy0 = fmax(y0, QMS_SIGFLOOR), fy1 = fmax(y1, QMS_SIGFLOOR);
     const double fz0 = fmax(z0, QMS_SIGFLOOR), fz1 = fmax(z1, QMS_SIGFLOOR);
     diag = (fx0 + fx1) * ihx2 + (fy0 + fy1) * ihy2 + (fz0 + fz1) * ihz2;
     double y = 0.0;
     if (blocks & B_VV) {
-        const double v0 = x.VV(0);
-        y += fz1 * ihz2 * (v0 - x.VV(-Z)) + fy1 * ihy2 * (v0 - x.VV(-Y)) + fx1 * ihx2 * (v0 - x.VV(-X))
-           + fx0 * ihx2 * (v0 - x.VV(X)) + fy0 * ihy2 * (v0 - x.VV(Y)) + fz0 * ihz2 * (v0 - x.VV(Z));
+        const double v0 = x.VV(0, 0, 0);
+        y += fz1 * ihz2 * (v0 - x.VV(0, 0, -1)) + fy1 * ihy2 * (v0 - x.VV(0, -1, 0)) + fx1 * ihx2 * (v0 - x.VV(-1, 0, 0))
+           + fx0 * ihx2 * (v0 - x.VV(1, 0, 0)) + fy0 * ihy2 * (v0 - x.VV(0, 1, 0)) + fz0 * ihz2 * (v0 - x.VV(0, 0, 1));
     }
     if (blocks & B_VA) {
         if (k.tr) {
-            y += x0 * k.iht[0] * x.AX(0) - x1 * k.iht[0] * x.AX(-X)
-               + y0 * k.iht[1] * x.AY(0) - y1 * k.iht[1] * x.AY(-Y)
-               + z0 * k.iht[2] * x.AZ(0) - z1 * k.iht[2] * x.AZ(-Z);
+            y += x0 * k.iht[0] * x.AX(0, 0, 0) - x1 * k.iht[0] * x.AX(-1, 0, 0)
+               + y0 * k.iht[1] * x.AY(0, 0, 0) - y1 * k.iht[1] * x.AY(0, -1, 0)
+               + z0 * k.iht[2] * x.AZ(0, 0, 0) - z1 * k.iht[2] * x.AZ(0, 0, -1);
         }
         if (k.adv) {
-            // W(a,b,c) = v_a / (2 h_b h_c); axes x=0, y=1, z=2
-#define W(a, b, cc) k.v2[a][b][cc]
+            // k.w holds the twelve v_a / (2 h_b h_c) of this row (axes x=0, y=1, z=2)
             double w = 0.0;
             // Ax columns
-            w += (x1 * W(2, 2, 0) - z1 * W(0, 2, 2)) * x.AX(-X - Z);
-            w += (-x0 * W(2, 2, 0) - z1 * W(0, 2, 2)) * x.AX(-Z);
-            w += (x1 * W(1, 1, 0) - y1 * W(0, 1, 1)) * x.AX(-X - Y);
-            w += (-x0 * W(1, 1, 0) - y1 * W(0, 1, 1)) * x.AX(-Y);
-            const double axc = (y0 + y1) * W(0, 1, 1) + (z0 + z1) * W(0, 2, 2);
-            w += axc * (x.AX(-X) + x.AX(0));
-            w += (-x1 * W(1, 1, 0) - y0 * W(0, 1, 1)) * x.AX(Y - X);
-            w += (x0 * W(1, 1, 0) - y0 * W(0, 1, 1)) * x.AX(Y);
-            w += (-x1 * W(2, 2, 0) - z0 * W(0, 2, 2)) * x.AX(Z - X);
-            w += (x0 * W(2, 2, 0) - z0 * W(0, 2, 2)) * x.AX(Z);
+            w += (x1 * k.w[0] - z1 * k.w[1]) * x.AX(-1, 0, -1);
+            w += (-x0 * k.w[0] - z1 * k.w[1]) * x.AX(0, 0, -1);
+            w += (x1 * k.w[2] - y1 * k.w[3]) * x.AX(-1, -1, 0);
+            w += (-x0 * k.w[2] - y1 * k.w[3]) * x.AX(0, -1, 0);
+            const double axc = (y0 + y1) * k.w[3] + (z0 + z1) * k.w[1];
+            w += axc * (x.AX(-1, 0, 0) + x.AX(0, 0, 0));
+            w += (-x1 * k.w[2] - y0 * k.w[3]) * x.AX(-1, 1, 0);
+            w += (x0 * k.w[2] - y0 * k.w[3]) * x.AX(0, 1, 0);
+            w += (-x1 * k.w[0] - z0 * k.w[1]) * x.AX(-1, 0, 1);
+            w += (x0 * k.w[0] - z0 * k.w[1]) * x.AX(0, 0, 1);
             // Ay columns
-            w += (y1 * W(2, 2, 1) - z1 * W(1, 2, 2)) * x.AY(-Y - Z);
-            w += (-y0 * W(2, 2, 1) - z1 * W(1, 2, 2)) * x.AY(-Z);
-            w += (-x1 * W(1, 0, 0) + y1 * W(0, 0, 1)) * x.AY(-X - Y);
-            const double ayc = (x0 + x1) * W(1, 0, 0) + (z0 + z1) * W(1, 2, 2);
-            w += ayc * (x.AY(-Y) + x.AY(0));
-            w += (-x0 * W(1, 0, 0) - y1 * W(0, 0, 1)) * x.AY(X - Y);
-            w += (-x1 * W(1, 0, 0) - y0 * W(0, 0, 1)) * x.AY(-X);
-            w += (-x0 * W(1, 0, 0) + y0 * W(0, 0, 1)) * x.AY(X);
-            w += (-y1 * W(2, 2, 1) - z0 * W(1, 2, 2)) * x.AY(Z - Y);
-            w += (y0 * W(2, 2, 1) - z0 * W(1, 2, 2)) * x.AY(Z);
+            w += (y1 * k.w[4] - z1 * k.w[5]) * x.AY(0, -1, -1);
+            w += (-y0 * k.w[4] - z1 * k.w[5]) * x.AY(0, 0, -1);
+            w += (-x1 * k.w[6] + y1 * k.w[7]) * x.AY(-1, -1, 0);
+            const double ayc = (x0 + x1) * k.w[6] + (z0 + z1) * k.w[5];
+            w += ayc * (x.AY(0, -1, 0) + x.AY(0, 0, 0));
+            w += (-x0 * k.w[6] - y1 * k.w[7]) * x.AY(1, -1, 0);
+            w += (-x1 * k.w[6] - y0 * k.w[7]) * x.AY(-1, 0, 0);
+            w += (-x0 * k.w[6] + y0 * k.w[7]) * x.AY(1, 0, 0);
+            w += (-y1 * k.w[4] - z0 * k.w[5]) * x.AY(0, -1, 1);
+            w += (y0 * k.w[4] - z0 * k.w[5]) * x.AY(0, 0, 1);
             // Az columns
-            w += (-y1 * W(2, 1, 1) + z1 * W(1, 1, 2)) * x.AZ(-Y - Z);
-            w += (-x1 * W(2, 0, 0) + z1 * W(0, 0, 2)) * x.AZ(-X - Z);
-            const double azc = (x0 + x1) * W(2, 0, 0) + (y0 + y1) * W(2, 1, 1);
-            w += azc * (x.AZ(-Z) + x.AZ(0));
-            w += (-x0 * W(2, 0, 0) - z1 * W(0, 0, 2)) * x.AZ(X - Z);
-            w += (-y0 * W(2, 1, 1) - z1 * W(1, 1, 2)) * x.AZ(Y - Z);
-            w += (-y1 * W(2, 1, 1) - z0 * W(1, 1, 2)) * x.AZ(-Y);
-            w += (-x1 * W(2, 0, 0) - z0 * W(0, 0, 2)) * x.AZ(-X);
-            w += (-x0 * W(2, 0, 0) + z0 * W(0, 0, 2)) * x.AZ(X);
-            w += (-y0 * W(2, 1, 1) + z0 * W(1, 1, 2)) * x.AZ(Y);
-#undef W
+            w += (-y1 * k.w[8] + z1 * k.w[9]) * x.AZ(0, -1, -1);
+            w += (-x1 * k.w[10] + z1 * k.w[11]) * x.AZ(-1, 0, -1);
+            const double azc = (x0 + x1) * k.w[10] + (y0 + y1) * k.w[8];
+            w += azc * (x.AZ(0, 0, -1) + x.AZ(0, 0, 0));
+            w += (-x0 * k.w[10] - z1 * k.w[11]) * x.AZ(1, 0, -1);
+            w += (-y0 * k.w[8] - z1 * k.w[9]) * x.AZ(0, 1, -1);
+            w += (-y1 * k.w[8] - z0 * k.w[9]) * x.AZ(0, -1, 0);
+            w += (-x1 * k.w[10] - z0 * k.w[11]) * x.AZ(-1, 0, 0);
+            w += (-x0 * k.w[10] + z0 * k.w[11]) * x.AZ(1, 0, 0);
+            w += (-y0 * k.w[8] + z0 * k.w[9]) * x.AZ(0, 1, 0);
             y += w;
         }
     }
@@ -259,11 +262,12 @@ __device__ __forceinline__ double row_v(const View<T> &x, const CView &c, const 
 }
 
 // ---------------------------------------------------------------------------
-// Tiled launch geometry.  A workgroup of 256 threads owns a tile of
-// tx x (256/tx) nodes of one z-plane.  Tiles are numbered x-fastest, then y,
-// then z, and the hardware workgroup index is remapped so that each of the
-// 8 XCDs (which receive workgroups round-robin) works on one contiguous
-// range of tiles: neighbouring rows and planes then share that XCD's L2.
+// Tiled launch geometry.  Tiles are numbered x-fastest, then y, then z, and
+// the hardware workgroup index is remapped so that each of the 8 XCDs (which
+// receive workgroups round-robin) works on one contiguous range of tiles:
+// neighbouring rows and planes then share that XCD's L2.  Plane tiles
+// (Tiling): a workgroup of 256 threads owns tx x (256/tx) nodes of one
+// z-plane.
 // ---------------------------------------------------------------------------
 struct Tiling {
     int txs;            // log2 of the tile width
@@ -286,59 +290,129 @@ __device__ __forceinline__ bool tile_node(const Grid &g, const Tiling &t, int &i
 }
 
 // ---------------------------------------------------------------------------
-// Fused operator kernel.
-//   OP_RESID: out = (b - M x)/D on free rows, b - x on fixed rows (x: FP64)
-//   OP_APPLY: out = (M x)/D on free rows, x on fixed rows
-//   OP_VA:    out_V = M_VA x_A on free V rows (A-part of the V rows only),
-//             other entries untouched; used by the block-triangular
-//             preconditioner.
-// The entries of `out` that do not exist in the reference's unknown vector
-// are written as 0.
+// z-marching operator.  A workgroup owns a 64 x 4 column of nodes and walks
+// it through kz consecutive planes.  The four fields of planes k-1, k, k+1
+// (with a one-node halo in x and y) sit in an LDS ring of three slots, so
+// each field value is read from HBM once per column instead of once per
+// stencil use; the material coefficients of plane k are read per node (their
+// -x and -y neighbours hit L1), and the plane-(k-1) values they need are
+// carried in registers from the previous plane.
 // ---------------------------------------------------------------------------
+constexpr int ZTX = 64, ZTY = 4, ZSY = ZTX + 2, ZPL = ZSY * (ZTY + 2);
+
+struct ZTiling {
+    int ntx, nty, kz;
+};
+
+struct LdsView {
+    const double *lds;     // [field][slot][ZPL]
+    int b[3];              // slot offsets (elements) of planes k-1, k, k+1
+    int ctr;               // this node's index within a plane slot
+    __device__ __forceinline__ double get(int f, int di, int dj, int dk) const {
+        return lds[f * 3 * ZPL + b[dk + 1] + ctr + di + dj * ZSY];
+    }
+    __device__ __forceinline__ double AX(int di, int dj, int dk) const { return get(0, di, dj, dk); }
+    __device__ __forceinline__ double AY(int di, int dj, int dk) const { return get(1, di, dj, dk); }
+    __device__ __forceinline__ double AZ(int di, int dj, int dk) const { return get(2, di, dj, dk); }
+    __device__ __forceinline__ double VV(int di, int dj, int dk) const { return get(3, di, dj, dk); }
+};
+
+struct ZmCView {
+    double imx0, imy0, imz0, sx0, sy0, sz0;    // at (0,0,0)
+    double imxz, imyz, imzz, szz;              // at (0,0,-1)
+    const double *imx, *imy, *imz, *sx, *sy_;  // pointers at the node, for the -x / -y neighbours
+    int dxm, dym;                              // offsets of those neighbours (0 at the domain edge)
+    __device__ __forceinline__ double pick(const double *a, double v0, double vz, int di, int dj, int dk) const {
+        return dk ? vz : di ? a[dxm] : dj ? a[dym] : v0;
+    }
+    __device__ __forceinline__ double IMX(int di, int dj, int dk) const { return pick(imx, imx0, imxz, di, dj, dk); }
+    __device__ __forceinline__ double IMY(int di, int dj, int dk) const { return pick(imy, imy0, imyz, di, dj, dk); }
+    __device__ __forceinline__ double IMZ(int di, int dj, int dk) const { return pick(imz, imz0, imzz, di, dj, dk); }
+    __device__ __forceinline__ double SX(int di, int, int) const { return di ? sx[dxm] : sx0; }
+    __device__ __forceinline__ double SY(int, int dj, int) const { return dj ? sy_[dym] : sy0; }
+    __device__ __forceinline__ double SZ(int, int, int dk) const { return dk ? szz : sz0; }
+};
+
 template <typename TIN, typename TOUT, int MODE>
-__global__ void __launch_bounds__(256) k_operator(Grid g, Tiling t, CoeffsD c, OpK k, const TIN *__restrict__ x,
-                                                  const double *__restrict__ b, TOUT *__restrict__ out) {
-    int i, j, kk;
-    if (!tile_node(g, t, i, j, kk)) return;
-    const int64_t p = gidx(g, i, j, kk);
-    const View<TIN> xv = view_at(x, g.n1, p);
-    const CView cv = cview_at(c, p);
-    if (MODE == OP_VA) {
-        if (free_v(g, i, j, kk)) {
-            double d;
-            out[3 * g.n1 + p] = (TOUT)row_v(xv, cv, k, B_VA, d);
+__global__ void __launch_bounds__(256) k_operator(Grid g, ZTiling t, CoeffsD c, OpK k, const TIN *__restrict__ x,
+                                                     const double *__restrict__ b, TOUT *__restrict__ out) {
+    __shared__ double lds[4 * 3 * ZPL];
+    const unsigned tid = xcd_tile(blockIdx.x, gridDim.x);
+    const unsigned tx = tid % (unsigned)t.ntx, r1 = tid / (unsigned)t.ntx;
+    const unsigned ty = r1 % (unsigned)t.nty, tz = r1 / (unsigned)t.nty;
+    const int i0 = (int)tx * ZTX, j0 = (int)ty * ZTY;
+    const int kb = (int)tz * t.kz, ke = min(kb + t.kz, g.nz);
+    const int li = (int)(threadIdx.x & 63), lj = (int)(threadIdx.x >> 6);
+    const int i = i0 + li, j = j0 + lj;
+    const int ic = min(i, g.nx - 1), jc = min(j, g.ny - 1);
+    const bool live = i < g.nx && j < g.ny;
+    const int ctr = (lj + 1) * ZSY + (li + 1);
+    const int64_t n1 = g.n1;
+    const int64_t plane = (int64_t)g.nx * g.ny;
+
+    auto stage = [&](int kp, int slot) {
+        kp = max(0, min(kp, g.nz - 1));
+        for (int e = (int)threadIdx.x; e < ZPL; e += 256) {
+            const int ei = e % ZSY, ej = e / ZSY;
+            const int gi = max(0, min(i0 - 1 + ei, g.nx - 1)), gj = max(0, min(j0 - 1 + ej, g.ny - 1));
+            const int64_t q = gi + (int64_t)g.nx * gj + plane * kp;
+#pragma unroll
+            for (int f = 0; f < 4; ++f) lds[(f * 3 + slot) * ZPL + e] = (double)x[f * n1 + q];
         }
-        return;
+    };
+
+    stage(kb - 1, 0);
+    stage(kb, 1);
+    const int64_t pc = ic + (int64_t)g.nx * jc;        // in-plane index of the node
+    double imxz, imyz, imzz, szz;
+    {
+        const int64_t q = pc + plane * max(kb - 1, 0);
+        imxz = c.im[0][q]; imyz = c.im[1][q]; imzz = c.im[2][q]; szz = c.sg[2][q];
     }
-    if (free_v(g, i, j, kk)) {
-        // interior node: all four rows are free
-        double d, mx;
-        mx = row_ax(xv, cv, k, B_ALL, d);
-        out[p] = (TOUT)((MODE == OP_RESID) ? (b[p] - mx) / d : mx / d);
-        mx = row_ay(xv, cv, k, B_ALL, d);
-        out[g.n1 + p] = (TOUT)((MODE == OP_RESID) ? (b[g.n1 + p] - mx) / d : mx / d);
-        mx = row_az(xv, cv, k, B_ALL, d);
-        out[2 * g.n1 + p] = (TOUT)((MODE == OP_RESID) ? (b[2 * g.n1 + p] - mx) / d : mx / d);
-        mx = row_v(xv, cv, k, B_ALL, d);
-        out[3 * g.n1 + p] = (TOUT)((MODE == OP_RESID) ? (b[3 * g.n1 + p] - mx) / d : mx / d);
-        return;
-    }
-    for (int f = 0; f < 4; ++f) {
-        const int64_t q = f * g.n1 + p;
-        double r;
-        if (!exists_f(g, f, i, j, kk)) {
-            r = 0.0;
-        } else if (!free_f(g, f, i, j, kk)) {
-            r = (MODE == OP_RESID) ? b[q] - (double)x[q] : (double)x[q];
+    const int dxm = ic > 0 ? -1 : 0, dym = jc > 0 ? -g.nx : 0;
+    for (int kk = kb; kk < ke; ++kk) {
+        const int r = kk - kb;
+        const int sm = r % 3, s0 = (r + 1) % 3, sp = (r + 2) % 3;
+        stage(kk + 1, sp);
+        __syncthreads();
+        const int64_t p = pc + plane * kk;
+        const ZmCView cv{c.im[0][p], c.im[1][p], c.im[2][p], c.sg[0][p], c.sg[1][p], c.sg[2][p],
+                         imxz, imyz, imzz, szz,
+                         c.im[0] + p, c.im[1] + p, c.im[2] + p, c.sg[0] + p, c.sg[1] + p, dxm, dym};
+        const LdsView xv{lds, {sm * ZPL, s0 * ZPL, sp * ZPL}, ctr};
+        if (MODE == OP_VA) {
+            if (live && free_v(g, i, j, kk)) {
+                double d;
+                out[3 * n1 + p] = (TOUT)row_v(xv, cv, k, B_VA, d);
+            }
         } else {
-            // only Ax (i = 1), Ay (j = 1) and Az (k = 1) rows are free outside the V interior
-            double d, mx;
-            if (f == FAX) mx = row_ax(xv, cv, k, B_ALL, d);
-            else if (f == FAY) mx = row_ay(xv, cv, k, B_ALL, d);
-            else mx = row_az(xv, cv, k, B_ALL, d);
-            r = (MODE == OP_RESID) ? (b[q] - mx) / d : mx / d;
+            double dax, day, daz, dv;
+            const double rax = row_ax(xv, cv, k, B_ALL, dax);
+            const double ray = row_ay(xv, cv, k, B_ALL, day);
+            const double raz = row_az(xv, cv, k, B_ALL, daz);
+            const double rv = row_v(xv, cv, k, B_ALL, dv);
+            if (live) {
+                auto emit = [&](int f, double mx, double d) {
+                    const int64_t q = f * n1 + p;
+                    double rr;
+                    if (free_f(g, f, i, j, kk)) {
+                        rr = (MODE == OP_RESID) ? (b[q] - mx) / d : mx / d;
+                    } else if (exists_f(g, f, i, j, kk)) {
+                        const double xq = xv.get(f, 0, 0, 0);
+                        rr = (MODE == OP_RESID) ? b[q] - xq : xq;
+                    } else {
+                        rr = 0.0;
+                    }
+                    out[q] = (TOUT)rr;
+                };
+                emit(0, rax, dax);
+                emit(1, ray, day);
+                emit(2, raz, daz);
+                emit(3, rv, dv);
+            }
         }
-        out[q] = (TOUT)r;
+        imxz = cv.imx0; imyz = cv.imy0; imzz = cv.imz0; szz = cv.sz0;
+        __syncthreads();
     }
 }
 
@@ -347,8 +421,8 @@ __global__ void k_diag(Grid g, CoeffsD c, OpK k, double *__restrict__ D) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= g.n1) return;
     const int i = (int)(p % g.nx), j = (int)((p / g.nx) % g.ny), kk = (int)(p / ((int64_t)g.nx * g.ny));
-    const View<double> xv{nullptr, nullptr, nullptr, nullptr};
-    const CView cv = cview_at(c, p);
+    const View<double> xv{nullptr, nullptr, nullptr, nullptr, k.sy, k.sz};
+    const CView cv = cview_at(c, p, k);
     for (int f = 0; f < 4; ++f) {
         double d = 0.0;
         if (exists_f(g, f, i, j, kk)) {
@@ -451,24 +525,43 @@ OpK make_opk(const Grid &g, const OpParams &op) {
         k.ih2[a] = 1.0 / (h[a] * h[a]);
         k.ih[a] = 1.0 / h[a];
         k.iht[a] = op.tr ? 1.0 / (h[a] * op.dt) : 0.0;
-        for (int b = 0; b < 3; ++b) {
-            k.ihh[a][b] = 1.0 / (h[a] * h[b]);
-            k.v1[a][b] = op.v[a] / (2.0 * h[b]);
-            for (int c = 0; c < 3; ++c) k.v2[a][b][c] = op.v[a] / (2.0 * h[b] * h[c]);
-        }
+        for (int b = 0; b < 3; ++b) k.v1[a][b] = op.v[a] / (2.0 * h[b]);
     }
+    k.ixy = 1.0 / (h[0] * h[1]);
+    k.ixz = 1.0 / (h[0] * h[2]);
+    k.iyz = 1.0 / (h[1] * h[2]);
+    // (a, b, c) of each v_a / (2 h_b h_c) term of the V row, in the order of OpK::w
+    static const int W[12][3] = {{2, 2, 0}, {0, 2, 2}, {1, 1, 0}, {0, 1, 1}, {2, 2, 1}, {1, 2, 2},
+                                 {1, 0, 0}, {0, 0, 1}, {2, 1, 1}, {1, 1, 2}, {2, 0, 0}, {0, 0, 2}};
+    for (int q = 0; q < 12; ++q) k.w[q] = op.v[W[q][0]] / (2.0 * h[W[q][1]] * h[W[q][2]]);
     return k;
 }
 
-static Tiling tiling(const Grid &g, int tile_x, dim3 &grid) {
-    if (tile_x != 32 && tile_x != 64 && tile_x != 128 && tile_x != 256) tile_x = 64;
+// The RHS kernel uses 128 x 2 tiles of one plane.
+static Tiling rhs_tiling(const Grid &g, dim3 &grid) {
     Tiling t;
-    t.txs = tile_x == 32 ? 5 : tile_x == 64 ? 6 : tile_x == 128 ? 7 : 8;
-    const int rows = 256 >> t.txs;
-    t.ntx = (g.nx + tile_x - 1) / tile_x;
-    t.nty = (g.ny + rows - 1) / rows;
+    t.txs = 7;
+    t.ntx = (g.nx + 127) / 128;
+    t.nty = (g.ny + 1) / 2;
     grid = dim3((unsigned)((int64_t)t.ntx * t.nty * g.nz));
     return t;
+}
+
+static ZTiling ztiling(const Grid &g, int kz, dim3 &grid) {
+    ZTiling t;
+    t.kz = kz >= 1 ? kz : 4;
+    t.ntx = (g.nx + ZTX - 1) / ZTX;
+    t.nty = (g.ny + ZTY - 1) / ZTY;
+    grid = dim3((unsigned)((int64_t)t.ntx * t.nty * ((g.nz + t.kz - 1) / t.kz)));
+    return t;
+}
+
+template <typename TIN, typename TOUT, int MODE>
+static void launch_operator(const Grid &g, const CoeffsD &c, const OpParams &op, const TIN *x, const double *b,
+                            TOUT *out, hipStream_t s) {
+    dim3 grid;
+    const ZTiling t = ztiling(g, op.zchunk, grid);
+    hipLaunchKernelGGL((k_operator<TIN, TOUT, MODE>), grid, dim3(256), 0, s, g, t, c, make_opk(g, op), x, b, out);
 }
 
 void launch_coeffs(const Grid &g, const uint16_t *slot, const double *mat, const Coeffs &c, hipStream_t s) {
@@ -480,7 +573,7 @@ void launch_diag(const Grid &g, const CoeffsD &c, const OpParams &op, double *D,
 void launch_rhs(const Grid &g, const CoeffsD &c, const BC &bc, const OpParams &op, const double *aprev, double *b,
                 hipStream_t s) {
     dim3 grid;
-    const Tiling t = tiling(g, op.tile_x, grid);
+    const Tiling t = rhs_tiling(g, grid);
     hipLaunchKernelGGL(k_rhs, grid, dim3(256), 0, s, g, t, c, bc, make_opk(g, op), aprev, b);
 }
 void launch_setbc(const Grid &g, const BC &bc, const OpParams &op, double *x, hipStream_t s) {
@@ -488,28 +581,16 @@ void launch_setbc(const Grid &g, const BC &bc, const OpParams &op, double *x, hi
 }
 void launch_resid(const Grid &g, const CoeffsD &c, const OpParams &op, const double *x, const double *b, double *r,
                   hipStream_t s) {
-    dim3 grid;
-    const Tiling t = tiling(g, op.tile_x, grid);
-    hipLaunchKernelGGL((k_operator<double, double, OP_RESID>), grid, dim3(256), 0, s, g, t, c, make_opk(g, op), x, b,
-                       r);
+    launch_operator<double, double, OP_RESID>(g, c, op, x, b, r, s);
 }
 void launch_apply(const Grid &g, const CoeffsD &c, const OpParams &op, const double *x, double *y, hipStream_t s) {
-    dim3 grid;
-    const Tiling t = tiling(g, op.tile_x, grid);
-    hipLaunchKernelGGL((k_operator<double, double, OP_APPLY>), grid, dim3(256), 0, s, g, t, c, make_opk(g, op), x,
-                       (const double *)nullptr, y);
+    launch_operator<double, double, OP_APPLY>(g, c, op, x, nullptr, y, s);
 }
 void launch_va_f32(const Grid &g, const CoeffsD &c, const OpParams &op, const float *x, float *y, hipStream_t s) {
-    dim3 grid;
-    const Tiling t = tiling(g, op.tile_x, grid);
-    hipLaunchKernelGGL((k_operator<float, float, OP_VA>), grid, dim3(256), 0, s, g, t, c, make_opk(g, op), x,
-                       (const double *)nullptr, y);
+    launch_operator<float, float, OP_VA>(g, c, op, x, nullptr, y, s);
 }
 void launch_va_f64(const Grid &g, const CoeffsD &c, const OpParams &op, const double *x, double *y, hipStream_t s) {
-    dim3 grid;
-    const Tiling t = tiling(g, op.tile_x, grid);
-    hipLaunchKernelGGL((k_operator<double, double, OP_VA>), grid, dim3(256), 0, s, g, t, c, make_opk(g, op), x,
-                       (const double *)nullptr, y);
+    launch_operator<double, double, OP_VA>(g, c, op, x, nullptr, y, s);
 }
 void launch_hfield(const Grid &g, const CoeffsD &c, const double *x, double *H, int unit_mu, hipStream_t s) {
     const int64_t nc = (int64_t)(g.nx - 1) * (g.ny - 1) * (g.nz - 1);

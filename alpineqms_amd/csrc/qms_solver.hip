@@ -60,7 +60,7 @@ struct qms_handle {
     double *partial = nullptr, *dsmall = nullptr;
     double *hsmall = nullptr;   // pinned
     int nblk = 1;
-    int tile_x = 64;            // operator tile width in nodes
+    int zchunk = 4;             // planes per operator workgroup
 
     // preconditioner work vectors (4*n1 in MG precision)
     void *rT = nullptr, *zT = nullptr, *tT = nullptr;
@@ -143,7 +143,7 @@ OpParams make_op(const qms_handle *h, const double v[3], double dt, int tr) {
     op.dt = dt;
     op.tr = tr;
     op.adv = (v[0] != 0.0 || v[1] != 0.0 || v[2] != 0.0) ? 1 : 0;
-    op.tile_x = h->tile_x;
+    op.zchunk = h->zchunk;
     return op;
 }
 
@@ -534,13 +534,13 @@ int qms_get_stats(const qms_handle *h, qms_stats *st) {
     return QMS_OK;
 }
 
-int qms_set_tile(qms_handle *h, int tile_x) {
+int qms_set_zchunk(qms_handle *h, int planes) {
     if (!h) return QMS_ERR_ARG;
-    if (tile_x != 32 && tile_x != 64 && tile_x != 128 && tile_x != 256) {
-        h->err = "qms_set_tile: tile width must be 32, 64, 128 or 256";
+    if (planes < 1 || planes > 64) {
+        h->err = "qms_set_zchunk: planes per workgroup must be in [1, 64]";
         return QMS_ERR_ARG;
     }
-    h->tile_x = tile_x;
+    h->zchunk = planes;
     return QMS_OK;
 }
 

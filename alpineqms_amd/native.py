@@ -27,7 +27,7 @@ QMS_MIXED = 1
 EXPORTS = (
     "qms_config_default", "qms_create", "qms_steady", "qms_step", "qms_run",
     "qms_get_fields", "qms_set_fields", "qms_get_h", "qms_apply", "qms_rhs",
-    "qms_get_stats", "qms_set_tile", "qms_time_kernel", "qms_destroy", "qms_last_error",
+    "qms_get_stats", "qms_set_zchunk", "qms_time_kernel", "qms_destroy", "qms_last_error",
 )
 
 # qms_time_kernel selectors
@@ -100,7 +100,7 @@ def load(path: Optional[str] = None):
     L.qms_apply.argtypes = [H, d, ctypes.c_double, d, d]
     L.qms_rhs.argtypes = [H, d, ctypes.c_double, d, d]
     L.qms_get_stats.argtypes = [H, ctypes.POINTER(Stats)]
-    L.qms_set_tile.argtypes = [H, ctypes.c_int]
+    L.qms_set_zchunk.argtypes = [H, ctypes.c_int]
     L.qms_time_kernel.argtypes = [H, ctypes.c_int, d, ctypes.c_int, d, d]
     L.qms_destroy.argtypes = [H]
     L.qms_destroy.restype = None
@@ -242,8 +242,10 @@ class Solver:
         self._check(self._L.qms_get_stats(self._h, ctypes.byref(s)))
         return s
 
-    def set_tile(self, tile_x: int):
-        self._check(self._L.qms_set_tile(self._h, int(tile_x)))
+    def set_zchunk(self, planes: int):
+        """Planes per operator workgroup (z-march depth of the LDS-staged
+        operator kernel)."""
+        self._check(self._L.qms_set_zchunk(self._h, int(planes)))
 
     def time_kernel(self, which: int, reps: int, v=None):
         """Mean milliseconds and algorithmic bytes per launch of one kernel

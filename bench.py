@@ -13,14 +13,14 @@ What is measured on the GPU: the fused A-V operator kernel at 128^3 (the
 matrix-free replacement of the reference's per-step CSR assembly and the
 operator inside every Krylov iteration) with the rows of the last ramp step
 (transient, v = 5 m/s), against the HBM roofline, with HIP events on the
-stream it is launched on; its tile width is picked from a sweep over
-32..256 nodes.  Beside it: the FP64 residual, the step right-hand side, the
+stream it is launched on; its z-chunk depth (planes per workgroup) is
+picked from a sweep over 2..8.  Beside it: the FP64 residual, the step right-hand side, the
 cell-centred H and a plain copy kernel of the same state (calibration).  The CPU baseline is the oracle (CSR assembly +
 equilibrated SuperLU factor/solve, the restated reference algorithm) timed
 on one core over one ramp step of a 24^3-node design drawn from the same
 box distribution.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--tile T] [--no-cpu-baseline]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--zchunk KZ] [--no-cpu-baseline]
 """
 import argparse
 import json
@@ -56,7 +56,7 @@ def cpu_baseline():
             "dof_steps_per_s": d.dof / t}
 
 
-TILES = (32, 64, 128, 256)
+ZCHUNKS = (2, 3, 4, 6, 8)
 
 
 def traffic_from_profile(kernel: str):
@@ -84,7 +84,8 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--nodes", type=int, default=128)
-    ap.add_argument("--tile", type=int, default=0, help="operator tile width (0: pick the fastest of 32..256)")
+    ap.add_argument("--zchunk", type=int, default=0,
+                    help="planes per operator workgroup (0: pick the fastest of 2..8)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -112,13 +113,13 @@ def main():
         return s.time_kernel(which, a.steps, v=v)
 
     sweep = {}
-    tiles = (a.tile,) if a.tile else TILES
-    for t in tiles:
-        s.set_tile(t)
-        sweep[t] = timed(native.K_APPLY)[0]
-        print(f"bench: operator tile {t}: {sweep[t] * 1e3:.1f} us", file=sys.stderr, flush=True)
-    tile = min(sweep, key=sweep.get)
-    s.set_tile(tile)
+    chunks = (a.zchunk,) if a.zchunk else ZCHUNKS
+    for kz in chunks:
+        s.set_zchunk(kz)
+        sweep[kz] = timed(native.K_APPLY)[0]
+        print(f"bench: operator z-chunk {kz}: {sweep[kz] * 1e3:.1f} us", file=sys.stderr, flush=True)
+    kz = min(sweep, key=sweep.get)
+    s.set_zchunk(kz)
     ms, nbytes = timed(native.K_APPLY)
     achieved = nbytes / (ms * 1e-3) / 1e9
     others = {}
@@ -152,11 +153,12 @@ def main():
                                f"(rows of the last ramp step: transient, v = {v[0]:g} m/s)",
                    "dof": d.dof},
         "status": "time-step rate not measured: " + (status or "solve path unavailable"),
-        "roofline": {"kernel": f"k_operator (fused A-V operator apply, FP64, tile {tile})", "bound": "hbm",
+        "roofline": {"kernel": f"k_operator (fused A-V operator apply, FP64, LDS z-march, z-chunk {kz})",
+                     "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": tsrc,
                      "algorithmic_bytes": nbytes, "ms_per_launch": ms, "launches": a.steps},
-        "tile_sweep_ms": {str(t): m for t, m in sweep.items()},
+        "zchunk_sweep_ms": {str(t): m for t, m in sweep.items()},
         "kernels": others,
         "cpu_baseline": None if a.no_cpu_baseline else cpu_baseline(),
     }

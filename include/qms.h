@@ -113,11 +113,12 @@ int qms_rhs(qms_handle *h, const double v[3], double dt, const double *aprev, do
 
 int qms_get_stats(const qms_handle *h, qms_stats *st);
 
-/* Operator tile width in nodes (32, 64, 128 or 256; default 64): each
- * 256-thread workgroup of the operator and RHS kernels covers tile_x x
- * (256/tile_x) nodes of one z-plane.  A tuning knob with no counterpart in
- * the reference (whose assembly loop is serial, QMS_3D.f90:448-1293). */
-int qms_set_tile(qms_handle *h, int tile_x);
+/* Planes per operator workgroup (1..64, default 4): each 256-thread
+ * workgroup of the operator kernel owns a 64 x 4 column of nodes and marches
+ * it through this many z-planes, staging the fields through LDS.  A tuning
+ * knob with no counterpart in the reference (whose assembly loop is serial,
+ * QMS_3D.f90:448-1293). */
+int qms_set_zchunk(qms_handle *h, int planes);
 
 /* Times `reps` launches of one kernel on the handle's stream with HIP events
  * and reports the mean duration and the algorithmic HBM bytes per launch.

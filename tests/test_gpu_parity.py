@@ -33,7 +33,7 @@ CASES = [
 ]
 
 
-def _operator_check(d, v, transient, tiles, seed=7):
+def _operator_check(d, v, transient, chunks, seed=7):
     p = problem(d)
     coef = O.coeffs(p)
     rng = np.random.default_rng(seed)
@@ -45,37 +45,38 @@ def _operator_check(d, v, transient, tiles, seed=7):
     scale = np.abs(M) @ np.abs(x)
     D = np.abs(M.diagonal())
     with solver_for(d) as s:
-        for tile in tiles:
-            s.set_tile(tile)
+        for kz in chunks:
+            s.set_zchunk(kz)
             y = O.pack(p, s.apply(v, dt, O.unpack(p, x)))
             err = np.max(np.abs(y - y_ref) / scale)
-            assert err < OP_TOL, (d.name, transient, tile, err)
+            assert err < OP_TOL, (d.name, transient, kz, err)
             assert rel(y / D, y_ref / D) < OP_TOL
 
 
 @pytest.mark.parametrize("name,mk", CASES, ids=[c[0] for c in CASES])
 @pytest.mark.parametrize("transient", [False, True])
 def test_operator_matches_csr(name, mk, transient):
-    _operator_check(mk(), np.array([1.5, -0.7, 0.4]), transient, (64, 32, 128, 256))
+    _operator_check(mk(), np.array([1.5, -0.7, 0.4]), transient, (4, 1, 3, 64))
 
 
 def test_operator_matches_csr_ragged_grid():
-    """Cell counts that fill no tile evenly (partial tiles on every axis)."""
+    """Node counts (38 x 46 x 30) that fill no workgroup column or z-chunk
+    evenly (partial tiles on every axis)."""
     d = configs.small_box_design(0, 11, nbox=4, v=2.0, ramp=False, gradient=True, cells=(37, 45, 29))
-    _operator_check(d, np.array([-2.0, 0.5, 1.25]), True, (32, 64, 128, 256), seed=13)
+    _operator_check(d, np.array([-2.0, 0.5, 1.25]), True, (4, 7, 2, 30), seed=13)
 
 
 def test_operator_matches_csr_config2_full_size():
     """64^3 nodes, the full size of the 8-box design at 0.5 mm (1,036,288 DoF),
     against the oracle's CSR (19.4M nonzeros)."""
-    _operator_check(configs.config2(N=64, steps=2), np.array([0.0, 0.0, 0.0]), True, (64,), seed=17)
-    _operator_check(configs.config2(N=64, steps=2, ramp_v=5.0), np.array([5.0, 0.0, 0.0]), True, (128,), seed=19)
+    _operator_check(configs.config2(N=64, steps=2), np.array([0.0, 0.0, 0.0]), True, (4,), seed=17)
+    _operator_check(configs.config2(N=64, steps=2, ramp_v=5.0), np.array([5.0, 0.0, 0.0]), True, (6,), seed=19)
 
 
 def test_operator_full_size_properties():
-    """128^3 nodes (8,339,456 DoF, config 3 geometry): every tile width gives
-    bit-identical output (each entry is written exactly once, by the same
-    arithmetic), and the operator is linear to round-off."""
+    """128^3 nodes (8,339,456 DoF, config 3 geometry): every z-chunk depth
+    gives bit-identical output (each entry is written exactly once, by the
+    same arithmetic), and the operator is linear to round-off."""
     d = configs.config3(N=128, ramp=True)
     nz, ny, nx = d.geom.shape[0] + 1, d.geom.shape[1] + 1, d.geom.shape[2] + 1
     rng = np.random.default_rng(23)
@@ -85,8 +86,8 @@ def test_operator_full_size_properties():
     dt = d.inputs.dt
     with solver_for(d) as s:
         ys = []
-        for tile in (64, 32, 128, 256):
-            s.set_tile(tile)
+        for kz in (4, 1, 5, 64):
+            s.set_zchunk(kz)
             ys.append(s.apply(v, dt, x1))
         for y in ys[1:]:
             assert np.array_equal(y, ys[0])
@@ -116,10 +117,8 @@ def test_rhs_matches_oracle(name, mk):
     for dt in (0.0, p.dt):
         _, b_ref = O.assemble(p, coef, v, dt, aprev)
         with solver_for(d) as s:
-            for tile in (64, 32, 256):
-                s.set_tile(tile)
-                b = O.pack(p, s.rhs(v, dt, aprev))
-                assert np.max(np.abs(b - b_ref)) <= 1e-13 * np.max(np.abs(b_ref)), (dt, tile)
+            b = O.pack(p, s.rhs(v, dt, aprev))
+        assert np.max(np.abs(b - b_ref)) <= 1e-13 * np.max(np.abs(b_ref)), dt
 
 
 @pytest.mark.parametrize("name,mk", CASES, ids=[c[0] for c in CASES])
