@@ -47,6 +47,24 @@ struct OpK {
 };
 OpK make_opk(const Grid &g, const OpParams &op);
 
+// Workgroup columns of the z-marching operator: 64 x 4 nodes, kz planes.
+struct ZTiling {
+    int ntx, nty, kz;
+};
+ZTiling ztiling(const Grid &g, int kz, unsigned &ncol);
+
+// One design of a batched operator launch (device resident).
+struct BatchItem {
+    Grid g;
+    ZTiling t;
+    CoeffsD c;
+    OpK k;
+    const double *x;
+    double *y;
+    unsigned col0, ncol;      // first workgroup column in the batch, column count
+};
+void launch_operator_batch(const BatchItem *items_dev, int nitems, unsigned total, int blocks, hipStream_t s);
+
 enum { OP_RESID = 0, OP_APPLY = 1, OP_VA = 2 };
 
 void launch_coeffs(const Grid &g, const uint16_t *slot, const double *mat, const Coeffs &c, hipStream_t s);

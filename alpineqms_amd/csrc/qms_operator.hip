@@ -14,6 +14,11 @@
 #include <hip/hip_runtime.h>
 #include "qms_kernels.h"
 
+// Multiply-adds are fused only within one source expression, never across
+// statements: the same row code then rounds identically in every kernel that
+// inlines it (single-design and batched launches give bit-identical rows).
+#pragma clang fp contract(on)
+
 namespace qms {
 
 // ---------------------------------------------------------------------------
@@ -300,9 +305,6 @@ __device__ __forceinline__ bool tile_node(const Grid &g, const Tiling &t, int &i
 // ---------------------------------------------------------------------------
 constexpr int ZTX = 64, ZTY = 4, ZSY = ZTX + 2, ZPL = ZSY * (ZTY + 2);
 
-struct ZTiling {
-    int ntx, nty, kz;
-};
 
 struct LdsView {
     const double *lds;     // [field][slot][ZPL]
@@ -333,11 +335,12 @@ struct ZmCView {
     __device__ __forceinline__ double SZ(int, int, int dk) const { return dk ? szz : sz0; }
 };
 
+// One workgroup column `tid` of one design.  Every thread of the workgroup
+// must call this (it synchronises the workgroup); `lds` holds 4*3*ZPL doubles.
 template <typename TIN, typename TOUT, int MODE>
-__global__ void __launch_bounds__(256) k_operator(Grid g, ZTiling t, CoeffsD c, OpK k, const TIN *__restrict__ x,
-                                                     const double *__restrict__ b, TOUT *__restrict__ out) {
-    __shared__ double lds[4 * 3 * ZPL];
-    const unsigned tid = xcd_tile(blockIdx.x, gridDim.x);
+__device__ __forceinline__ void op_column(const Grid &g, const ZTiling &t, unsigned tid, const CoeffsD &c,
+                                          const OpK &k, const TIN *__restrict__ x, const double *__restrict__ b,
+                                          TOUT *__restrict__ out, double *lds) {
     const unsigned tx = tid % (unsigned)t.ntx, r1 = tid / (unsigned)t.ntx;
     const unsigned ty = r1 % (unsigned)t.nty, tz = r1 / (unsigned)t.nty;
     const int i0 = (int)tx * ZTX, j0 = (int)ty * ZTY;
@@ -416,6 +419,36 @@ __global__ void __launch_bounds__(256) k_operator(Grid g, ZTiling t, CoeffsD c, 
     }
 }
 
+template <typename TIN, typename TOUT, int MODE>
+__global__ void __launch_bounds__(256) k_operator(Grid g, ZTiling t, CoeffsD c, OpK k, const TIN *__restrict__ x,
+                                                  const double *__restrict__ b, TOUT *__restrict__ out) {
+    __shared__ double lds[4 * 3 * ZPL];
+    op_column<TIN, TOUT, MODE>(g, t, xcd_tile(blockIdx.x, gridDim.x), c, k, x, b, out, lds);
+}
+
+// ---------------------------------------------------------------------------
+// Batched operator: y_d = D_d^-1 M_d x_d for many independent designs in one
+// persistent launch.  The work list is the concatenation of every design's
+// workgroup columns (design d owns columns [col0, col0 + ncol)).  The grid
+// is sized to the machine; the columns are split into 8 contiguous ranges,
+// one per XCD, and the workgroups of an XCD stride through its range, so
+// neighbouring columns of a design run together and share that XCD's L2.
+// Every loop bound is workgroup-uniform and every workgroup leaves after its
+// range, so the grid always drains.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_operator_batch(const BatchItem *__restrict__ items, int nitems,
+                                                        unsigned total) {
+    __shared__ double lds[4 * 3 * ZPL];
+    const unsigned xcd = blockIdx.x & 7u, per = gridDim.x >> 3, l = blockIdx.x >> 3;
+    const unsigned lo = (unsigned)(((uint64_t)total * xcd) >> 3), hi = (unsigned)(((uint64_t)total * (xcd + 1)) >> 3);
+    int d = 0;
+    for (unsigned w = lo + l; w < hi; w += per) {
+        while (d + 1 < nitems && items[d + 1].col0 <= w) ++d;     // w only grows: forward scan
+        const BatchItem &it = items[d];
+        op_column<double, double, OP_APPLY>(it.g, it.t, w - it.col0, it.c, it.k, it.x, nullptr, it.y, lds);
+    }
+}
+
 // Row diagonal D for every entry (1 on fixed rows, 0 where no unknown exists).
 __global__ void k_diag(Grid g, CoeffsD c, OpK k, double *__restrict__ D) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -484,27 +517,36 @@ __global__ void k_setbc(Grid g, BC bc, OpK k, double *__restrict__ x) {
 // Cell-centred H (or B when unit_mu), QMS_3D.f90:1346-1376.
 // Output: 3 arrays of cx*cy*cz, x fastest.
 // ---------------------------------------------------------------------------
-__global__ void k_hfield(Grid g, CoeffsD c, const double *__restrict__ x, double *__restrict__ H, int unit_mu) {
+__global__ void __launch_bounds__(256) k_hfield(Grid g, CoeffsD c, const double *__restrict__ x,
+                                                double *__restrict__ H, int unit_mu, double ihx, double ihy,
+                                                double ihz) {
     const int cx = g.nx - 1, cy = g.ny - 1, cz = g.nz - 1;
     const int64_t nc = (int64_t)cx * cy * cz;
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t q = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (q >= nc) return;
     const int i = (int)(q % cx), j = (int)((q / cx) % cy), k = (int)(q / ((int64_t)cx * cy));
-    const int64_t n1 = g.n1;
-    const double *Ax = x, *Ay = x + n1, *Az = x + 2 * n1;
-    const double hx = g.hx, hy = g.hy, hz = g.hz;
-    auto F = [&](const double *a, int ii, int jj, int kk) { return a[gidx(g, ii, jj, kk)]; };
-    auto MU = [&](int comp, int ii, int jj, int kk) { return unit_mu ? 1.0 : 1.0 / c.im[comp][gidx(g, ii, jj, kk)]; };
-    // The reference divides by Mu; 1/(1/Mu) is not always Mu bit-for-bit, so
-    // the multiplication form below uses the stored inverse directly.
-    auto IM = [&](int comp, int ii, int jj, int kk) { return unit_mu ? 1.0 : c.im[comp][gidx(g, ii, jj, kk)]; };
-    (void)MU;
-    H[q] = 0.5 * (((F(Az, i, j + 1, k) - F(Az, i, j, k)) / hy - (F(Ay, i, j, k + 1) - F(Ay, i, j, k)) / hz) * IM(0, i, j, k)
-                + ((F(Az, i + 1, j + 1, k) - F(Az, i + 1, j, k)) / hy - (F(Ay, i + 1, j, k + 1) - F(Ay, i + 1, j, k)) / hz) * IM(0, i + 1, j, k));
-    H[nc + q] = 0.5 * (((F(Ax, i, j, k + 1) - F(Ax, i, j, k)) / hz - (F(Az, i + 1, j, k) - F(Az, i, j, k)) / hx) * IM(1, i, j, k)
-                     + ((F(Ax, i, j + 1, k + 1) - F(Ax, i, j + 1, k)) / hz - (F(Az, i + 1, j + 1, k) - F(Az, i, j + 1, k)) / hx) * IM(1, i, j + 1, k));
-    H[2 * nc + q] = 0.5 * (((F(Ay, i + 1, j, k) - F(Ay, i, j, k)) / hx - (F(Ax, i, j + 1, k) - F(Ax, i, j, k)) / hy) * IM(2, i, j, k)
-                         + ((F(Ay, i + 1, j, k + 1) - F(Ay, i, j, k + 1)) / hx - (F(Ax, i, j + 1, k + 1) - F(Ax, i, j, k + 1)) / hy) * IM(2, i, j, k + 1));
+    const int64_t n1 = g.n1, p = gidx(g, i, j, k);
+    const int X = 1, Y = g.nx, Z = g.nx * g.ny;
+    const double *Ax = x + p, *Ay = x + n1 + p, *Az = x + 2 * n1 + p;
+    // Face curls (B on the x-, y- and z-faces) at the two faces of the cell
+    // along each axis, multiplied by the stored 1/Mu (the reference divides
+    // by Mu; 1/(1/Mu) is not always Mu bit for bit).
+    const double bx0 = (Az[Y] - Az[0]) * ihy - (Ay[Z] - Ay[0]) * ihz;
+    const double bx1 = (Az[X + Y] - Az[X]) * ihy - (Ay[X + Z] - Ay[X]) * ihz;
+    const double by0 = (Ax[Z] - Ax[0]) * ihz - (Az[X] - Az[0]) * ihx;
+    const double by1 = (Ax[Y + Z] - Ax[Y]) * ihz - (Az[X + Y] - Az[Y]) * ihx;
+    const double bz0 = (Ay[X] - Ay[0]) * ihx - (Ax[Y] - Ax[0]) * ihy;
+    const double bz1 = (Ay[X + Z] - Ay[Z]) * ihx - (Ax[Y + Z] - Ax[Z]) * ihy;
+    if (unit_mu) {
+        H[q] = 0.5 * (bx0 + bx1);
+        H[nc + q] = 0.5 * (by0 + by1);
+        H[2 * nc + q] = 0.5 * (bz0 + bz1);
+    } else {
+        const double *imx = c.im[0] + p, *imy = c.im[1] + p, *imz = c.im[2] + p;
+        H[q] = 0.5 * (bx0 * imx[0] + bx1 * imx[X]);
+        H[nc + q] = 0.5 * (by0 * imy[0] + by1 * imy[Y]);
+        H[2 * nc + q] = 0.5 * (bz0 * imz[0] + bz1 * imz[Z]);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -547,21 +589,31 @@ static Tiling rhs_tiling(const Grid &g, dim3 &grid) {
     return t;
 }
 
-static ZTiling ztiling(const Grid &g, int kz, dim3 &grid) {
+ZTiling ztiling(const Grid &g, int kz, unsigned &ncol) {
     ZTiling t;
     t.kz = kz >= 1 ? kz : 4;
     t.ntx = (g.nx + ZTX - 1) / ZTX;
     t.nty = (g.ny + ZTY - 1) / ZTY;
-    grid = dim3((unsigned)((int64_t)t.ntx * t.nty * ((g.nz + t.kz - 1) / t.kz)));
+    ncol = (unsigned)((int64_t)t.ntx * t.nty * ((g.nz + t.kz - 1) / t.kz));
     return t;
 }
 
 template <typename TIN, typename TOUT, int MODE>
 static void launch_operator(const Grid &g, const CoeffsD &c, const OpParams &op, const TIN *x, const double *b,
                             TOUT *out, hipStream_t s) {
-    dim3 grid;
-    const ZTiling t = ztiling(g, op.zchunk, grid);
-    hipLaunchKernelGGL((k_operator<TIN, TOUT, MODE>), grid, dim3(256), 0, s, g, t, c, make_opk(g, op), x, b, out);
+    unsigned ncol;
+    const ZTiling t = ztiling(g, op.zchunk, ncol);
+    hipLaunchKernelGGL((k_operator<TIN, TOUT, MODE>), dim3(ncol), dim3(256), 0, s, g, t, c, make_opk(g, op), x, b, out);
+}
+
+// blocks: persistent grid size (rounded up to a multiple of 8 XCDs), or 0 for
+// one workgroup per column.
+void launch_operator_batch(const BatchItem *items_dev, int nitems, unsigned total, int blocks, hipStream_t s) {
+    unsigned g = blocks > 0 ? (unsigned)blocks : total;
+    g = (g + 7u) & ~7u;
+    if (g > ((total + 7u) & ~7u)) g = (total + 7u) & ~7u;
+    if (g < 8u) g = 8u;
+    hipLaunchKernelGGL(k_operator_batch, dim3(g), dim3(256), 0, s, items_dev, nitems, total);
 }
 
 void launch_coeffs(const Grid &g, const uint16_t *slot, const double *mat, const Coeffs &c, hipStream_t s) {
@@ -594,7 +646,7 @@ void launch_va_f64(const Grid &g, const CoeffsD &c, const OpParams &op, const do
 }
 void launch_hfield(const Grid &g, const CoeffsD &c, const double *x, double *H, int unit_mu, hipStream_t s) {
     const int64_t nc = (int64_t)(g.nx - 1) * (g.ny - 1) * (g.nz - 1);
-    hipLaunchKernelGGL(k_hfield, grid1(nc), dim3(256), 0, s, g, c, x, H, unit_mu);
+    hipLaunchKernelGGL(k_hfield, grid1(nc), dim3(256), 0, s, g, c, x, H, unit_mu, 1.0 / g.hx, 1.0 / g.hy, 1.0 / g.hz);
 }
 
 }  // namespace qms

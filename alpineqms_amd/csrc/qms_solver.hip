@@ -16,6 +16,7 @@
 #include <vector>
 #include <stdexcept>
 #include <memory>
+#include <functional>
 #include "qms_kernels.h"
 #include "../../include/qms.h"
 
@@ -515,6 +516,20 @@ int qms_apply(qms_handle *h, const double v[3], double dt, const double *x, doub
     QMS_CATCH(h)
 }
 
+int qms_residual(qms_handle *h, const double v[3], double dt, const double *x, const double *b, double *r) {
+    if (!h || !v || !x || !b || !r) return QMS_ERR_ARG;
+    QMS_TRY(h)
+    const int tr = dt > 0 ? 1 : 0;
+    const OpParams op = make_op(h, v, dt, tr);
+    check(hipMemcpyAsync(h->r, x, sizeof(double) * h->n, hipMemcpyHostToDevice, h->s), "h2d");
+    check(hipMemcpyAsync(h->tmp, b, sizeof(double) * h->n, hipMemcpyHostToDevice, h->s), "h2d");
+    launch_resid(h->g, h->c64, op, h->r, h->tmp, h->w, h->s);
+    check(hipMemcpyAsync(r, h->w, sizeof(double) * h->n, hipMemcpyDeviceToHost, h->s), "d2h");
+    check(hipStreamSynchronize(h->s), "sync");
+    return QMS_OK;
+    QMS_CATCH(h)
+}
+
 int qms_rhs(qms_handle *h, const double v[3], double dt, const double *aprev, double *b) {
     if (!h || !v || !aprev || !b) return QMS_ERR_ARG;
     QMS_TRY(h)
@@ -609,6 +624,204 @@ int qms_time_kernel(qms_handle *h, int which, const double v[3], int reps, doubl
     return QMS_OK;
     QMS_CATCH(h)
 }
+
+int qms_get_work(qms_handle *h, double *out) {
+    if (!h || !out) return QMS_ERR_ARG;
+    QMS_TRY(h)
+    check(hipMemcpyAsync(out, h->w, sizeof(double) * h->n, hipMemcpyDeviceToHost, h->s), "d2h");
+    check(hipStreamSynchronize(h->s), "sync");
+    return QMS_OK;
+    QMS_CATCH(h)
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batches: one operator launch over many designs.
+// ---------------------------------------------------------------------------
+struct qms_batch {
+    std::vector<qms_handle *> hs;
+    hipStream_t s = nullptr;
+    BatchItem *items_dev = nullptr;
+    std::vector<BatchItem> items;
+    unsigned total = 0;
+    int blocks = 0;            // persistent grid size, 0 = one workgroup per column
+    std::string err;
+    ~qms_batch() {
+        if (s) hipStreamSynchronize(s);
+        if (items_dev) hipFree(items_dev);
+        if (s) hipStreamDestroy(s);
+    }
+};
+
+namespace {
+
+int bfail(qms_batch *b, const std::exception &e, int code) {
+    if (b) b->err = e.what();
+    g_last_error = e.what();
+    return code;
+}
+
+// Work items for rows at per-design velocities v[3d..3d+2]; transient rows
+// use each design's own dt.  Waits for every handle's stream first.
+void batch_items(qms_batch *b, const double *v, int transient) {
+    b->total = 0;
+    for (size_t d = 0; d < b->hs.size(); ++d) {
+        qms_handle *h = b->hs[d];
+        check(hipStreamSynchronize(h->s), "sync");
+        const int tr = (transient && h->cfg.dt > 0) ? 1 : 0;
+        const OpParams op = make_op(h, v + 3 * d, h->cfg.dt, tr);
+        BatchItem &it = b->items[d];
+        unsigned ncol = 0;
+        it.g = h->g;
+        it.t = ztiling(h->g, h->zchunk, ncol);
+        it.c = h->c64;
+        it.k = make_opk(h->g, op);
+        it.x = h->x;
+        it.y = h->w;
+        it.col0 = b->total;
+        it.ncol = ncol;
+        if ((uint64_t)b->total + ncol > 0xffffffffull) throw std::runtime_error("batch too large");
+        b->total += ncol;
+    }
+    check(hipMemcpyAsync(b->items_dev, b->items.data(), sizeof(BatchItem) * b->items.size(), hipMemcpyHostToDevice,
+                         b->s), "h2d");
+    check(hipStreamSynchronize(b->s), "sync");
+}
+
+void batch_launch(qms_batch *b, int mode, const double *v, int transient) {
+    if (mode == 0) {
+        launch_operator_batch(b->items_dev, (int)b->items.size(), b->total, b->blocks, b->s);
+    } else {
+        for (size_t d = 0; d < b->hs.size(); ++d) {
+            qms_handle *h = b->hs[d];
+            const int tr = (transient && h->cfg.dt > 0) ? 1 : 0;
+            launch_apply(h->g, h->c64, make_op(h, v + 3 * d, h->cfg.dt, tr), h->x, h->w, b->s);
+        }
+    }
+}
+
+double time_on(hipStream_t s, int reps, const std::function<void()> &f) {
+    hipEvent_t e0, e1;
+    check(hipEventCreate(&e0), "event");
+    check(hipEventCreate(&e1), "event");
+    f();
+    check(hipEventRecord(e0, s), "event");
+    for (int r = 0; r < reps; ++r) f();
+    check(hipEventRecord(e1, s), "event");
+    check(hipEventSynchronize(e1), "event");
+    float ms = 0;
+    check(hipEventElapsedTime(&ms, e0, e1), "event");
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return ms / reps;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qms_batch_create(qms_handle *const *handles, int n, qms_batch **out) {
+    if (!handles || n < 1 || !out) { g_last_error = "qms_batch_create: no handles"; return QMS_ERR_ARG; }
+    *out = nullptr;
+    std::unique_ptr<qms_batch> b(new qms_batch());
+    try {
+        for (int d = 0; d < n; ++d) {
+            if (!handles[d]) throw std::runtime_error("qms_batch_create: null handle");
+            for (int e = 0; e < d; ++e)
+                if (handles[e] == handles[d]) throw std::runtime_error("qms_batch_create: handle listed twice");
+            b->hs.push_back(handles[d]);
+        }
+        b->items.resize(n);
+        check(hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking), "hipStreamCreate");
+        check(hipMalloc(&b->items_dev, sizeof(BatchItem) * n), "hipMalloc");
+    } catch (const HipError &e) {
+        return bfail(nullptr, e, e.code);
+    } catch (const std::exception &e) {
+        return bfail(nullptr, e, QMS_ERR_ARG);
+    }
+    *out = b.release();
+    return QMS_OK;
+}
+
+int qms_batch_set_blocks(qms_batch *b, int blocks) {
+    if (!b || blocks < 0) return QMS_ERR_ARG;
+    b->blocks = blocks;
+    return QMS_OK;
+}
+
+int qms_batch_apply(qms_batch *b, const double *v, int transient) {
+    if (!b || !v) return QMS_ERR_ARG;
+    try {
+        batch_items(b, v, transient);
+        batch_launch(b, 0, v, transient);
+        check(hipStreamSynchronize(b->s), "sync");
+    } catch (const HipError &e) {
+        return bfail(b, e, e.code);
+    } catch (const std::exception &e) {
+        return bfail(b, e, QMS_ERR_ARG);
+    }
+    return QMS_OK;
+}
+
+int qms_batch_tune(qms_batch *b, const double *v, int transient, int reps, int *zchunks, int *blocks) {
+    if (!b || !v || reps < 1) return QMS_ERR_ARG;
+    try {
+        static const int KZ[] = {2, 3, 4, 6, 8};
+        for (size_t d = 0; d < b->hs.size(); ++d) {
+            qms_handle *h = b->hs[d];
+            check(hipStreamSynchronize(h->s), "sync");
+            const int tr = (transient && h->cfg.dt > 0) ? 1 : 0;
+            double best = 1e30;
+            int bestkz = h->zchunk;
+            for (int kz : KZ) {
+                h->zchunk = kz;
+                const OpParams op = make_op(h, v + 3 * d, h->cfg.dt, tr);
+                const double ms = time_on(b->s, reps, [&]() { launch_apply(h->g, h->c64, op, h->x, h->w, b->s); });
+                if (ms < best) { best = ms; bestkz = kz; }
+            }
+            h->zchunk = bestkz;
+            if (zchunks) zchunks[d] = bestkz;
+        }
+        batch_items(b, v, transient);
+        static const int BL[] = {0, 1024, 2048, 4096, 8192};
+        double best = 1e30;
+        int bestb = 0;
+        for (int bl : BL) {
+            b->blocks = bl;
+            const double ms = time_on(b->s, reps, [&]() { batch_launch(b, 0, v, transient); });
+            if (ms < best) { best = ms; bestb = bl; }
+        }
+        b->blocks = bestb;
+        if (blocks) *blocks = bestb;
+    } catch (const HipError &e) {
+        return bfail(b, e, e.code);
+    } catch (const std::exception &e) {
+        return bfail(b, e, QMS_ERR_ARG);
+    }
+    return QMS_OK;
+}
+
+int qms_batch_time(qms_batch *b, int mode, const double *v, int transient, int reps, double *ms_per_launch,
+                   double *bytes_per_launch) {
+    if (!b || !v || reps < 1 || !ms_per_launch || !bytes_per_launch || mode < 0 || mode > 1) return QMS_ERR_ARG;
+    try {
+        batch_items(b, v, transient);
+        *ms_per_launch = time_on(b->s, reps, [&]() { batch_launch(b, mode, v, transient); });
+        double bytes = 0;
+        for (qms_handle *h : b->hs) bytes += 8.0 * (2.0 * (double)h->n + 6.0 * (double)h->g.n1);
+        *bytes_per_launch = bytes;
+    } catch (const HipError &e) {
+        return bfail(b, e, e.code);
+    } catch (const std::exception &e) {
+        return bfail(b, e, QMS_ERR_ARG);
+    }
+    return QMS_OK;
+}
+
+const char *qms_batch_last_error(const qms_batch *b) { return b ? b->err.c_str() : g_last_error.c_str(); }
+
+void qms_batch_destroy(qms_batch *b) { delete b; }
 
 void qms_destroy(qms_handle *h) { delete h; }
 

@@ -26,8 +26,10 @@ QMS_MIXED = 1
 # Every symbol include/qms.h declares.
 EXPORTS = (
     "qms_config_default", "qms_create", "qms_steady", "qms_step", "qms_run",
-    "qms_get_fields", "qms_set_fields", "qms_get_h", "qms_apply", "qms_rhs",
-    "qms_get_stats", "qms_set_zchunk", "qms_time_kernel", "qms_destroy", "qms_last_error",
+    "qms_get_fields", "qms_set_fields", "qms_get_h", "qms_apply", "qms_residual", "qms_rhs",
+    "qms_get_stats", "qms_set_zchunk", "qms_time_kernel", "qms_get_work", "qms_destroy", "qms_last_error",
+    "qms_batch_create", "qms_batch_set_blocks", "qms_batch_apply", "qms_batch_tune", "qms_batch_time",
+    "qms_batch_last_error", "qms_batch_destroy",
 )
 
 # qms_time_kernel selectors
@@ -102,12 +104,24 @@ def load(path: Optional[str] = None):
     L.qms_get_stats.argtypes = [H, ctypes.POINTER(Stats)]
     L.qms_set_zchunk.argtypes = [H, ctypes.c_int]
     L.qms_time_kernel.argtypes = [H, ctypes.c_int, d, ctypes.c_int, d, d]
+    L.qms_get_work.argtypes = [H, d]
+    L.qms_residual.argtypes = [H, d, ctypes.c_double, d, d, d]
+    L.qms_batch_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+    L.qms_batch_set_blocks.argtypes = [H, ctypes.c_int]
+    L.qms_batch_apply.argtypes = [H, d, ctypes.c_int]
+    L.qms_batch_tune.argtypes = [H, d, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                 ctypes.POINTER(ctypes.c_int)]
+    L.qms_batch_time.argtypes = [H, ctypes.c_int, d, ctypes.c_int, ctypes.c_int, d, d]
+    L.qms_batch_last_error.argtypes = [H]
+    L.qms_batch_last_error.restype = ctypes.c_char_p
+    L.qms_batch_destroy.argtypes = [H]
+    L.qms_batch_destroy.restype = None
     L.qms_destroy.argtypes = [H]
     L.qms_destroy.restype = None
     L.qms_last_error.argtypes = [H]
     L.qms_last_error.restype = ctypes.c_char_p
     for name in EXPORTS:
-        if name not in ("qms_destroy", "qms_last_error"):
+        if name not in ("qms_destroy", "qms_last_error", "qms_batch_destroy", "qms_batch_last_error"):
             getattr(L, name).restype = ctypes.c_int
     if path is None:
         _lib = L
@@ -151,6 +165,7 @@ class Solver:
         cfg.mg_alpha = float(mg_alpha)
         self.cells = tuple(int(c) for c in cells)
         self.nodes = tuple(c + 1 for c in self.cells)
+        self.dt = float(dt)
         slot = np.ascontiguousarray(cell_slot, dtype=np.uint16).ravel()
         if slot.size != np.prod(self.cells):
             raise ValueError("cell_slot must hold cells_x*cells_y*cells_z entries")
@@ -230,6 +245,15 @@ class Solver:
         self._check(self._L.qms_apply(self._h, _dp(vv), float(dt), _dp(xx), _dp(y)))
         return y
 
+    def residual(self, v, dt: float, x: np.ndarray, b: np.ndarray) -> np.ndarray:
+        """D^-1 (b - M x) on free rows, b - x on fixed rows."""
+        vv = np.ascontiguousarray(v, dtype=np.float64)
+        xx = np.ascontiguousarray(x, dtype=np.float64)
+        bb = np.ascontiguousarray(b, dtype=np.float64)
+        r = np.empty_like(xx)
+        self._check(self._L.qms_residual(self._h, _dp(vv), float(dt), _dp(xx), _dp(bb), _dp(r)))
+        return r
+
     def rhs(self, v, dt: float, aprev: np.ndarray) -> np.ndarray:
         vv = np.ascontiguousarray(v, dtype=np.float64)
         a = np.ascontiguousarray(aprev, dtype=np.float64)
@@ -258,3 +282,74 @@ class Solver:
             vp = _dp(vv)
         self._check(self._L.qms_time_kernel(self._h, which, vp, reps, ctypes.byref(ms), ctypes.byref(by)))
         return ms.value, by.value
+
+    def get_work(self) -> np.ndarray:
+        """The last operator output D^-1 M x (4 node-sized arrays)."""
+        nx, ny, nz = self.nodes
+        out = np.empty((4, nz, ny, nx), dtype=np.float64)
+        self._check(self._L.qms_get_work(self._h, _dp(out)))
+        return out
+
+
+class Batch:
+    """Many designs in one persistent operator launch (a ``qms_batch``).
+    The solvers must stay alive and idle while the batch is used."""
+
+    def __init__(self, solvers: Sequence[Solver]):
+        L = load()
+        self.solvers = list(solvers)
+        arr = (ctypes.c_void_p * len(self.solvers))(*[s._h for s in self.solvers])
+        out = ctypes.c_void_p()
+        rc = L.qms_batch_create(arr, len(self.solvers), ctypes.byref(out))
+        if rc != QMS_OK:
+            raise QmsError(rc, L.qms_batch_last_error(None).decode())
+        self._b = out
+        self._L = L
+
+    def _check(self, rc: int):
+        if rc != QMS_OK:
+            raise QmsError(rc, self._L.qms_batch_last_error(self._b).decode())
+
+    def _v(self, v):
+        vv = np.ascontiguousarray(np.broadcast_to(np.asarray(v, dtype=np.float64), (len(self.solvers), 3)))
+        return vv
+
+    def set_blocks(self, blocks: int):
+        self._check(self._L.qms_batch_set_blocks(self._b, int(blocks)))
+
+    def apply(self, v, transient: bool = True):
+        """Operator output of every design into its solver's work vector."""
+        vv = self._v(v)
+        self._check(self._L.qms_batch_apply(self._b, _dp(vv), 1 if transient else 0))
+
+    def tune(self, v, transient: bool = True, reps: int = 10):
+        vv = self._v(v)
+        kz = (ctypes.c_int * len(self.solvers))()
+        bl = ctypes.c_int()
+        self._check(self._L.qms_batch_tune(self._b, _dp(vv), 1 if transient else 0, int(reps), kz, ctypes.byref(bl)))
+        return list(kz), bl.value
+
+    def time(self, v, transient: bool = True, reps: int = 20, sequential: bool = False):
+        vv = self._v(v)
+        ms = ctypes.c_double()
+        by = ctypes.c_double()
+        self._check(self._L.qms_batch_time(self._b, 1 if sequential else 0, _dp(vv), 1 if transient else 0,
+                                           int(reps), ctypes.byref(ms), ctypes.byref(by)))
+        return ms.value, by.value
+
+    def close(self):
+        if getattr(self, "_b", None):
+            self._L.qms_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

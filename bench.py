@@ -15,12 +15,15 @@ operator inside every Krylov iteration) with the rows of the last ramp step
 (transient, v = 5 m/s), against the HBM roofline, with HIP events on the
 stream it is launched on; its z-chunk depth (planes per workgroup) is
 picked from a sweep over 2..8.  Beside it: the FP64 residual, the step right-hand side, the
-cell-centred H and a plain copy kernel of the same state (calibration).  The CPU baseline is the oracle (CSR assembly +
+cell-centred H and a plain copy kernel of the same state (calibration).
+The `batch` object measures the config-4 workload (64 uneven designs): the
+operator of every design in one persistent batched launch against one
+launch per design.  The CPU baseline is the oracle (CSR assembly +
 equilibrated SuperLU factor/solve, the restated reference algorithm) timed
 on one core over one ramp step of a 24^3-node design drawn from the same
 box distribution.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--zchunk KZ] [--no-cpu-baseline]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--zchunk KZ] [--no-cpu-baseline] [--no-batch]
 """
 import argparse
 import json
@@ -78,6 +81,41 @@ def traffic_from_profile(kernel: str):
     return t * 1024.0, os.path.relpath(path, ROOT)
 
 
+def batch_config4(reps: int, ndesigns: int = 64):
+    """Config 4: 64 independent designs (24..96 nodes per axis, 4-12 boxes
+    each), operator rows of a transient step at a random frame velocity per
+    design, applied in one persistent batched launch (z-chunks and grid size
+    from the autotuner) and, for comparison, as one launch per design."""
+    import numpy as np
+    from alpineqms_amd import configs, native
+    from alpineqms_amd.qms_io import material_slots
+    ds = configs.config4_designs(ndesigns)
+    rng = np.random.default_rng(4)
+    ss = []
+    for d in ds:
+        slots, props = material_slots(d.inputs, d.geom)
+        s = native.Solver(d.inputs.cells, d.inputs.h, d.inputs.dt, d.inputs.bc_vector(), slots, props, restart=1)
+        nx, ny, nz = s.nodes
+        s.set_fields(rng.normal(size=(4, nz, ny, nx)))
+        ss.append(s)
+    v = rng.normal(size=(len(ss), 3))
+    dof = sum(d.dof for d in ds)
+    print(f"bench: config 4 batch of {len(ds)} designs, {dof} DoF", file=sys.stderr, flush=True)
+    with native.Batch(ss) as b:
+        seq0, _ = b.time(v, reps=reps, sequential=True)          # default z-chunk, one launch per design
+        kz, blocks = b.tune(v, reps=10)
+        ms_b, nbytes = b.time(v, reps=reps)
+        ms_s, _ = b.time(v, reps=reps, sequential=True)
+    for s in ss:
+        s.close()
+    gbs = nbytes / (ms_b * 1e-3) / 1e9
+    return {"workload": f"config 4: {len(ds)} designs, 24-96 nodes per axis, {dof} DoF, transient rows",
+            "dof": dof, "algorithmic_bytes": nbytes, "zchunks": kz, "blocks": blocks,
+            "batched_ms": ms_b, "sequential_ms": ms_s, "sequential_default_zchunk_ms": seq0,
+            "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS, "dof_applies_per_s": dof / (ms_b * 1e-3),
+            "speedup_vs_sequential": ms_s / ms_b}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +125,7 @@ def main():
     ap.add_argument("--zchunk", type=int, default=0,
                     help="planes per operator workgroup (0: pick the fastest of 2..8)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the config-4 batched operator measurement")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py runs on one GPU only: every design here fits one MI355X, and the multi-GPU "
@@ -160,6 +199,7 @@ def main():
                      "algorithmic_bytes": nbytes, "ms_per_launch": ms, "launches": a.steps},
         "zchunk_sweep_ms": {str(t): m for t, m in sweep.items()},
         "kernels": others,
+        "batch": None if a.no_batch else batch_config4(a.steps),
         "cpu_baseline": None if a.no_cpu_baseline else cpu_baseline(),
     }
     print(json.dumps(line), flush=True)

@@ -111,6 +111,12 @@ int qms_apply(qms_handle *h, const double v[3], double dt, const double *x, doub
  * node-sized arrays, QMS_3D.f90:459-470, :1475, :1624, :1772, :1906-1928. */
 int qms_rhs(qms_handle *h, const double v[3], double dt, const double *aprev, double *b);
 
+/* r = D^-1 (b - M(v, dt) x) on free rows, b - x on fixed rows, 0 where no
+ * unknown exists (D = diag(M)): the FP64 residual the Krylov solver restarts
+ * from, i.e. the row-scaled residual of the reference's system
+ * (QMS_3D.f90:1314-1317, :2193-2203), on 4 node-sized arrays. */
+int qms_residual(qms_handle *h, const double v[3], double dt, const double *x, const double *b, double *r);
+
 int qms_get_stats(const qms_handle *h, qms_stats *st);
 
 /* Planes per operator workgroup (1..64, default 4): each 256-thread
@@ -130,6 +136,37 @@ int qms_set_zchunk(qms_handle *h, int planes);
  * No reference counterpart (measurement only). */
 int qms_time_kernel(qms_handle *h, int which, const double v[3], int reps, double *ms_per_launch,
                     double *bytes_per_launch);
+
+/* Copy the handle's work vector (4 node-sized arrays) to the host: the last
+ * operator output D^-1 M x written by qms_apply, qms_time_kernel or
+ * qms_batch_apply, where D = diag(M) and fixed rows are identity rows. */
+int qms_get_work(qms_handle *h, double *out);
+
+/* ---- Batches: many independent designs in one persistent operator launch.
+ * No counterpart in the reference (one design per process, QMS_3D.f90:117);
+ * the building block of stepping many designs together.  A batch borrows n
+ * existing handles (each must stay alive and idle while the batch is used)
+ * and owns one HIP stream. */
+typedef struct qms_batch qms_batch;
+int qms_batch_create(qms_handle *const *handles, int n, qms_batch **out);
+/* Persistent grid size of the batched launch (0: one workgroup per column). */
+int qms_batch_set_blocks(qms_batch *b, int blocks);
+/* w_d = D_d^-1 M_d(v_d) x_d for every design d (x_d: the handle's state,
+ * v: 3n velocities; transient != 0: transient rows with each design's dt).
+ * Results land in each handle's work vector (qms_get_work), identical bit
+ * for bit to a single-design qms_apply of the same rows. */
+int qms_batch_apply(qms_batch *b, const double *v, int transient);
+/* Autotuner: times each design's operator at z-chunks {2,3,4,6,8} and keeps
+ * the fastest (qms_set_zchunk), then times the batched launch at grid sizes
+ * {per column, 1024, 2048, 4096, 8192} and keeps the fastest.  zchunks[n]
+ * and blocks (either may be NULL) receive the choices. */
+int qms_batch_tune(qms_batch *b, const double *v, int transient, int reps, int *zchunks, int *blocks);
+/* Mean time of one batched launch (mode 0) or of the n single-design
+ * launches it replaces (mode 1), and the algorithmic bytes they move. */
+int qms_batch_time(qms_batch *b, int mode, const double *v, int transient, int reps, double *ms_per_launch,
+                   double *bytes_per_launch);
+const char *qms_batch_last_error(const qms_batch *b);
+void qms_batch_destroy(qms_batch *b);
 
 void qms_destroy(qms_handle *h);
 

@@ -98,6 +98,30 @@ def test_operator_full_size_properties():
     assert np.max(np.abs(y12 - lin)) <= 1e-12 * np.max(np.abs(lin))
 
 
+@pytest.mark.parametrize("name,mk", CASES, ids=[c[0] for c in CASES])
+def test_residual_matches_oracle(name, mk):
+    """The FP64 residual kernel: D^-1 (b - M x) on free rows, b - x on fixed
+    rows, against the oracle's CSR with b the oracle's RHS plus noise."""
+    d = mk()
+    p = problem(d)
+    coef = O.coeffs(p)
+    rng = np.random.default_rng(29)
+    v = np.array([-1.2, 0.8, 2.5])
+    aprev = rng.normal(size=(4,) + coef.shape[1:])
+    for dt in (0.0, p.dt):
+        M, b = O.assemble(p, coef, v, dt, aprev)
+        x = rng.normal(size=M.shape[0])
+        b = b + rng.normal(size=b.size) * np.abs(M).sum(axis=1).A1
+        Dg = M.diagonal()
+        ref = (b - M @ x) / Dg
+        scale = (np.abs(b) + np.abs(M) @ np.abs(x)) / np.abs(Dg)
+        with solver_for(d) as s:
+            for kz in (4, 1):
+                s.set_zchunk(kz)
+                r = O.pack(p, s.residual(v, dt, O.unpack(p, x), O.unpack(p, b)))
+                assert np.max(np.abs(r - ref) / scale) < OP_TOL, (dt, kz)
+
+
 def test_time_kernel_selectors():
     d = configs.small_box_design(16, 3, nbox=2, v=1.0, ramp=False)
     with solver_for(d) as s:
