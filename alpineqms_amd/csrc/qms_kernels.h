@@ -28,6 +28,7 @@ struct OpParams {
     int tr;        // 1: transient rows T(v,dt), 0: steady rows S(v)
     int adv;       // 1 if any velocity component is non-zero
     int zchunk;    // planes per operator workgroup (z-march depth)
+    int colw;      // operator column width in nodes (64 or 32)
 };
 
 // Per-launch constants of the operator rows, computed once on the host so
@@ -44,14 +45,16 @@ struct OpK {
     double v1[3][3];             // v_a / (2 h_b)
     double w[12];                // v_a / (2 h_b h_c) for the (a,b,c) of the V row's advection terms
     double v[3];                 // frame velocity (Dirichlet values of V)
+    double hv[3];                // v_a / 2 (edge average of the advection terms)
 };
 OpK make_opk(const Grid &g, const OpParams &op);
 
-// Workgroup columns of the z-marching operator: 64 x 4 nodes, kz planes.
+// Workgroup columns of the z-marching operator: tx x (256/tx) nodes
+// (tx = 64 or 32), kz planes.
 struct ZTiling {
-    int ntx, nty, kz;
+    int ntx, nty, kz, tx;
 };
-ZTiling ztiling(const Grid &g, int kz, unsigned &ncol);
+ZTiling ztiling(const Grid &g, int kz, int tx, unsigned &ncol);
 
 // One design of a batched operator launch (device resident).
 struct BatchItem {

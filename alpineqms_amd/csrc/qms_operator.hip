@@ -267,6 +267,102 @@ __device__ __forceinline__ double row_v(const XV &x, const CV &c, const OpK &k, 
 }
 
 // ---------------------------------------------------------------------------
+// All four rows of one node in flux form: the same operator as row_ax..row_v,
+// regrouped around the Yee grid's face and edge quantities so that each is
+// formed once and shared between rows:
+//   B  = curl A on the 12 faces around the node (z-faces at (0,0),(0,-1),
+//        (-1,0),(-1,-1) in (i,j); y-faces and x-faces likewise),
+//   F  = B / Mu on those faces      -> curl-curl part of the A rows,
+//   P  = (1/Mu_a) dA_a/da           -> gauge-penalty part of the A rows,
+//   E  = sigma (v x avg B) on edges -> advection of the A rows and, through
+//        G^T, of the V row (QMS_3D.f90:546-592, :1003-1285),
+//   Phi = max(sigma, floor) dV/da   -> Laplacian of the V row.
+// A row:  curl(F) - div(P)_a + sigma A/dt + sigma dV/da + E     (+ transient)
+// V row:  sum_a [ (sigma A/dt)_a - Phi_a - E_a ]_(-a)^(0) / h_a
+// Every term is a difference of neighbouring values, as in the row form.
+// ---------------------------------------------------------------------------
+template <typename XV, typename CV>
+__device__ __forceinline__ void rows_flux(const XV &x, const CV &c, const OpK &k, double r[4], double d[4]) {
+    const double ihx = k.ih[0], ihy = k.ih[1], ihz = k.ih[2];
+    // face curls; Bz(di,dj) at plane k, By(di,dk) at row j, Bx(dj,dk) at column i
+    auto Bz = [&](int di, int dj) {
+        return (x.AY(di + 1, dj, 0) - x.AY(di, dj, 0)) * ihx - (x.AX(di, dj + 1, 0) - x.AX(di, dj, 0)) * ihy;
+    };
+    auto By = [&](int di, int dk) {
+        return (x.AX(di, 0, dk + 1) - x.AX(di, 0, dk)) * ihz - (x.AZ(di + 1, 0, dk) - x.AZ(di, 0, dk)) * ihx;
+    };
+    auto Bx = [&](int dj, int dk) {
+        return (x.AZ(0, dj + 1, dk) - x.AZ(0, dj, dk)) * ihy - (x.AY(0, dj, dk + 1) - x.AY(0, dj, dk)) * ihz;
+    };
+    const double bz00 = Bz(0, 0), bz0m = Bz(0, -1), bzm0 = Bz(-1, 0);
+    const double by00 = By(0, 0), by0m = By(0, -1), bym0 = By(-1, 0);
+    const double bx00 = Bx(0, 0), bx0m = Bx(0, -1), bxm0 = Bx(-1, 0);
+    const double imx0 = c.IMX(0, 0, 0), imy0 = c.IMY(0, 0, 0), imz0 = c.IMZ(0, 0, 0);
+    const double imx_y = c.IMX(0, -1, 0), imx_z = c.IMX(0, 0, -1), imx_x = c.IMX(-1, 0, 0);
+    const double imy_x = c.IMY(-1, 0, 0), imy_z = c.IMY(0, 0, -1), imy_y = c.IMY(0, -1, 0);
+    const double imz_x = c.IMZ(-1, 0, 0), imz_y = c.IMZ(0, -1, 0), imz_z = c.IMZ(0, 0, -1);
+    const double sx0 = c.SX(0, 0, 0), sy0 = c.SY(0, 0, 0), sz0 = c.SZ(0, 0, 0);
+    const double sx1 = c.SX(-1, 0, 0), sy1 = c.SY(0, -1, 0), sz1 = c.SZ(0, 0, -1);
+    // curl-curl
+    const double fz0 = imz0 * bz00, fzy = imz_y * bz0m, fzx = imz_x * bzm0;
+    const double fy0 = imy0 * by00, fyz = imy_z * by0m, fyx = imy_x * bym0;
+    const double fx0 = imx0 * bx00, fxz = imx_z * bx0m, fxy = imx_y * bxm0;
+    double rax = (fz0 - fzy) * ihy - (fy0 - fyz) * ihz;
+    double ray = (fx0 - fxz) * ihz - (fz0 - fzx) * ihx;
+    double raz = (fy0 - fyx) * ihx - (fx0 - fxy) * ihy;
+    // gauge penalty
+    const double ax0 = x.AX(0, 0, 0), ay0 = x.AY(0, 0, 0), az0 = x.AZ(0, 0, 0), v0 = x.VV(0, 0, 0);
+    const double axm = x.AX(-1, 0, 0), aym = x.AY(0, -1, 0), azm = x.AZ(0, 0, -1);
+    rax -= (imx0 * (x.AX(1, 0, 0) - ax0) * ihx - imx_x * (ax0 - axm) * ihx) * ihx;
+    ray -= (imy0 * (x.AY(0, 1, 0) - ay0) * ihy - imy_y * (ay0 - aym) * ihy) * ihy;
+    raz -= (imz0 * (x.AZ(0, 0, 1) - az0) * ihz - imz_z * (az0 - azm) * ihz) * ihz;
+    // sigma grad V on the A rows
+    const double vxp = x.VV(1, 0, 0), vyp = x.VV(0, 1, 0), vzp = x.VV(0, 0, 1);
+    const double vxm = x.VV(-1, 0, 0), vym = x.VV(0, -1, 0), vzm = x.VV(0, 0, -1);
+    rax += sx0 * (vxp - v0) * ihx;
+    ray += sy0 * (vyp - v0) * ihy;
+    raz += sz0 * (vzp - v0) * ihz;
+    // V row: floored-sigma Laplacian
+    const double fx0f = fmax(sx0, QMS_SIGFLOOR), fx1f = fmax(sx1, QMS_SIGFLOOR);
+    const double fy0f = fmax(sy0, QMS_SIGFLOOR), fy1f = fmax(sy1, QMS_SIGFLOOR);
+    const double fz0f = fmax(sz0, QMS_SIGFLOOR), fz1f = fmax(sz1, QMS_SIGFLOOR);
+    double rv = -((fx0f * (vxp - v0) * ihx - fx1f * (v0 - vxm) * ihx) * ihx
+                + (fy0f * (vyp - v0) * ihy - fy1f * (v0 - vym) * ihy) * ihy
+                + (fz0f * (vzp - v0) * ihz - fz1f * (v0 - vzm) * ihz) * ihz);
+    double dax = (imz0 + imz_y) * k.ih2[1] + (imy0 + imy_z) * k.ih2[2] + (imx0 + imx_x) * k.ih2[0];
+    double day = (imz0 + imz_x) * k.ih2[0] + (imx0 + imx_z) * k.ih2[2] + (imy0 + imy_y) * k.ih2[1];
+    double daz = (imy0 + imy_x) * k.ih2[0] + (imx0 + imx_y) * k.ih2[1] + (imz0 + imz_z) * k.ih2[2];
+    d[3] = (fx0f + fx1f) * k.ih2[0] + (fy0f + fy1f) * k.ih2[1] + (fz0f + fz1f) * k.ih2[2];
+    if (k.tr) {
+        dax = dax + sx0 * k.idt;
+        day = day + sy0 * k.idt;
+        daz = daz + sz0 * k.idt;
+        rax += (sx0 * k.idt) * ax0;
+        ray += (sy0 * k.idt) * ay0;
+        raz += (sz0 * k.idt) * az0;
+        rv += (sx0 * ax0 - sx1 * axm) * k.iht[0] + (sy0 * ay0 - sy1 * aym) * k.iht[1]
+            + (sz0 * az0 - sz1 * azm) * k.iht[2];
+    }
+    if (k.adv) {
+        const double hvx = k.hv[0], hvy = k.hv[1], hvz = k.hv[2];
+        const double bzmm = Bz(-1, -1), bymm = By(-1, -1), bxmm = Bx(-1, -1);
+        // sigma (v x avg B) on the node's own edges and on the -x, -y, -z edges
+        const double ex0 = sx0 * (hvy * (bz00 + bz0m) - hvz * (by00 + by0m));
+        const double ey0 = sy0 * (hvz * (bx00 + bx0m) - hvx * (bz00 + bzm0));
+        const double ez0 = sz0 * (hvx * (by00 + bym0) - hvy * (bx00 + bxm0));
+        const double exm = sx1 * (hvy * (bzm0 + bzmm) - hvz * (bym0 + bymm));
+        const double eym = sy1 * (hvz * (bxm0 + bxmm) - hvx * (bz0m + bzmm));
+        const double ezm = sz1 * (hvx * (by0m + bymm) - hvy * (bx0m + bxmm));
+        rax += ex0;
+        ray += ey0;
+        raz += ez0;
+        rv -= (ex0 - exm) * ihx + (ey0 - eym) * ihy + (ez0 - ezm) * ihz;
+    }
+    r[0] = rax; r[1] = ray; r[2] = raz; r[3] = rv;
+    d[0] = dax; d[1] = day; d[2] = daz;
+}
+
+// ---------------------------------------------------------------------------
 // Tiled launch geometry.  Tiles are numbered x-fastest, then y, then z, and
 // the hardware workgroup index is remapped so that each of the 8 XCDs (which
 // receive workgroups round-robin) works on one contiguous range of tiles:
@@ -295,23 +391,29 @@ __device__ __forceinline__ bool tile_node(const Grid &g, const Tiling &t, int &i
 }
 
 // ---------------------------------------------------------------------------
-// z-marching operator.  A workgroup owns a 64 x 4 column of nodes and walks
-// it through kz consecutive planes.  The four fields of planes k-1, k, k+1
+// z-marching operator.  A workgroup owns a TX x (256/TX) column of nodes
+// (TX = 64 or 32) and walks it through kz consecutive planes.  The four fields of planes k-1, k, k+1
 // (with a one-node halo in x and y) sit in an LDS ring of three slots, so
 // each field value is read from HBM once per column instead of once per
 // stencil use; the material coefficients of plane k are read per node (their
 // -x and -y neighbours hit L1), and the plane-(k-1) values they need are
 // carried in registers from the previous plane.
 // ---------------------------------------------------------------------------
-constexpr int ZTX = 64, ZTY = 4, ZSY = ZTX + 2, ZPL = ZSY * (ZTY + 2);
+template <int TX>
+struct ZShape {
+    static constexpr int TY = 256 / TX;              // rows per column
+    static constexpr int SY = TX + 2;                // LDS row stride (one-node halo each side)
+    static constexpr int PL = SY * (TY + 2);         // LDS plane size
+};
+constexpr int ZPL_MAX = ZShape<64>::PL > ZShape<32>::PL ? ZShape<64>::PL : ZShape<32>::PL;
 
-
+template <int TX>
 struct LdsView {
-    const double *lds;     // [field][slot][ZPL]
+    const double *lds;     // [field][slot][PL]
     int b[3];              // slot offsets (elements) of planes k-1, k, k+1
     int ctr;               // this node's index within a plane slot
     __device__ __forceinline__ double get(int f, int di, int dj, int dk) const {
-        return lds[f * 3 * ZPL + b[dk + 1] + ctr + di + dj * ZSY];
+        return lds[f * 3 * ZShape<TX>::PL + b[dk + 1] + ctr + di + dj * ZShape<TX>::SY];
     }
     __device__ __forceinline__ double AX(int di, int dj, int dk) const { return get(0, di, dj, dk); }
     __device__ __forceinline__ double AY(int di, int dj, int dk) const { return get(1, di, dj, dk); }
@@ -336,31 +438,33 @@ struct ZmCView {
 };
 
 // One workgroup column `tid` of one design.  Every thread of the workgroup
-// must call this (it synchronises the workgroup); `lds` holds 4*3*ZPL doubles.
-template <typename TIN, typename TOUT, int MODE>
+// must call this (it synchronises the workgroup); `lds` holds 4*3*ZPL_MAX
+// doubles.
+template <typename TIN, typename TOUT, int MODE, int TX>
 __device__ __forceinline__ void op_column(const Grid &g, const ZTiling &t, unsigned tid, const CoeffsD &c,
                                           const OpK &k, const TIN *__restrict__ x, const double *__restrict__ b,
                                           TOUT *__restrict__ out, double *lds) {
     const unsigned tx = tid % (unsigned)t.ntx, r1 = tid / (unsigned)t.ntx;
     const unsigned ty = r1 % (unsigned)t.nty, tz = r1 / (unsigned)t.nty;
-    const int i0 = (int)tx * ZTX, j0 = (int)ty * ZTY;
+    using S = ZShape<TX>;
+    const int i0 = (int)tx * TX, j0 = (int)ty * S::TY;
     const int kb = (int)tz * t.kz, ke = min(kb + t.kz, g.nz);
-    const int li = (int)(threadIdx.x & 63), lj = (int)(threadIdx.x >> 6);
+    const int li = (int)(threadIdx.x % TX), lj = (int)(threadIdx.x / TX);
     const int i = i0 + li, j = j0 + lj;
     const int ic = min(i, g.nx - 1), jc = min(j, g.ny - 1);
     const bool live = i < g.nx && j < g.ny;
-    const int ctr = (lj + 1) * ZSY + (li + 1);
+    const int ctr = (lj + 1) * S::SY + (li + 1);
     const int64_t n1 = g.n1;
     const int64_t plane = (int64_t)g.nx * g.ny;
 
     auto stage = [&](int kp, int slot) {
         kp = max(0, min(kp, g.nz - 1));
-        for (int e = (int)threadIdx.x; e < ZPL; e += 256) {
-            const int ei = e % ZSY, ej = e / ZSY;
+        for (int e = (int)threadIdx.x; e < S::PL; e += 256) {
+            const int ei = e % S::SY, ej = e / S::SY;
             const int gi = max(0, min(i0 - 1 + ei, g.nx - 1)), gj = max(0, min(j0 - 1 + ej, g.ny - 1));
             const int64_t q = gi + (int64_t)g.nx * gj + plane * kp;
 #pragma unroll
-            for (int f = 0; f < 4; ++f) lds[(f * 3 + slot) * ZPL + e] = (double)x[f * n1 + q];
+            for (int f = 0; f < 4; ++f) lds[(f * 3 + slot) * S::PL + e] = (double)x[f * n1 + q];
         }
     };
 
@@ -382,18 +486,17 @@ __device__ __forceinline__ void op_column(const Grid &g, const ZTiling &t, unsig
         const ZmCView cv{c.im[0][p], c.im[1][p], c.im[2][p], c.sg[0][p], c.sg[1][p], c.sg[2][p],
                          imxz, imyz, imzz, szz,
                          c.im[0] + p, c.im[1] + p, c.im[2] + p, c.sg[0] + p, c.sg[1] + p, dxm, dym};
-        const LdsView xv{lds, {sm * ZPL, s0 * ZPL, sp * ZPL}, ctr};
+        const LdsView<TX> xv{lds, {sm * S::PL, s0 * S::PL, sp * S::PL}, ctr};
         if (MODE == OP_VA) {
             if (live && free_v(g, i, j, kk)) {
                 double d;
                 out[3 * n1 + p] = (TOUT)row_v(xv, cv, k, B_VA, d);
             }
         } else {
-            double dax, day, daz, dv;
-            const double rax = row_ax(xv, cv, k, B_ALL, dax);
-            const double ray = row_ay(xv, cv, k, B_ALL, day);
-            const double raz = row_az(xv, cv, k, B_ALL, daz);
-            const double rv = row_v(xv, cv, k, B_ALL, dv);
+            double rr4[4], dd4[4];
+            rows_flux(xv, cv, k, rr4, dd4);
+            const double rax = rr4[0], ray = rr4[1], raz = rr4[2], rv = rr4[3];
+            const double dax = dd4[0], day = dd4[1], daz = dd4[2], dv = dd4[3];
             if (live) {
                 auto emit = [&](int f, double mx, double d) {
                     const int64_t q = f * n1 + p;
@@ -419,11 +522,11 @@ __device__ __forceinline__ void op_column(const Grid &g, const ZTiling &t, unsig
     }
 }
 
-template <typename TIN, typename TOUT, int MODE>
+template <typename TIN, typename TOUT, int MODE, int TX>
 __global__ void __launch_bounds__(256) k_operator(Grid g, ZTiling t, CoeffsD c, OpK k, const TIN *__restrict__ x,
                                                   const double *__restrict__ b, TOUT *__restrict__ out) {
-    __shared__ double lds[4 * 3 * ZPL];
-    op_column<TIN, TOUT, MODE>(g, t, xcd_tile(blockIdx.x, gridDim.x), c, k, x, b, out, lds);
+    __shared__ double lds[4 * 3 * ZShape<TX>::PL];
+    op_column<TIN, TOUT, MODE, TX>(g, t, xcd_tile(blockIdx.x, gridDim.x), c, k, x, b, out, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -438,14 +541,17 @@ __global__ void __launch_bounds__(256) k_operator(Grid g, ZTiling t, CoeffsD c, 
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_operator_batch(const BatchItem *__restrict__ items, int nitems,
                                                         unsigned total) {
-    __shared__ double lds[4 * 3 * ZPL];
+    __shared__ double lds[4 * 3 * ZPL_MAX];
     const unsigned xcd = blockIdx.x & 7u, per = gridDim.x >> 3, l = blockIdx.x >> 3;
     const unsigned lo = (unsigned)(((uint64_t)total * xcd) >> 3), hi = (unsigned)(((uint64_t)total * (xcd + 1)) >> 3);
     int d = 0;
     for (unsigned w = lo + l; w < hi; w += per) {
         while (d + 1 < nitems && items[d + 1].col0 <= w) ++d;     // w only grows: forward scan
         const BatchItem &it = items[d];
-        op_column<double, double, OP_APPLY>(it.g, it.t, w - it.col0, it.c, it.k, it.x, nullptr, it.y, lds);
+        if (it.t.tx == 32)
+            op_column<double, double, OP_APPLY, 32>(it.g, it.t, w - it.col0, it.c, it.k, it.x, nullptr, it.y, lds);
+        else
+            op_column<double, double, OP_APPLY, 64>(it.g, it.t, w - it.col0, it.c, it.k, it.x, nullptr, it.y, lds);
     }
 }
 
@@ -564,6 +670,7 @@ OpK make_opk(const Grid &g, const OpParams &op) {
     k.idt = op.tr ? 1.0 / op.dt : 0.0;
     for (int a = 0; a < 3; ++a) {
         k.v[a] = op.v[a];
+        k.hv[a] = 0.5 * op.v[a];
         k.ih2[a] = 1.0 / (h[a] * h[a]);
         k.ih[a] = 1.0 / h[a];
         k.iht[a] = op.tr ? 1.0 / (h[a] * op.dt) : 0.0;
@@ -589,11 +696,12 @@ static Tiling rhs_tiling(const Grid &g, dim3 &grid) {
     return t;
 }
 
-ZTiling ztiling(const Grid &g, int kz, unsigned &ncol) {
+ZTiling ztiling(const Grid &g, int kz, int tx, unsigned &ncol) {
     ZTiling t;
     t.kz = kz >= 1 ? kz : 4;
-    t.ntx = (g.nx + ZTX - 1) / ZTX;
-    t.nty = (g.ny + ZTY - 1) / ZTY;
+    t.tx = tx == 32 ? 32 : 64;
+    t.ntx = (g.nx + t.tx - 1) / t.tx;
+    t.nty = (g.ny + 256 / t.tx - 1) / (256 / t.tx);
     ncol = (unsigned)((int64_t)t.ntx * t.nty * ((g.nz + t.kz - 1) / t.kz));
     return t;
 }
@@ -602,8 +710,13 @@ template <typename TIN, typename TOUT, int MODE>
 static void launch_operator(const Grid &g, const CoeffsD &c, const OpParams &op, const TIN *x, const double *b,
                             TOUT *out, hipStream_t s) {
     unsigned ncol;
-    const ZTiling t = ztiling(g, op.zchunk, ncol);
-    hipLaunchKernelGGL((k_operator<TIN, TOUT, MODE>), dim3(ncol), dim3(256), 0, s, g, t, c, make_opk(g, op), x, b, out);
+    const ZTiling t = ztiling(g, op.zchunk, op.colw, ncol);
+    if (t.tx == 32)
+        hipLaunchKernelGGL((k_operator<TIN, TOUT, MODE, 32>), dim3(ncol), dim3(256), 0, s, g, t, c, make_opk(g, op), x, b,
+                           out);
+    else
+        hipLaunchKernelGGL((k_operator<TIN, TOUT, MODE, 64>), dim3(ncol), dim3(256), 0, s, g, t, c, make_opk(g, op), x, b,
+                           out);
 }
 
 // blocks: persistent grid size (rounded up to a multiple of 8 XCDs), or 0 for

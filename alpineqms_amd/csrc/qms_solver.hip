@@ -62,6 +62,7 @@ struct qms_handle {
     double *hsmall = nullptr;   // pinned
     int nblk = 1;
     int zchunk = 4;             // planes per operator workgroup
+    int colw = 64;              // operator column width (nodes)
 
     // preconditioner work vectors (4*n1 in MG precision)
     void *rT = nullptr, *zT = nullptr, *tT = nullptr;
@@ -145,6 +146,7 @@ OpParams make_op(const qms_handle *h, const double v[3], double dt, int tr) {
     op.tr = tr;
     op.adv = (v[0] != 0.0 || v[1] != 0.0 || v[2] != 0.0) ? 1 : 0;
     op.zchunk = h->zchunk;
+    op.colw = h->colw;
     return op;
 }
 
@@ -559,6 +561,16 @@ int qms_set_zchunk(qms_handle *h, int planes) {
     return QMS_OK;
 }
 
+int qms_set_colwidth(qms_handle *h, int nodes) {
+    if (!h) return QMS_ERR_ARG;
+    if (nodes != 32 && nodes != 64) {
+        h->err = "qms_set_colwidth: column width must be 32 or 64 nodes";
+        return QMS_ERR_ARG;
+    }
+    h->colw = nodes;
+    return QMS_OK;
+}
+
 int qms_time_kernel(qms_handle *h, int which, const double v[3], int reps, double *ms_per_launch,
                     double *bytes_per_launch) {
     if (!h || reps < 1 || !ms_per_launch || !bytes_per_launch || which < 0 || which > 6) return QMS_ERR_ARG;
@@ -674,7 +686,7 @@ void batch_items(qms_batch *b, const double *v, int transient) {
         BatchItem &it = b->items[d];
         unsigned ncol = 0;
         it.g = h->g;
-        it.t = ztiling(h->g, h->zchunk, ncol);
+        it.t = ztiling(h->g, h->zchunk, h->colw, ncol);
         it.c = h->c64;
         it.k = make_opk(h->g, op);
         it.x = h->x;
@@ -764,24 +776,32 @@ int qms_batch_apply(qms_batch *b, const double *v, int transient) {
     return QMS_OK;
 }
 
-int qms_batch_tune(qms_batch *b, const double *v, int transient, int reps, int *zchunks, int *blocks) {
+int qms_batch_tune(qms_batch *b, const double *v, int transient, int reps, int *zchunks, int *colwidths,
+                   int *blocks) {
     if (!b || !v || reps < 1) return QMS_ERR_ARG;
     try {
         static const int KZ[] = {2, 3, 4, 6, 8};
+        static const int CW[] = {64, 32};
         for (size_t d = 0; d < b->hs.size(); ++d) {
             qms_handle *h = b->hs[d];
             check(hipStreamSynchronize(h->s), "sync");
             const int tr = (transient && h->cfg.dt > 0) ? 1 : 0;
             double best = 1e30;
-            int bestkz = h->zchunk;
-            for (int kz : KZ) {
-                h->zchunk = kz;
-                const OpParams op = make_op(h, v + 3 * d, h->cfg.dt, tr);
-                const double ms = time_on(b->s, reps, [&]() { launch_apply(h->g, h->c64, op, h->x, h->w, b->s); });
-                if (ms < best) { best = ms; bestkz = kz; }
+            int bestkz = h->zchunk, bestcw = h->colw;
+            for (int cw : CW) {
+                for (int kz : KZ) {
+                    h->zchunk = kz;
+                    h->colw = cw;
+                    const OpParams op = make_op(h, v + 3 * d, h->cfg.dt, tr);
+                    const double ms =
+                        time_on(b->s, reps, [&]() { launch_apply(h->g, h->c64, op, h->x, h->w, b->s); });
+                    if (ms < best) { best = ms; bestkz = kz; bestcw = cw; }
+                }
             }
             h->zchunk = bestkz;
+            h->colw = bestcw;
             if (zchunks) zchunks[d] = bestkz;
+            if (colwidths) colwidths[d] = bestcw;
         }
         batch_items(b, v, transient);
         static const int BL[] = {0, 1024, 2048, 4096, 8192};

@@ -27,7 +27,7 @@ QMS_MIXED = 1
 EXPORTS = (
     "qms_config_default", "qms_create", "qms_steady", "qms_step", "qms_run",
     "qms_get_fields", "qms_set_fields", "qms_get_h", "qms_apply", "qms_residual", "qms_rhs",
-    "qms_get_stats", "qms_set_zchunk", "qms_time_kernel", "qms_get_work", "qms_destroy", "qms_last_error",
+    "qms_get_stats", "qms_set_zchunk", "qms_set_colwidth", "qms_time_kernel", "qms_get_work", "qms_destroy", "qms_last_error",
     "qms_batch_create", "qms_batch_set_blocks", "qms_batch_apply", "qms_batch_tune", "qms_batch_time",
     "qms_batch_last_error", "qms_batch_destroy",
 )
@@ -110,7 +110,8 @@ def load(path: Optional[str] = None):
     L.qms_batch_set_blocks.argtypes = [H, ctypes.c_int]
     L.qms_batch_apply.argtypes = [H, d, ctypes.c_int]
     L.qms_batch_tune.argtypes = [H, d, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
-                                 ctypes.POINTER(ctypes.c_int)]
+                                 ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    L.qms_set_colwidth.argtypes = [H, ctypes.c_int]
     L.qms_batch_time.argtypes = [H, ctypes.c_int, d, ctypes.c_int, ctypes.c_int, d, d]
     L.qms_batch_last_error.argtypes = [H]
     L.qms_batch_last_error.restype = ctypes.c_char_p
@@ -271,6 +272,10 @@ class Solver:
         operator kernel)."""
         self._check(self._L.qms_set_zchunk(self._h, int(planes)))
 
+    def set_colwidth(self, nodes: int):
+        """Operator column width in nodes (64 or 32)."""
+        self._check(self._L.qms_set_colwidth(self._h, int(nodes)))
+
     def time_kernel(self, which: int, reps: int, v=None):
         """Mean milliseconds and algorithmic bytes per launch of one kernel
         (selectors K_*), timed with HIP events on the handle's stream."""
@@ -325,9 +330,11 @@ class Batch:
     def tune(self, v, transient: bool = True, reps: int = 10):
         vv = self._v(v)
         kz = (ctypes.c_int * len(self.solvers))()
+        cw = (ctypes.c_int * len(self.solvers))()
         bl = ctypes.c_int()
-        self._check(self._L.qms_batch_tune(self._b, _dp(vv), 1 if transient else 0, int(reps), kz, ctypes.byref(bl)))
-        return list(kz), bl.value
+        self._check(self._L.qms_batch_tune(self._b, _dp(vv), 1 if transient else 0, int(reps), kz, cw,
+                                           ctypes.byref(bl)))
+        return list(kz), list(cw), bl.value
 
     def time(self, v, transient: bool = True, reps: int = 20, sequential: bool = False):
         vv = self._v(v)

@@ -13,8 +13,8 @@ What is measured on the GPU: the fused A-V operator kernel at 128^3 (the
 matrix-free replacement of the reference's per-step CSR assembly and the
 operator inside every Krylov iteration) with the rows of the last ramp step
 (transient, v = 5 m/s), against the HBM roofline, with HIP events on the
-stream it is launched on; its z-chunk depth (planes per workgroup) is
-picked from a sweep over 2..8.  Beside it: the FP64 residual, the step right-hand side, the
+stream it is launched on; its column shape and z-chunk depth (planes per
+workgroup) are picked from a sweep.  Beside it: the FP64 residual, the step right-hand side, the
 cell-centred H and a plain copy kernel of the same state (calibration).
 The `batch` object measures the config-4 workload (64 uneven designs): the
 operator of every design in one persistent batched launch against one
@@ -23,7 +23,7 @@ equilibrated SuperLU factor/solve, the restated reference algorithm) timed
 on one core over one ramp step of a 24^3-node design drawn from the same
 box distribution.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--zchunk KZ] [--no-cpu-baseline] [--no-batch]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--zchunk KZ --colwidth CW] [--no-cpu-baseline] [--no-batch]
 """
 import argparse
 import json
@@ -59,7 +59,7 @@ def cpu_baseline():
             "dof_steps_per_s": d.dof / t}
 
 
-ZCHUNKS = (2, 3, 4, 6, 8)
+SHAPES = ((64, 2), (64, 3), (64, 4), (64, 6), (64, 8), (32, 4))     # (column width, z-chunk)
 
 
 def traffic_from_profile(kernel: str):
@@ -103,14 +103,14 @@ def batch_config4(reps: int, ndesigns: int = 64):
     print(f"bench: config 4 batch of {len(ds)} designs, {dof} DoF", file=sys.stderr, flush=True)
     with native.Batch(ss) as b:
         seq0, _ = b.time(v, reps=reps, sequential=True)          # default z-chunk, one launch per design
-        kz, blocks = b.tune(v, reps=10)
+        kz, cw, blocks = b.tune(v, reps=10)
         ms_b, nbytes = b.time(v, reps=reps)
         ms_s, _ = b.time(v, reps=reps, sequential=True)
     for s in ss:
         s.close()
     gbs = nbytes / (ms_b * 1e-3) / 1e9
     return {"workload": f"config 4: {len(ds)} designs, 24-96 nodes per axis, {dof} DoF, transient rows",
-            "dof": dof, "algorithmic_bytes": nbytes, "zchunks": kz, "blocks": blocks,
+            "dof": dof, "algorithmic_bytes": nbytes, "zchunks": kz, "colwidths": cw, "blocks": blocks,
             "batched_ms": ms_b, "sequential_ms": ms_s, "sequential_default_zchunk_ms": seq0,
             "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS, "dof_applies_per_s": dof / (ms_b * 1e-3),
             "speedup_vs_sequential": ms_s / ms_b}
@@ -123,7 +123,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--nodes", type=int, default=128)
     ap.add_argument("--zchunk", type=int, default=0,
-                    help="planes per operator workgroup (0: pick the fastest of 2..8)")
+                    help="planes per operator workgroup (0: sweep column shapes and z-chunks)")
+    ap.add_argument("--colwidth", type=int, default=64, help="operator column width with --zchunk (64 or 32)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the config-4 batched operator measurement")
     a = ap.parse_args()
@@ -152,12 +153,14 @@ def main():
         return s.time_kernel(which, a.steps, v=v)
 
     sweep = {}
-    chunks = (a.zchunk,) if a.zchunk else ZCHUNKS
-    for kz in chunks:
+    shapes = ((a.colwidth, a.zchunk),) if a.zchunk else SHAPES
+    for cw, kz in shapes:
+        s.set_colwidth(cw)
         s.set_zchunk(kz)
-        sweep[kz] = timed(native.K_APPLY)[0]
-        print(f"bench: operator z-chunk {kz}: {sweep[kz] * 1e3:.1f} us", file=sys.stderr, flush=True)
-    kz = min(sweep, key=sweep.get)
+        sweep[(cw, kz)] = timed(native.K_APPLY)[0]
+        print(f"bench: operator column {cw}, z-chunk {kz}: {sweep[(cw, kz)] * 1e3:.1f} us", file=sys.stderr, flush=True)
+    cw, kz = min(sweep, key=sweep.get)
+    s.set_colwidth(cw)
     s.set_zchunk(kz)
     ms, nbytes = timed(native.K_APPLY)
     achieved = nbytes / (ms * 1e-3) / 1e9
@@ -192,12 +195,12 @@ def main():
                                f"(rows of the last ramp step: transient, v = {v[0]:g} m/s)",
                    "dof": d.dof},
         "status": "time-step rate not measured: " + (status or "solve path unavailable"),
-        "roofline": {"kernel": f"k_operator (fused A-V operator apply, FP64, LDS z-march, z-chunk {kz})",
+        "roofline": {"kernel": f"k_operator (fused A-V operator apply, FP64, LDS z-march, column {cw}, z-chunk {kz})",
                      "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": tsrc,
                      "algorithmic_bytes": nbytes, "ms_per_launch": ms, "launches": a.steps},
-        "zchunk_sweep_ms": {str(t): m for t, m in sweep.items()},
+        "shape_sweep_ms": {f"{c}x{z}": m for (c, z), m in sweep.items()},
         "kernels": others,
         "batch": None if a.no_batch else batch_config4(a.steps),
         "cpu_baseline": None if a.no_cpu_baseline else cpu_baseline(),

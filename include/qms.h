@@ -126,6 +126,12 @@ int qms_get_stats(const qms_handle *h, qms_stats *st);
  * QMS_3D.f90:448-1293). */
 int qms_set_zchunk(qms_handle *h, int planes);
 
+/* Operator column width in nodes (64, the default, or 32): a workgroup owns
+ * 64 x 4 or 32 x 8 nodes of each plane it marches through; 32 fits designs
+ * whose x extent is not near a multiple of 64.  Tuning knob, no reference
+ * counterpart. */
+int qms_set_colwidth(qms_handle *h, int nodes);
+
 /* Times `reps` launches of one kernel on the handle's stream with HIP events
  * and reports the mean duration and the algorithmic HBM bytes per launch.
  * v: frame velocity of the rows (NULL: 0); rows are transient when dt > 0.
@@ -156,11 +162,13 @@ int qms_batch_set_blocks(qms_batch *b, int blocks);
  * Results land in each handle's work vector (qms_get_work), identical bit
  * for bit to a single-design qms_apply of the same rows. */
 int qms_batch_apply(qms_batch *b, const double *v, int transient);
-/* Autotuner: times each design's operator at z-chunks {2,3,4,6,8} and keeps
- * the fastest (qms_set_zchunk), then times the batched launch at grid sizes
- * {per column, 1024, 2048, 4096, 8192} and keeps the fastest.  zchunks[n]
- * and blocks (either may be NULL) receive the choices. */
-int qms_batch_tune(qms_batch *b, const double *v, int transient, int reps, int *zchunks, int *blocks);
+/* Autotuner: times each design's operator at column widths {64, 32} x
+ * z-chunks {2,3,4,6,8} and keeps the fastest (qms_set_colwidth,
+ * qms_set_zchunk), then times the batched launch at grid sizes {per column,
+ * 1024, 2048, 4096, 8192} and keeps the fastest.  zchunks[n], colwidths[n]
+ * and blocks (any may be NULL) receive the choices. */
+int qms_batch_tune(qms_batch *b, const double *v, int transient, int reps, int *zchunks, int *colwidths,
+                   int *blocks);
 /* Mean time of one batched launch (mode 0) or of the n single-design
  * launches it replaces (mode 1), and the algorithmic bytes they move. */
 int qms_batch_time(qms_batch *b, int mode, const double *v, int transient, int reps, double *ms_per_launch,

@@ -70,13 +70,15 @@ def test_batch_mixed_zchunks_and_tuner(batch_setup):
     ds, ss, xs, v = batch_setup
     for k, s in enumerate(ss):
         s.set_zchunk((1, 2, 3, 5, 7, 64)[k % 6])
+        s.set_colwidth((32, 64)[k % 2])
     ref = _single(ss, xs, v, True)
     with native.Batch(ss) as b:
         b.apply(v)
         for k, s in enumerate(ss):
             assert np.array_equal(s.get_work(), ref[k]), k
-        kz, blocks = b.tune(v, reps=2)
-        assert all(c in (2, 3, 4, 6, 8) for c in kz) and blocks in (0, 1024, 2048, 4096, 8192)
+        kz, cw, blocks = b.tune(v, reps=2)
+        assert all(c in (2, 3, 4, 6, 8) for c in kz) and all(c in (32, 64) for c in cw)
+        assert blocks in (0, 1024, 2048, 4096, 8192)
         b.apply(v)
         for k, s in enumerate(ss):
             assert np.array_equal(s.get_work(), ref[k]), k

@@ -33,7 +33,7 @@ CASES = [
 ]
 
 
-def _operator_check(d, v, transient, chunks, seed=7):
+def _operator_check(d, v, transient, chunks, seed=7, widths=(64, 32)):
     p = problem(d)
     coef = O.coeffs(p)
     rng = np.random.default_rng(seed)
@@ -45,12 +45,14 @@ def _operator_check(d, v, transient, chunks, seed=7):
     scale = np.abs(M) @ np.abs(x)
     D = np.abs(M.diagonal())
     with solver_for(d) as s:
-        for kz in chunks:
-            s.set_zchunk(kz)
-            y = O.pack(p, s.apply(v, dt, O.unpack(p, x)))
-            err = np.max(np.abs(y - y_ref) / scale)
-            assert err < OP_TOL, (d.name, transient, kz, err)
-            assert rel(y / D, y_ref / D) < OP_TOL
+        for cw in widths:
+            s.set_colwidth(cw)
+            for kz in chunks:
+                s.set_zchunk(kz)
+                y = O.pack(p, s.apply(v, dt, O.unpack(p, x)))
+                err = np.max(np.abs(y - y_ref) / scale)
+                assert err < OP_TOL, (d.name, transient, cw, kz, err)
+                assert rel(y / D, y_ref / D) < OP_TOL
 
 
 @pytest.mark.parametrize("name,mk", CASES, ids=[c[0] for c in CASES])
@@ -74,9 +76,9 @@ def test_operator_matches_csr_config2_full_size():
 
 
 def test_operator_full_size_properties():
-    """128^3 nodes (8,339,456 DoF, config 3 geometry): every z-chunk depth
-    gives bit-identical output (each entry is written exactly once, by the
-    same arithmetic), and the operator is linear to round-off."""
+    """128^3 nodes (8,339,456 DoF, config 3 geometry): every column width and
+    z-chunk depth gives bit-identical output (each entry is written exactly
+    once, by the same arithmetic), and the operator is linear to round-off."""
     d = configs.config3(N=128, ramp=True)
     nz, ny, nx = d.geom.shape[0] + 1, d.geom.shape[1] + 1, d.geom.shape[2] + 1
     rng = np.random.default_rng(23)
@@ -86,7 +88,8 @@ def test_operator_full_size_properties():
     dt = d.inputs.dt
     with solver_for(d) as s:
         ys = []
-        for kz in (4, 1, 5, 64):
+        for cw, kz in ((64, 4), (64, 1), (32, 5), (32, 64)):
+            s.set_colwidth(cw)
             s.set_zchunk(kz)
             ys.append(s.apply(v, dt, x1))
         for y in ys[1:]:
