@@ -56,7 +56,8 @@ def _inputs(cells, h, dt, steps, B0, v_axis_x=0.0, ramp=False, mats=None, save_a
         trajx = (0.0, v_axis_x / (steps * dt), steps)
     else:
         trajx = (v_axis_x, 0.0, 0)
-    return Inputs(cells=tuple(cells), h=(h, h, h), dt=dt, steps=steps, tt=tt, slice_axis=slice_axis,
+    hh = tuple(float(a) for a in h) if isinstance(h, (tuple, list)) else (h, h, h)
+    return Inputs(cells=tuple(cells), h=hh, dt=dt, steps=steps, tt=tt, slice_axis=slice_axis,
                   slice_loc=slice_loc or (cells[2] // 2 + 1), save_all=save_all, B0=tuple(B0),
                   G=((0.0, 0.0, 0.0), (0.0, 0.0, 0.0), (0.0, 0.0, 0.0)),
                   traj=(trajx, (0.0, 0.0, 0), (0.0, 0.0, 0)), materials=mats or [], geom_name="geom.bin")
@@ -159,11 +160,13 @@ def config5_stream(count: int, rate: float, seed: int = 5000, nmin: int = 32, nm
     return out
 
 
-def small_box_design(N: int, seed: int, nbox: int = 3, h: float = 5e-4, dt: float = 5e-6, steps: int = 4,
-                     v: float = 0.0, ramp: bool = True, gradient: bool = False, cells=None) -> Design:
+def small_box_design(N: int, seed: int, nbox: int = 3, h=5e-4, dt: float = 5e-6, steps: int = 4,
+                     v: float = 0.0, ramp: bool = True, gradient: bool = False, cells=None,
+                     aniso: bool = False) -> Design:
     """Small designs for parity tests: random boxes, optional frame velocity
     ramp and an optional traceless symmetric background gradient.  `cells`
-    (x, y, z) overrides the cubic N-1 cells per axis."""
+    (x, y, z) overrides the cubic N-1 cells per axis; `h` may be a per-axis
+    (dx, dy, dz); `aniso` draws each box's mu_r and sigma per axis."""
     rng = np.random.default_rng(seed)
     cz = tuple(cells) if cells is not None else (N - 1,) * 3
     geom = np.ones((cz[2], cz[1], cz[0]))
@@ -176,7 +179,12 @@ def small_box_design(N: int, seed: int, nbox: int = 3, h: float = 5e-4, dt: floa
         lo = [int(rng.integers(3, max(4, cz[a] - 3 - e[a]))) for a in range(3)]
         mur = float(10 ** rng.uniform(0, 3))
         sig = float(10 ** rng.uniform(5, np.log10(SIG_MAX)))
-        mats.append(Material(b + 2, (mur, mur, mur), (sig, sig, sig)))
+        if aniso:
+            murs = tuple(float(10 ** rng.uniform(0, 3)) for _ in range(3))
+            sigs = tuple(float(10 ** rng.uniform(5, np.log10(SIG_MAX))) for _ in range(3))
+            mats.append(Material(b + 2, murs, sigs))
+        else:
+            mats.append(Material(b + 2, (mur, mur, mur), (sig, sig, sig)))
         geom[lo[2]:lo[2] + e[2], lo[1]:lo[1] + e[1], lo[0]:lo[0] + e[0]] = b + 2
     B0 = 0.05 * _unit(rng)
     inp = _inputs(cz, h, dt, steps, B0, v_axis_x=v, ramp=ramp, mats=mats)

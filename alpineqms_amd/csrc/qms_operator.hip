@@ -281,8 +281,27 @@ __device__ __forceinline__ double row_v(const XV &x, const CV &c, const OpK &k, 
 // V row:  sum_a [ (sigma A/dt)_a - Phi_a - E_a ]_(-a)^(0) / h_a
 // Every term is a difference of neighbouring values, as in the row form.
 // ---------------------------------------------------------------------------
+// 1/d for a positive normal d (a row diagonal): the hardware reciprocal
+// estimate refined by two Newton steps, r <- r + r (1 - d r), which leaves
+// it within about one unit in the last place -- cheaper than a correctly
+// rounded division, whose result it may differ from in the last bit.
+__device__ __forceinline__ double recip(double d) {
+    double r = __builtin_amdgcn_rcp(d);
+    r = fma(fma(-d, r, 1.0), r, r);
+    r = fma(fma(-d, r, 1.0), r, r);
+    return r;
+}
+
+// Faces of plane k-1 that the rows of plane k reuse: y-faces at (i,j) and
+// (i-1,j), x-faces at (i,j) and (i,j-1).  A z-marching thread carries them
+// from one plane to the next (`first`: compute them from plane k-1 instead).
+struct FaceCarry {
+    double by0, bym, bx0, bxm;
+};
+
 template <typename XV, typename CV>
-__device__ __forceinline__ void rows_flux(const XV &x, const CV &c, const OpK &k, double r[4], double d[4]) {
+__device__ __forceinline__ void rows_flux(const XV &x, const CV &c, const OpK &k, double r[4], double d[4],
+                                          FaceCarry &fc, bool first) {
     const double ihx = k.ih[0], ihy = k.ih[1], ihz = k.ih[2];
     // face curls; Bz(di,dj) at plane k, By(di,dk) at row j, Bx(dj,dk) at column i
     auto Bz = [&](int di, int dj) {
@@ -294,9 +313,15 @@ __device__ __forceinline__ void rows_flux(const XV &x, const CV &c, const OpK &k
     auto Bx = [&](int dj, int dk) {
         return (x.AZ(0, dj + 1, dk) - x.AZ(0, dj, dk)) * ihy - (x.AY(0, dj, dk + 1) - x.AY(0, dj, dk)) * ihz;
     };
+    if (first) {
+        fc.by0 = By(0, -1);
+        fc.bym = By(-1, -1);
+        fc.bx0 = Bx(0, -1);
+        fc.bxm = Bx(-1, -1);
+    }
     const double bz00 = Bz(0, 0), bz0m = Bz(0, -1), bzm0 = Bz(-1, 0);
-    const double by00 = By(0, 0), by0m = By(0, -1), bym0 = By(-1, 0);
-    const double bx00 = Bx(0, 0), bx0m = Bx(0, -1), bxm0 = Bx(-1, 0);
+    const double by00 = By(0, 0), by0m = fc.by0, bym0 = By(-1, 0);
+    const double bx00 = Bx(0, 0), bx0m = fc.bx0, bxm0 = Bx(-1, 0);
     const double imx0 = c.IMX(0, 0, 0), imy0 = c.IMY(0, 0, 0), imz0 = c.IMZ(0, 0, 0);
     const double imx_y = c.IMX(0, -1, 0), imx_z = c.IMX(0, 0, -1), imx_x = c.IMX(-1, 0, 0);
     const double imy_x = c.IMY(-1, 0, 0), imy_z = c.IMY(0, 0, -1), imy_y = c.IMY(0, -1, 0);
@@ -345,7 +370,7 @@ __device__ __forceinline__ void rows_flux(const XV &x, const CV &c, const OpK &k
     }
     if (k.adv) {
         const double hvx = k.hv[0], hvy = k.hv[1], hvz = k.hv[2];
-        const double bzmm = Bz(-1, -1), bymm = By(-1, -1), bxmm = Bx(-1, -1);
+        const double bzmm = Bz(-1, -1), bymm = fc.bym, bxmm = fc.bxm;
         // sigma (v x avg B) on the node's own edges and on the -x, -y, -z edges
         const double ex0 = sx0 * (hvy * (bz00 + bz0m) - hvz * (by00 + by0m));
         const double ey0 = sy0 * (hvz * (bx00 + bx0m) - hvx * (bz00 + bzm0));
@@ -360,6 +385,10 @@ __device__ __forceinline__ void rows_flux(const XV &x, const CV &c, const OpK &k
     }
     r[0] = rax; r[1] = ray; r[2] = raz; r[3] = rv;
     d[0] = dax; d[1] = day; d[2] = daz;
+    fc.by0 = by00;
+    fc.bym = bym0;
+    fc.bx0 = bx00;
+    fc.bxm = bxm0;
 }
 
 // ---------------------------------------------------------------------------
@@ -477,6 +506,7 @@ __device__ __forceinline__ void op_column(const Grid &g, const ZTiling &t, unsig
         imxz = c.im[0][q]; imyz = c.im[1][q]; imzz = c.im[2][q]; szz = c.sg[2][q];
     }
     const int dxm = ic > 0 ? -1 : 0, dym = jc > 0 ? -g.nx : 0;
+    FaceCarry fc{0.0, 0.0, 0.0, 0.0};
     for (int kk = kb; kk < ke; ++kk) {
         const int r = kk - kb;
         const int sm = r % 3, s0 = (r + 1) % 3, sp = (r + 2) % 3;
@@ -494,7 +524,7 @@ __device__ __forceinline__ void op_column(const Grid &g, const ZTiling &t, unsig
             }
         } else {
             double rr4[4], dd4[4];
-            rows_flux(xv, cv, k, rr4, dd4);
+            rows_flux(xv, cv, k, rr4, dd4, fc, r == 0);
             const double rax = rr4[0], ray = rr4[1], raz = rr4[2], rv = rr4[3];
             const double dax = dd4[0], day = dd4[1], daz = dd4[2], dv = dd4[3];
             if (live) {
@@ -502,7 +532,7 @@ __device__ __forceinline__ void op_column(const Grid &g, const ZTiling &t, unsig
                     const int64_t q = f * n1 + p;
                     double rr;
                     if (free_f(g, f, i, j, kk)) {
-                        rr = (MODE == OP_RESID) ? (b[q] - mx) / d : mx / d;
+                        rr = ((MODE == OP_RESID) ? b[q] - mx : mx) * recip(d);
                     } else if (exists_f(g, f, i, j, kk)) {
                         const double xq = xv.get(f, 0, 0, 0);
                         rr = (MODE == OP_RESID) ? b[q] - xq : xq;

@@ -800,8 +800,28 @@ int qms_batch_tune(qms_batch *b, const double *v, int transient, int reps, int *
             }
             h->zchunk = bestkz;
             h->colw = bestcw;
-            if (zchunks) zchunks[d] = bestkz;
-            if (colwidths) colwidths[d] = bestcw;
+        }
+        // The per-design choice favours many small columns (a lone small
+        // design must fill the GPU); in a batch the designs fill it together,
+        // so also try one shape for all designs and keep what runs fastest.
+        std::vector<int> bkz(b->hs.size()), bcw(b->hs.size());
+        for (size_t d = 0; d < b->hs.size(); ++d) { bkz[d] = b->hs[d]->zchunk; bcw[d] = b->hs[d]->colw; }
+        batch_items(b, v, transient);
+        double best_shape = time_on(b->s, reps, [&]() { batch_launch(b, 0, v, transient); });
+        int ucw = 0, ukz = 0;
+        for (int cw : CW) {
+            for (int kz : {2, 4, 8}) {
+                for (qms_handle *h : b->hs) { h->zchunk = kz; h->colw = cw; }
+                batch_items(b, v, transient);
+                const double ms = time_on(b->s, reps, [&]() { batch_launch(b, 0, v, transient); });
+                if (ms < best_shape) { best_shape = ms; ucw = cw; ukz = kz; }
+            }
+        }
+        for (size_t d = 0; d < b->hs.size(); ++d) {
+            b->hs[d]->zchunk = ukz ? ukz : bkz[d];
+            b->hs[d]->colw = ucw ? ucw : bcw[d];
+            if (zchunks) zchunks[d] = b->hs[d]->zchunk;
+            if (colwidths) colwidths[d] = b->hs[d]->colw;
         }
         batch_items(b, v, transient);
         static const int BL[] = {0, 1024, 2048, 4096, 8192};

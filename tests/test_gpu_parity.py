@@ -30,6 +30,9 @@ CASES = [
     ("N10_grad_v", lambda: configs.small_box_design(10, 1, nbox=2, v=3.0, ramp=False, gradient=True)),
     ("N13_boxes", lambda: configs.small_box_design(13, 2, nbox=3, v=0.0, gradient=True)),
     ("N16_cfg1", lambda: configs.config1(16)),
+    # unequal cell sizes and per-axis mu_r / sigma
+    ("aniso", lambda: configs.small_box_design(0, 5, nbox=4, h=(4e-4, 7e-4, 5.5e-4), v=1.0, ramp=False,
+                                              gradient=True, cells=(21, 17, 19), aniso=True)),
 ]
 
 

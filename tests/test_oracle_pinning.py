@@ -190,6 +190,9 @@ def oracle_on_layout(p, M):
 DESIGNS = [
     ("boxes_v_grad", lambda: configs.small_box_design(11, 3, nbox=3, v=0.0, gradient=True)),
     ("cfg1", lambda: configs.config1(12)),
+    # unequal cell sizes and per-axis mu_r / sigma: a swapped axis or component shows up here
+    ("aniso", lambda: configs.small_box_design(0, 5, nbox=4, h=(4e-4, 7e-4, 5.5e-4), gradient=True,
+                                               cells=(14, 11, 13), aniso=True)),
 ]
 
 
