@@ -16,6 +16,11 @@ struct Coeffs {
 struct CoeffsD {
     const double *im[3];
     const double *sg[3];
+    // Optional compressed form of the same six arrays: a 2-byte class per
+    // node and a table of the distinct (1/Mu_x, 1/Mu_y, 1/Mu_z, Sigma_x,
+    // Sigma_y, Sigma_z) tuples, 6 doubles per class (null when not built).
+    const uint16_t *cls;
+    const double *ctab;
 };
 struct CoeffsF {
     const float *im[3];

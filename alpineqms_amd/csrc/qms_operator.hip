@@ -12,6 +12,7 @@
 // c*(x_p - x_q) so that a constant on a conductor produces its true (tiny)
 // residual.
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include "qms_kernels.h"
 
 // Multiply-adds are fused only within one source expression, never across
@@ -466,10 +467,30 @@ struct ZmCView {
     __device__ __forceinline__ double SZ(int, int, int dk) const { return dk ? szz : sz0; }
 };
 
+// Coefficients of plane k from the class table: the node's class and those of
+// its -x and -y neighbours select rows of the (small, cache-resident) table,
+// so the six FP64 arrays are not streamed from HBM.  Values are the same
+// doubles as the arrays hold.
+struct ZmClassView {
+    double imx0, imy0, imz0, sx0, sy0, sz0;    // at (0,0,0)
+    double imxz, imyz, imzz, szz;              // at (0,0,-1)
+    double imxx, imyx, imzx, sxx;              // at (-1,0,0)
+    double imxy, imyy, imzy, syy;              // at (0,-1,0)
+    __device__ __forceinline__ double pick(double v0, double vx, double vy, double vz, int di, int dj, int dk) const {
+        return dk ? vz : di ? vx : dj ? vy : v0;
+    }
+    __device__ __forceinline__ double IMX(int di, int dj, int dk) const { return pick(imx0, imxx, imxy, imxz, di, dj, dk); }
+    __device__ __forceinline__ double IMY(int di, int dj, int dk) const { return pick(imy0, imyx, imyy, imyz, di, dj, dk); }
+    __device__ __forceinline__ double IMZ(int di, int dj, int dk) const { return pick(imz0, imzx, imzy, imzz, di, dj, dk); }
+    __device__ __forceinline__ double SX(int di, int, int) const { return di ? sxx : sx0; }
+    __device__ __forceinline__ double SY(int, int dj, int) const { return dj ? syy : sy0; }
+    __device__ __forceinline__ double SZ(int, int, int dk) const { return dk ? szz : sz0; }
+};
+
 // One workgroup column `tid` of one design.  Every thread of the workgroup
 // must call this (it synchronises the workgroup); `lds` holds 4*3*ZPL_MAX
 // doubles.
-template <typename TIN, typename TOUT, int MODE, int TX>
+template <typename TIN, typename TOUT, int MODE, int TX, bool CLS>
 __device__ __forceinline__ void op_column(const Grid &g, const ZTiling &t, unsigned tid, const CoeffsD &c,
                                           const OpK &k, const TIN *__restrict__ x, const double *__restrict__ b,
                                           TOUT *__restrict__ out, double *lds) {
@@ -503,7 +524,12 @@ __device__ __forceinline__ void op_column(const Grid &g, const ZTiling &t, unsig
     double imxz, imyz, imzz, szz;
     {
         const int64_t q = pc + plane * max(kb - 1, 0);
-        imxz = c.im[0][q]; imyz = c.im[1][q]; imzz = c.im[2][q]; szz = c.sg[2][q];
+        if (CLS) {
+            const double *t0 = c.ctab + 6 * (int)c.cls[q];
+            imxz = t0[0]; imyz = t0[1]; imzz = t0[2]; szz = t0[5];
+        } else {
+            imxz = c.im[0][q]; imyz = c.im[1][q]; imzz = c.im[2][q]; szz = c.sg[2][q];
+        }
     }
     const int dxm = ic > 0 ? -1 : 0, dym = jc > 0 ? -g.nx : 0;
     FaceCarry fc{0.0, 0.0, 0.0, 0.0};
@@ -513,9 +539,18 @@ __device__ __forceinline__ void op_column(const Grid &g, const ZTiling &t, unsig
         stage(kk + 1, sp);
         __syncthreads();
         const int64_t p = pc + plane * kk;
-        const ZmCView cv{c.im[0][p], c.im[1][p], c.im[2][p], c.sg[0][p], c.sg[1][p], c.sg[2][p],
+        typename std::conditional<CLS, ZmClassView, ZmCView>::type cv;
+        if constexpr (CLS) {
+            const double *t0 = c.ctab + 6 * (int)c.cls[p];
+            const double *tx_ = c.ctab + 6 * (int)c.cls[p + dxm];
+            const double *ty_ = c.ctab + 6 * (int)c.cls[p + dym];
+            cv = ZmClassView{t0[0], t0[1], t0[2], t0[3], t0[4], t0[5], imxz, imyz, imzz, szz,
+                             tx_[0], tx_[1], tx_[2], tx_[3], ty_[0], ty_[1], ty_[2], ty_[4]};
+        } else {
+            cv = ZmCView{c.im[0][p], c.im[1][p], c.im[2][p], c.sg[0][p], c.sg[1][p], c.sg[2][p],
                          imxz, imyz, imzz, szz,
                          c.im[0] + p, c.im[1] + p, c.im[2] + p, c.sg[0] + p, c.sg[1] + p, dxm, dym};
+        }
         const LdsView<TX> xv{lds, {sm * S::PL, s0 * S::PL, sp * S::PL}, ctr};
         if (MODE == OP_VA) {
             if (live && free_v(g, i, j, kk)) {
@@ -552,11 +587,11 @@ __device__ __forceinline__ void op_column(const Grid &g, const ZTiling &t, unsig
     }
 }
 
-template <typename TIN, typename TOUT, int MODE, int TX>
+template <typename TIN, typename TOUT, int MODE, int TX, bool CLS>
 __global__ void __launch_bounds__(256) k_operator(Grid g, ZTiling t, CoeffsD c, OpK k, const TIN *__restrict__ x,
                                                   const double *__restrict__ b, TOUT *__restrict__ out) {
     __shared__ double lds[4 * 3 * ZShape<TX>::PL];
-    op_column<TIN, TOUT, MODE, TX>(g, t, xcd_tile(blockIdx.x, gridDim.x), c, k, x, b, out, lds);
+    op_column<TIN, TOUT, MODE, TX, CLS>(g, t, xcd_tile(blockIdx.x, gridDim.x), c, k, x, b, out, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -578,10 +613,18 @@ __global__ void __launch_bounds__(256) k_operator_batch(const BatchItem *__restr
     for (unsigned w = lo + l; w < hi; w += per) {
         while (d + 1 < nitems && items[d + 1].col0 <= w) ++d;     // w only grows: forward scan
         const BatchItem &it = items[d];
-        if (it.t.tx == 32)
-            op_column<double, double, OP_APPLY, 32>(it.g, it.t, w - it.col0, it.c, it.k, it.x, nullptr, it.y, lds);
-        else
-            op_column<double, double, OP_APPLY, 64>(it.g, it.t, w - it.col0, it.c, it.k, it.x, nullptr, it.y, lds);
+        const unsigned col = w - it.col0;
+        if (it.c.cls) {
+            if (it.t.tx == 32)
+                op_column<double, double, OP_APPLY, 32, true>(it.g, it.t, col, it.c, it.k, it.x, nullptr, it.y, lds);
+            else
+                op_column<double, double, OP_APPLY, 64, true>(it.g, it.t, col, it.c, it.k, it.x, nullptr, it.y, lds);
+        } else {
+            if (it.t.tx == 32)
+                op_column<double, double, OP_APPLY, 32, false>(it.g, it.t, col, it.c, it.k, it.x, nullptr, it.y, lds);
+            else
+                op_column<double, double, OP_APPLY, 64, false>(it.g, it.t, col, it.c, it.k, it.x, nullptr, it.y, lds);
+        }
     }
 }
 
@@ -741,12 +784,15 @@ static void launch_operator(const Grid &g, const CoeffsD &c, const OpParams &op,
                             TOUT *out, hipStream_t s) {
     unsigned ncol;
     const ZTiling t = ztiling(g, op.zchunk, op.colw, ncol);
-    if (t.tx == 32)
-        hipLaunchKernelGGL((k_operator<TIN, TOUT, MODE, 32>), dim3(ncol), dim3(256), 0, s, g, t, c, make_opk(g, op), x, b,
-                           out);
-    else
-        hipLaunchKernelGGL((k_operator<TIN, TOUT, MODE, 64>), dim3(ncol), dim3(256), 0, s, g, t, c, make_opk(g, op), x, b,
-                           out);
+    const OpK k = make_opk(g, op);
+    const dim3 grid(ncol), block(256);
+    if (c.cls) {
+        if (t.tx == 32) hipLaunchKernelGGL((k_operator<TIN, TOUT, MODE, 32, true>), grid, block, 0, s, g, t, c, k, x, b, out);
+        else hipLaunchKernelGGL((k_operator<TIN, TOUT, MODE, 64, true>), grid, block, 0, s, g, t, c, k, x, b, out);
+    } else {
+        if (t.tx == 32) hipLaunchKernelGGL((k_operator<TIN, TOUT, MODE, 32, false>), grid, block, 0, s, g, t, c, k, x, b, out);
+        else hipLaunchKernelGGL((k_operator<TIN, TOUT, MODE, 64, false>), grid, block, 0, s, g, t, c, k, x, b, out);
+    }
 }
 
 // blocks: persistent grid size (rounded up to a multiple of 8 XCDs), or 0 for

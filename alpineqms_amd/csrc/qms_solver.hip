@@ -17,6 +17,8 @@
 #include <stdexcept>
 #include <memory>
 #include <functional>
+#include <array>
+#include <unordered_map>
 #include "qms_kernels.h"
 #include "../../include/qms.h"
 
@@ -54,6 +56,10 @@ struct qms_handle {
     CoeffsF c32;
 
     int64_t n = 0;              // 4*n1
+    uint16_t *cls = nullptr;    // coefficient class per node (null: not built)
+    double *ctab = nullptr;     // class table, 6 doubles per class
+    int ncls = 0;
+    bool use_cls = true;
     double *x = nullptr, *b = nullptr, *r = nullptr, *w = nullptr, *tmp = nullptr;
     double *Dst = nullptr, *Dtr = nullptr;   // row scaling, steady / transient
     std::vector<double *> V, Z;
@@ -139,6 +145,55 @@ struct qms_handle {
 
 namespace {
 
+// Distinct (1/Mu_x, 1/Mu_y, 1/Mu_z, Sigma_x, Sigma_y, Sigma_z) tuples of the
+// design's nodes, keyed by their bit patterns.  Designs built from boxes of a
+// few materials have a few hundred; above 65535 the operator keeps streaming
+// the six arrays.
+struct TupleHash {
+    size_t operator()(const std::array<uint64_t, 6> &k) const {
+        uint64_t hsh = 0x9e3779b97f4a7c15ull;
+        for (uint64_t v : k) hsh = (hsh ^ v) * 0x100000001b3ull + (hsh >> 29);
+        return (size_t)hsh;
+    }
+};
+
+void build_classes(qms_handle *h) {
+    const int64_t n1 = h->g.n1;
+    std::vector<double> a((size_t)6 * n1);
+    for (int q = 0; q < 6; ++q)
+        check(hipMemcpyAsync(a.data() + (size_t)q * n1, q < 3 ? h->coef.im[q] : h->coef.sg[q - 3],
+                             sizeof(double) * n1, hipMemcpyDeviceToHost, h->s), "d2h");
+    check(hipStreamSynchronize(h->s), "sync");
+    std::unordered_map<std::array<uint64_t, 6>, uint16_t, TupleHash> map;
+    std::vector<uint16_t> cls((size_t)n1);
+    std::vector<double> tab;
+    for (int64_t p = 0; p < n1; ++p) {
+        std::array<uint64_t, 6> key;
+        for (int q = 0; q < 6; ++q) memcpy(&key[q], &a[(size_t)q * n1 + p], 8);
+        auto it = map.find(key);
+        if (it == map.end()) {
+            if (map.size() >= 65535) return;           // too many: keep the streamed arrays
+            it = map.emplace(key, (uint16_t)map.size()).first;
+            for (int q = 0; q < 6; ++q) tab.push_back(a[(size_t)q * n1 + p]);
+        }
+        cls[(size_t)p] = it->second;
+    }
+    h->cls = h->dalloc<uint16_t>(n1);
+    h->ctab = h->dalloc<double>((int64_t)tab.size());
+    check(hipMemcpyAsync(h->cls, cls.data(), sizeof(uint16_t) * n1, hipMemcpyHostToDevice, h->s), "h2d");
+    check(hipMemcpyAsync(h->ctab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice, h->s), "h2d");
+    check(hipStreamSynchronize(h->s), "sync");
+    h->ncls = (int)map.size();
+}
+
+// The coefficient view the operator kernels get: with the class table when
+// it was built and is enabled.
+CoeffsD op_coeffs(const qms_handle *h) {
+    CoeffsD c = h->c64;
+    if (h->cls && h->use_cls) { c.cls = h->cls; c.ctab = h->ctab; }
+    return c;
+}
+
 OpParams make_op(const qms_handle *h, const double v[3], double dt, int tr) {
     OpParams op;
     op.v[0] = v[0]; op.v[1] = v[1]; op.v[2] = v[2];
@@ -165,8 +220,8 @@ template <typename T> Hierarchy<T> &hierV(qms_handle *h);
 template <> Hierarchy<float> &hierV<float>(qms_handle *h) { return h->hV32; }
 template <> Hierarchy<double> &hierV<double>(qms_handle *h) { return h->hV64; }
 
-void launch_va(qms_handle *h, const OpParams &op, const float *x, float *y) { launch_va_f32(h->g, h->c64, op, x, y, h->s); }
-void launch_va(qms_handle *h, const OpParams &op, const double *x, double *y) { launch_va_f64(h->g, h->c64, op, x, y, h->s); }
+void launch_va(qms_handle *h, const OpParams &op, const float *x, float *y) { launch_va_f32(h->g, op_coeffs(h), op, x, y, h->s); }
+void launch_va(qms_handle *h, const OpParams &op, const double *x, double *y) { launch_va_f64(h->g, op_coeffs(h), op, x, y, h->s); }
 
 template <typename T>
 void ensure_mg(qms_handle *h, int tr) {
@@ -220,14 +275,14 @@ int fgmres(qms_handle *h, const OpParams &op, int tr) {
     hipStream_t s = h->s;
     // ||D^-1 b||: residual of x = 0
     launch_scale(h->w, h->x, 0.0, n, s);
-    launch_resid(h->g, h->c64, op, h->w, h->b, h->r, s);
+    launch_resid(h->g, op_coeffs(h), op, h->w, h->b, h->r, s);
     const double bnorm = norm_dev(h, h->r);
     std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), gv(m + 1), y(m);
     int total = 0, restarts = 0;
     double rel = 0.0, rel0 = -1.0;
     const double tol = h->cfg.tol;
     while (true) {
-        launch_resid(h->g, h->c64, op, h->x, h->b, h->r, s);
+        launch_resid(h->g, op_coeffs(h), op, h->x, h->b, h->r, s);
         const double beta = norm_dev(h, h->r);
         rel = bnorm > 0 ? beta / bnorm : beta;
         if (rel0 < 0) rel0 = rel;
@@ -248,7 +303,7 @@ int fgmres(qms_handle *h, const OpParams &op, int tr) {
         int k = 0;
         for (int j = 0; j < m && total < h->cfg.max_iters; ++j) {
             precond<T>(h, op, tr, D, h->V[j], h->Z[j]);
-            launch_apply(h->g, h->c64, op, h->Z[j], h->w, s);
+            launch_apply(h->g, op_coeffs(h), op, h->Z[j], h->w, s);
             // classical Gram-Schmidt, one fused pass for the dots, one for the update
             launch_multidot((const double *const *)h->dV, j + 1, h->w, n, h->partial, h->nblk, s);
             launch_reduce_partials(h->partial, j + 1, h->nblk, h->dsmall, s);
@@ -396,7 +451,10 @@ int qms_create(const qms_config *cfg, const uint16_t *cell_slot, const double *s
         h->c64.im[a] = h->coef.im[a]; h->c64.sg[a] = h->coef.sg[a];
         h->c32.im[a] = h->coef.im32[a]; h->c32.sg[a] = h->coef.sg32[a];
     }
+    h->c64.cls = nullptr;
+    h->c64.ctab = nullptr;
     launch_coeffs(g, h->slot, h->mat, h->coef, h->s);
+    build_classes(h.get());
     h->n = 4 * g.n1;
     h->x = h->dalloc<double>(h->n);
     h->b = h->dalloc<double>(h->n);
@@ -507,7 +565,7 @@ int qms_apply(qms_handle *h, const double v[3], double dt, const double *x, doub
     const int tr = dt > 0 ? 1 : 0;
     const OpParams op = make_op(h, v, dt, tr);
     check(hipMemcpyAsync(h->r, x, sizeof(double) * h->n, hipMemcpyHostToDevice, h->s), "h2d");
-    launch_apply(h->g, h->c64, op, h->r, h->w, h->s);
+    launch_apply(h->g, op_coeffs(h), op, h->r, h->w, h->s);
     launch_diag(h->g, h->c64, op, h->tmp, h->s);
     check(hipMemcpyAsync(y, h->w, sizeof(double) * h->n, hipMemcpyDeviceToHost, h->s), "d2h");
     std::vector<double> D(h->n);
@@ -525,7 +583,7 @@ int qms_residual(qms_handle *h, const double v[3], double dt, const double *x, c
     const OpParams op = make_op(h, v, dt, tr);
     check(hipMemcpyAsync(h->r, x, sizeof(double) * h->n, hipMemcpyHostToDevice, h->s), "h2d");
     check(hipMemcpyAsync(h->tmp, b, sizeof(double) * h->n, hipMemcpyHostToDevice, h->s), "h2d");
-    launch_resid(h->g, h->c64, op, h->r, h->tmp, h->w, h->s);
+    launch_resid(h->g, op_coeffs(h), op, h->r, h->tmp, h->w, h->s);
     check(hipMemcpyAsync(r, h->w, sizeof(double) * h->n, hipMemcpyDeviceToHost, h->s), "d2h");
     check(hipStreamSynchronize(h->s), "sync");
     return QMS_OK;
@@ -548,6 +606,7 @@ int qms_get_stats(const qms_handle *h, qms_stats *st) {
     if (!h || !st) return QMS_ERR_ARG;
     *st = h->st;
     st->device_bytes = h->bytes;
+    st->coef_classes = h->ncls;
     return QMS_OK;
 }
 
@@ -558,6 +617,12 @@ int qms_set_zchunk(qms_handle *h, int planes) {
         return QMS_ERR_ARG;
     }
     h->zchunk = planes;
+    return QMS_OK;
+}
+
+int qms_use_coef_classes(qms_handle *h, int on) {
+    if (!h) return QMS_ERR_ARG;
+    h->use_cls = on != 0;
     return QMS_OK;
 }
 
@@ -591,7 +656,7 @@ int qms_time_kernel(qms_handle *h, int which, const double v[3], int reps, doubl
     check(hipEventCreate(&e1), "event");
     auto launch = [&]() {
         if (which == 0) {
-            launch_apply(h->g, h->c64, op, h->x, h->w, h->s);
+            launch_apply(h->g, op_coeffs(h), op, h->x, h->w, h->s);
         } else if (which == 3) {
             launch_copy(h->w, h->x, n, h->s);
         } else if (which == 4) {
@@ -599,7 +664,7 @@ int qms_time_kernel(qms_handle *h, int which, const double v[3], int reps, doubl
         } else if (which == 5) {
             launch_hfield(h->g, h->c64, h->x, h->tmp, 0, h->s);
         } else if (which == 6) {
-            launch_resid(h->g, h->c64, op, h->x, h->b, h->w, h->s);
+            launch_resid(h->g, op_coeffs(h), op, h->x, h->b, h->w, h->s);
         } else {
             const Fine f = h->fine(tr ? h->cfg.dt : 0.0);
             const MGParams mp = mgp(h);
@@ -624,6 +689,9 @@ int qms_time_kernel(qms_handle *h, int which, const double v[3], int reps, doubl
     *ms_per_launch = ms / reps;
     // Algorithmic bytes: each input and output array streamed once.
     const double N = (double)n, N1 = (double)n1, NC = (double)nc;
+    // Operator: the workload's algorithmic bytes, x and y (and b) plus the six
+    // FP64 coefficient arrays of the reference's operator, whichever way the
+    // kernel reads its coefficients.
     switch (which) {
     case 0: *bytes_per_launch = 8.0 * (2.0 * N + 6.0 * N1); break;           // x, y, 6 coefficient arrays
     case 6: *bytes_per_launch = 8.0 * (3.0 * N + 6.0 * N1); break;           // x, b, r, 6 coefficient arrays
@@ -687,7 +755,7 @@ void batch_items(qms_batch *b, const double *v, int transient) {
         unsigned ncol = 0;
         it.g = h->g;
         it.t = ztiling(h->g, h->zchunk, h->colw, ncol);
-        it.c = h->c64;
+        it.c = op_coeffs(h);
         it.k = make_opk(h->g, op);
         it.x = h->x;
         it.y = h->w;
@@ -708,7 +776,7 @@ void batch_launch(qms_batch *b, int mode, const double *v, int transient) {
         for (size_t d = 0; d < b->hs.size(); ++d) {
             qms_handle *h = b->hs[d];
             const int tr = (transient && h->cfg.dt > 0) ? 1 : 0;
-            launch_apply(h->g, h->c64, make_op(h, v + 3 * d, h->cfg.dt, tr), h->x, h->w, b->s);
+            launch_apply(h->g, op_coeffs(h), make_op(h, v + 3 * d, h->cfg.dt, tr), h->x, h->w, b->s);
         }
     }
 }
@@ -794,7 +862,7 @@ int qms_batch_tune(qms_batch *b, const double *v, int transient, int reps, int *
                     h->colw = cw;
                     const OpParams op = make_op(h, v + 3 * d, h->cfg.dt, tr);
                     const double ms =
-                        time_on(b->s, reps, [&]() { launch_apply(h->g, h->c64, op, h->x, h->w, b->s); });
+                        time_on(b->s, reps, [&]() { launch_apply(h->g, op_coeffs(h), op, h->x, h->w, b->s); });
                     if (ms < best) { best = ms; bestkz = kz; bestcw = cw; }
                 }
             }

@@ -27,7 +27,7 @@ QMS_MIXED = 1
 EXPORTS = (
     "qms_config_default", "qms_create", "qms_steady", "qms_step", "qms_run",
     "qms_get_fields", "qms_set_fields", "qms_get_h", "qms_apply", "qms_residual", "qms_rhs",
-    "qms_get_stats", "qms_set_zchunk", "qms_set_colwidth", "qms_time_kernel", "qms_get_work", "qms_destroy", "qms_last_error",
+    "qms_get_stats", "qms_set_zchunk", "qms_set_colwidth", "qms_use_coef_classes", "qms_time_kernel", "qms_get_work", "qms_destroy", "qms_last_error",
     "qms_batch_create", "qms_batch_set_blocks", "qms_batch_apply", "qms_batch_tune", "qms_batch_time",
     "qms_batch_last_error", "qms_batch_destroy",
 )
@@ -73,6 +73,7 @@ class Stats(ctypes.Structure):
         ("device_bytes", ctypes.c_int64),
         ("n_levels_a", ctypes.c_int),
         ("n_levels_v", ctypes.c_int),
+        ("coef_classes", ctypes.c_int),
     ]
 
 
@@ -112,6 +113,7 @@ def load(path: Optional[str] = None):
     L.qms_batch_tune.argtypes = [H, d, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                  ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     L.qms_set_colwidth.argtypes = [H, ctypes.c_int]
+    L.qms_use_coef_classes.argtypes = [H, ctypes.c_int]
     L.qms_batch_time.argtypes = [H, ctypes.c_int, d, ctypes.c_int, ctypes.c_int, d, d]
     L.qms_batch_last_error.argtypes = [H]
     L.qms_batch_last_error.restype = ctypes.c_char_p
@@ -271,6 +273,10 @@ class Solver:
         """Planes per operator workgroup (z-march depth of the LDS-staged
         operator kernel)."""
         self._check(self._L.qms_set_zchunk(self._h, int(planes)))
+
+    def use_coef_classes(self, on: bool):
+        """Operator coefficients from the class table (True) or streamed."""
+        self._check(self._L.qms_use_coef_classes(self._h, 1 if on else 0))
 
     def set_colwidth(self, nodes: int):
         """Operator column width in nodes (64 or 32)."""

@@ -180,6 +180,11 @@ def main():
     s.set_zchunk(kz)
     ms, nbytes = timed(native.K_APPLY)
     achieved = nbytes / (ms * 1e-3) / 1e9
+    ncls = s.stats().coef_classes
+    # the same apply with the six coefficient arrays streamed from HBM
+    s.use_coef_classes(False)
+    ms_st, bytes_st = timed(native.K_APPLY)
+    s.use_coef_classes(True)
     others = {}
     for name, which in (("k_operator_resid", native.K_RESID), ("k_rhs", native.K_RHS),
                         ("k_hfield", native.K_HFIELD), ("k_copy", native.K_COPY)):
@@ -193,7 +198,7 @@ def main():
     except native.QmsError as e:
         status = str(e)
     s.close()
-    traffic, tsrc = traffic_from_profile("k_operator_apply_config3_128")
+    traffic, tsrc = traffic_from_profile("k_operator_apply_config3_128" + ("_classes" if ncls else ""))
     line = {
         "metric": "A-V time steps/s at 128^3, mixed vs FP64 @1e-8 rel err; served p50/p99 latency",
         "value": None,
@@ -211,12 +216,21 @@ def main():
                                f"(rows of the last ramp step: transient, v = {v[0]:g} m/s)",
                    "dof": d.dof},
         "status": "time-step rate not measured: " + (status or "solve path unavailable"),
-        "roofline": {"kernel": f"k_operator (fused A-V operator apply, FP64, LDS z-march, column {cw}, z-chunk {kz})",
+        "roofline": {"kernel": f"k_operator (fused A-V operator apply, FP64, LDS z-march, column {cw}, z-chunk {kz}"
+                               f"{', coefficient classes' if ncls else ''})",
                      "bound": "hbm",
+                     "algorithmic_bytes_definition": "8*(2n + 6*N^3): x and y (n = 4N^3 doubles each) and the six "
+                                                     "FP64 coefficient arrays of the reference operator",
+                     "kernel_stream_bytes": (16.0 * 4 * nx * ny * nz + 2.0 * nx * ny * nz + 48.0 * ncls) if ncls
+                     else nbytes,
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": tsrc,
                      "algorithmic_bytes": nbytes, "ms_per_launch": ms, "launches": a.steps},
         "shape_sweep_ms": {f"{c}x{z}": m for (c, z), m in sweep.items()},
+        "coefficient_classes": ncls,
+        "operator_streamed_coefficients": {"ms_per_launch": ms_st, "algorithmic_bytes": bytes_st,
+                                           "achieved_GBs": bytes_st / (ms_st * 1e-3) / 1e9,
+                                           "frac": bytes_st / (ms_st * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "kernels": others,
         "batch": None if a.no_batch else batch_config4(a.steps),
         "cpu_baseline": None if a.no_cpu_baseline else cpu_baseline(),

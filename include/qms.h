@@ -65,6 +65,7 @@ typedef struct {
     int64_t solves;
     int64_t device_bytes;    /* device memory held by the handle */
     int n_levels_a, n_levels_v;
+    int coef_classes;        /* distinct coefficient tuples (0: operator streams the arrays) */
 } qms_stats;
 
 /* Fills defaults: FP64, tol 1e-12, restart 30, 3000 iterations, nu 4,
@@ -126,6 +127,14 @@ int qms_get_stats(const qms_handle *h, qms_stats *st);
  * QMS_3D.f90:448-1293). */
 int qms_set_zchunk(qms_handle *h, int planes);
 
+/* Operator coefficients from the class table (on != 0, the default when
+ * the design has at most 65535 distinct (1/Mu, Sigma) node tuples) or
+ * streamed from the six node arrays (on = 0); the rows are bit-identical
+ * either way.  qms_create builds the table: a 2-byte class per node and the
+ * distinct tuples, instead of 48 bytes per node read on every apply.
+ * Performance knob, no reference counterpart. */
+int qms_use_coef_classes(qms_handle *h, int on);
+
 /* Operator column width in nodes (64, the default, or 32): a workgroup owns
  * 64 x 4 or 32 x 8 nodes of each plane it marches through; 32 fits designs
  * whose x extent is not near a multiple of 64.  Tuning knob, no reference
@@ -139,6 +148,8 @@ int qms_set_colwidth(qms_handle *h, int nodes);
  * 2 level-0 V-block smoother sweep (precision of the handle's multigrid),
  * 3 plain copy of the 4-field state (calibration of byte counters),
  * 4 step right-hand side, 5 cell-centred H, 6 FP64 residual b - M x.
+ * Operator bytes are the workload's: x, y (and b) plus the six FP64
+ * coefficient arrays, also when the kernel reads the class table instead.
  * No reference counterpart (measurement only). */
 int qms_time_kernel(qms_handle *h, int which, const double v[3], int reps, double *ms_per_launch,
                     double *bytes_per_launch);

@@ -48,6 +48,7 @@ def _operator_check(d, v, transient, chunks, seed=7, widths=(64, 32)):
     scale = np.abs(M) @ np.abs(x)
     D = np.abs(M.diagonal())
     with solver_for(d) as s:
+        assert s.stats().coef_classes > 0          # small box designs always fit the class table
         for cw in widths:
             s.set_colwidth(cw)
             for kz in chunks:
@@ -56,6 +57,10 @@ def _operator_check(d, v, transient, chunks, seed=7, widths=(64, 32)):
                 err = np.max(np.abs(y - y_ref) / scale)
                 assert err < OP_TOL, (d.name, transient, cw, kz, err)
                 assert rel(y / D, y_ref / D) < OP_TOL
+        # coefficients streamed from the six arrays instead of the class table: same bits
+        s.use_coef_classes(False)
+        y2 = O.pack(p, s.apply(v, dt, O.unpack(p, x)))
+        assert np.array_equal(y2, y)
 
 
 @pytest.mark.parametrize("name,mk", CASES, ids=[c[0] for c in CASES])
@@ -80,8 +85,9 @@ def test_operator_matches_csr_config2_full_size():
 
 def test_operator_full_size_properties():
     """128^3 nodes (8,339,456 DoF, config 3 geometry): every column width and
-    z-chunk depth gives bit-identical output (each entry is written exactly
-    once, by the same arithmetic), and the operator is linear to round-off."""
+    z-chunk depth, with coefficients from the class table or streamed, gives
+    bit-identical output (each entry is written exactly once, by the same
+    arithmetic), and the operator is linear to round-off."""
     d = configs.config3(N=128, ramp=True)
     nz, ny, nx = d.geom.shape[0] + 1, d.geom.shape[1] + 1, d.geom.shape[2] + 1
     rng = np.random.default_rng(23)
@@ -91,10 +97,12 @@ def test_operator_full_size_properties():
     dt = d.inputs.dt
     with solver_for(d) as s:
         ys = []
-        for cw, kz in ((64, 4), (64, 1), (32, 5), (32, 64)):
+        for cw, kz, cls in ((64, 4, True), (64, 1, True), (32, 5, True), (32, 64, True), (64, 8, False)):
+            s.use_coef_classes(cls)
             s.set_colwidth(cw)
             s.set_zchunk(kz)
             ys.append(s.apply(v, dt, x1))
+        s.use_coef_classes(True)
         for y in ys[1:]:
             assert np.array_equal(y, ys[0])
         assert np.all(np.isfinite(ys[0]))
