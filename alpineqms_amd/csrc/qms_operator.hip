@@ -664,6 +664,8 @@ __global__ void __launch_bounds__(256) k_rhs(Grid g, Tiling t, CoeffsD c, BC bc,
             if (!free_f(g, f, i, j, kk)) {
                 r = bc_value(g, bc, k.v, f, i, j, kk);
             } else if (k.tr) {
+                // sigma streamed from the arrays: here the class-table gathers
+                // measured slower than the coalesced array reads
                 if (f < 3) {
                     r = (c.sg[f][p] * k.idt) * aprev[f * n1 + p];
                 } else {
@@ -720,6 +722,12 @@ __global__ void __launch_bounds__(256) k_hfield(Grid g, CoeffsD c, const double 
         H[q] = 0.5 * (bx0 + bx1);
         H[nc + q] = 0.5 * (by0 + by1);
         H[2 * nc + q] = 0.5 * (bz0 + bz1);
+    } else if (c.cls) {
+        // 1/Mu from the class table (the same doubles as the arrays)
+        const double *t0 = c.ctab + 6 * (int)c.cls[p];
+        H[q] = 0.5 * (bx0 * t0[0] + bx1 * c.ctab[6 * (int)c.cls[p + X]]);
+        H[nc + q] = 0.5 * (by0 * t0[1] + by1 * c.ctab[6 * (int)c.cls[p + Y] + 1]);
+        H[2 * nc + q] = 0.5 * (bz0 * t0[2] + bz1 * c.ctab[6 * (int)c.cls[p + Z] + 2]);
     } else {
         const double *imx = c.im[0] + p, *imy = c.im[1] + p, *imz = c.im[2] + p;
         H[q] = 0.5 * (bx0 * imx[0] + bx1 * imx[X]);

@@ -497,7 +497,7 @@ int qms_steady(qms_handle *h, const double v[3]) {
     const OpParams op = make_op(h, v, 0.0, 0);
     const double zero[3] = {0, 0, 0};
     (void)zero;
-    launch_rhs(h->g, h->c64, h->bc, op, h->x, h->b, h->s);
+    launch_rhs(h->g, op_coeffs(h), h->bc, op, h->x, h->b, h->s);
     launch_scale(h->x, h->x, 0.0, h->n, h->s);
     launch_setbc(h->g, h->bc, op, h->x, h->s);
     solve(h, op, 0);
@@ -513,7 +513,7 @@ int qms_step(qms_handle *h, const double v[3]) {
     if (!solve_enabled(h)) return QMS_ERR_STATE;
     QMS_TRY(h)
     const OpParams op = make_op(h, v, h->cfg.dt, 1);
-    launch_rhs(h->g, h->c64, h->bc, op, h->x, h->b, h->s);
+    launch_rhs(h->g, op_coeffs(h), h->bc, op, h->x, h->b, h->s);
     launch_setbc(h->g, h->bc, op, h->x, h->s);
     solve(h, op, 1);
     return QMS_OK;
@@ -552,7 +552,7 @@ int qms_get_h(qms_handle *h, int unit_mu, double *out) {
     if (!h || !out) return QMS_ERR_ARG;
     QMS_TRY(h)
     const int64_t nc = (int64_t)h->cfg.cells[0] * h->cfg.cells[1] * h->cfg.cells[2];
-    launch_hfield(h->g, h->c64, h->x, h->tmp, unit_mu, h->s);
+    launch_hfield(h->g, op_coeffs(h), h->x, h->tmp, unit_mu, h->s);
     check(hipMemcpyAsync(out, h->tmp, sizeof(double) * 3 * nc, hipMemcpyDeviceToHost, h->s), "d2h");
     check(hipStreamSynchronize(h->s), "sync");
     return QMS_OK;
@@ -595,7 +595,7 @@ int qms_rhs(qms_handle *h, const double v[3], double dt, const double *aprev, do
     QMS_TRY(h)
     const OpParams op = make_op(h, v, dt, dt > 0 ? 1 : 0);
     check(hipMemcpyAsync(h->r, aprev, sizeof(double) * h->n, hipMemcpyHostToDevice, h->s), "h2d");
-    launch_rhs(h->g, h->c64, h->bc, op, h->r, h->w, h->s);
+    launch_rhs(h->g, op_coeffs(h), h->bc, op, h->r, h->w, h->s);
     check(hipMemcpyAsync(b, h->w, sizeof(double) * h->n, hipMemcpyDeviceToHost, h->s), "d2h");
     check(hipStreamSynchronize(h->s), "sync");
     return QMS_OK;
@@ -660,9 +660,9 @@ int qms_time_kernel(qms_handle *h, int which, const double v[3], int reps, doubl
         } else if (which == 3) {
             launch_copy(h->w, h->x, n, h->s);
         } else if (which == 4) {
-            launch_rhs(h->g, h->c64, h->bc, op, h->x, h->w, h->s);
+            launch_rhs(h->g, op_coeffs(h), h->bc, op, h->x, h->w, h->s);
         } else if (which == 5) {
-            launch_hfield(h->g, h->c64, h->x, h->tmp, 0, h->s);
+            launch_hfield(h->g, op_coeffs(h), h->x, h->tmp, 0, h->s);
         } else if (which == 6) {
             launch_resid(h->g, op_coeffs(h), op, h->x, h->b, h->w, h->s);
         } else {
