@@ -298,6 +298,7 @@ __device__ __forceinline__ double recip(double d) {
 // from one plane to the next (`first`: compute them from plane k-1 instead).
 struct FaceCarry {
     double by0, bym, bx0, bxm;
+    double ez;             // sigma (v x avg B)_z on the Az edge of plane k-1
 };
 
 template <typename XV, typename CV>
@@ -339,9 +340,9 @@ __device__ __forceinline__ void rows_flux(const XV &x, const CV &c, const OpK &k
     // gauge penalty
     const double ax0 = x.AX(0, 0, 0), ay0 = x.AY(0, 0, 0), az0 = x.AZ(0, 0, 0), v0 = x.VV(0, 0, 0);
     const double axm = x.AX(-1, 0, 0), aym = x.AY(0, -1, 0), azm = x.AZ(0, 0, -1);
-    rax -= (imx0 * (x.AX(1, 0, 0) - ax0) * ihx - imx_x * (ax0 - axm) * ihx) * ihx;
-    ray -= (imy0 * (x.AY(0, 1, 0) - ay0) * ihy - imy_y * (ay0 - aym) * ihy) * ihy;
-    raz -= (imz0 * (x.AZ(0, 0, 1) - az0) * ihz - imz_z * (az0 - azm) * ihz) * ihz;
+    rax -= (imx0 * (x.AX(1, 0, 0) - ax0) - imx_x * (ax0 - axm)) * k.ih2[0];
+    ray -= (imy0 * (x.AY(0, 1, 0) - ay0) - imy_y * (ay0 - aym)) * k.ih2[1];
+    raz -= (imz0 * (x.AZ(0, 0, 1) - az0) - imz_z * (az0 - azm)) * k.ih2[2];
     // sigma grad V on the A rows
     const double vxp = x.VV(1, 0, 0), vyp = x.VV(0, 1, 0), vzp = x.VV(0, 0, 1);
     const double vxm = x.VV(-1, 0, 0), vym = x.VV(0, -1, 0), vzm = x.VV(0, 0, -1);
@@ -352,9 +353,9 @@ __device__ __forceinline__ void rows_flux(const XV &x, const CV &c, const OpK &k
     const double fx0f = fmax(sx0, QMS_SIGFLOOR), fx1f = fmax(sx1, QMS_SIGFLOOR);
     const double fy0f = fmax(sy0, QMS_SIGFLOOR), fy1f = fmax(sy1, QMS_SIGFLOOR);
     const double fz0f = fmax(sz0, QMS_SIGFLOOR), fz1f = fmax(sz1, QMS_SIGFLOOR);
-    double rv = -((fx0f * (vxp - v0) * ihx - fx1f * (v0 - vxm) * ihx) * ihx
-                + (fy0f * (vyp - v0) * ihy - fy1f * (v0 - vym) * ihy) * ihy
-                + (fz0f * (vzp - v0) * ihz - fz1f * (v0 - vzm) * ihz) * ihz);
+    double rv = -((fx0f * (vxp - v0) - fx1f * (v0 - vxm)) * k.ih2[0]
+                + (fy0f * (vyp - v0) - fy1f * (v0 - vym)) * k.ih2[1]
+                + (fz0f * (vzp - v0) - fz1f * (v0 - vzm)) * k.ih2[2]);
     double dax = (imz0 + imz_y) * k.ih2[1] + (imy0 + imy_z) * k.ih2[2] + (imx0 + imx_x) * k.ih2[0];
     double day = (imz0 + imz_x) * k.ih2[0] + (imx0 + imx_z) * k.ih2[2] + (imy0 + imy_y) * k.ih2[1];
     double daz = (imy0 + imy_x) * k.ih2[0] + (imx0 + imx_y) * k.ih2[1] + (imz0 + imz_z) * k.ih2[2];
@@ -378,7 +379,9 @@ __device__ __forceinline__ void rows_flux(const XV &x, const CV &c, const OpK &k
         const double ez0 = sz0 * (hvx * (by00 + bym0) - hvy * (bx00 + bxm0));
         const double exm = sx1 * (hvy * (bzm0 + bzmm) - hvz * (bym0 + bymm));
         const double eym = sy1 * (hvz * (bxm0 + bxmm) - hvx * (bz0m + bzmm));
-        const double ezm = sz1 * (hvx * (by0m + bymm) - hvy * (bx0m + bxmm));
+        // the Az edge of plane k-1 was the previous plane's own edge
+        const double ezm = first ? sz1 * (hvx * (by0m + bymm) - hvy * (bx0m + bxmm)) : fc.ez;
+        fc.ez = ez0;
         rax += ex0;
         ray += ey0;
         raz += ez0;
@@ -532,7 +535,7 @@ __device__ __forceinline__ void op_column(const Grid &g, const ZTiling &t, unsig
         }
     }
     const int dxm = ic > 0 ? -1 : 0, dym = jc > 0 ? -g.nx : 0;
-    FaceCarry fc{0.0, 0.0, 0.0, 0.0};
+    FaceCarry fc{0.0, 0.0, 0.0, 0.0, 0.0};
     for (int kk = kb; kk < ke; ++kk) {
         const int r = kk - kb;
         const int sm = r % 3, s0 = (r + 1) % 3, sp = (r + 2) % 3;
