@@ -878,7 +878,7 @@ int qms_batch_tune(qms_batch *b, const double *v, int transient, int reps, int *
         double best_shape = time_on(b->s, reps, [&]() { batch_launch(b, 0, v, transient); });
         int ucw = 0, ukz = 0;
         for (int cw : CW) {
-            for (int kz : {2, 4, 8}) {
+            for (int kz : {2, 4, 8, 16}) {
                 for (qms_handle *h : b->hs) { h->zchunk = kz; h->colw = cw; }
                 batch_items(b, v, transient);
                 const double ms = time_on(b->s, reps, [&]() { batch_launch(b, 0, v, transient); });

@@ -176,7 +176,7 @@ int qms_batch_apply(qms_batch *b, const double *v, int transient);
 /* Autotuner: times each design's operator at column widths {64, 32} x
  * z-chunks {2,3,4,6,8} and keeps the fastest (qms_set_colwidth,
  * qms_set_zchunk); then times the batched launch with those per-design
- * shapes and with one shape for all designs ({64, 32} x {2, 4, 8}) and keeps
+ * shapes and with one shape for all designs ({64, 32} x {2, 4, 8, 16}) and keeps
  * the fastest; then times grid sizes {per column, 1024, 2048, 4096, 8192}.  zchunks[n], colwidths[n]
  * and blocks (any may be NULL) receive the choices. */
 int qms_batch_tune(qms_batch *b, const double *v, int transient, int reps, int *zchunks, int *colwidths,

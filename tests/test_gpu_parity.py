@@ -136,6 +136,30 @@ def test_residual_matches_oracle(name, mk):
                 assert np.max(np.abs(r - ref) / scale) < OP_TOL, (dt, kz)
 
 
+def test_operator_largest_served_size():
+    """160^3 nodes (16,307,200 DoF), the largest design of the served request
+    stream: column shapes and z-chunks that split the grid differently give
+    bit-identical output, and the operator is linear to round-off."""
+    d = configs.small_box_design(160, 51, nbox=8, v=2.0, ramp=False, gradient=True)
+    nx, ny, nz = d.nodes
+    rng = np.random.default_rng(53)
+    x1 = rng.normal(size=(4, nz, ny, nx))
+    x2 = rng.normal(size=(4, nz, ny, nx))
+    v = np.array([2.0, -1.0, 0.5])
+    with solver_for(d, restart=1) as s:
+        s.set_colwidth(64)
+        s.set_zchunk(8)
+        y1 = s.apply(v, d.inputs.dt, x1)
+        s.set_colwidth(32)
+        s.set_zchunk(3)
+        assert np.array_equal(s.apply(v, d.inputs.dt, x1), y1)
+        y2 = s.apply(v, d.inputs.dt, x2)
+        y12 = s.apply(v, d.inputs.dt, 0.5 * x1 + 2.0 * x2)
+    lin = 0.5 * y1 + 2.0 * y2
+    assert np.all(np.isfinite(y1))
+    assert np.max(np.abs(y12 - lin)) <= 1e-12 * np.max(np.abs(lin))
+
+
 def test_time_kernel_selectors():
     d = configs.small_box_design(16, 3, nbox=2, v=1.0, ramp=False)
     with solver_for(d) as s:
