@@ -71,7 +71,9 @@ struct BatchItem {
     double *y;
     unsigned col0, ncol;      // first workgroup column in the batch, column count
 };
-void launch_operator_batch(const BatchItem *items_dev, int nitems, unsigned total, int blocks, hipStream_t s);
+// tx, cls: column width and coefficient form shared by every design, or tx = 0.
+void launch_operator_batch(const BatchItem *items_dev, int nitems, unsigned total, int blocks, int tx, int cls,
+                           hipStream_t s);
 
 enum { OP_RESID = 0, OP_APPLY = 1, OP_VA = 2 };
 

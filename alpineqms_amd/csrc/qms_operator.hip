@@ -492,8 +492,9 @@ struct ZmClassView {
 
 // One workgroup column `tid` of one design.  Every thread of the workgroup
 // must call this (it synchronises the workgroup); `lds` holds 4*3*ZPL_MAX
-// doubles.
-template <typename TIN, typename TOUT, int MODE, int TX, bool CLS>
+// doubles.  STG selects the staging form (see `stage`); the results do not
+// depend on it.
+template <typename TIN, typename TOUT, int MODE, int TX, bool CLS, bool STG = false>
 __device__ __forceinline__ void op_column(const Grid &g, const ZTiling &t, unsigned tid, const CoeffsD &c,
                                           const OpK &k, const TIN *__restrict__ x, const double *__restrict__ b,
                                           TOUT *__restrict__ out, double *lds) {
@@ -510,14 +511,45 @@ __device__ __forceinline__ void op_column(const Grid &g, const ZTiling &t, unsig
     const int64_t n1 = g.n1;
     const int64_t plane = (int64_t)g.nx * g.ny;
 
+    // Stage plane kp into LDS slot `slot`: all loads of the plane are issued
+    // before the first LDS write, so one memory latency is exposed per plane.
+    static_assert(S::PL > 256 && S::PL <= 512, "two staging rounds per plane");
+    // Stage plane kp into LDS slot `slot`.  STG: all eight loads of a thread
+    // are issued before its first LDS write (one exposed memory latency per
+    // plane, at the cost of the registers that hold them); otherwise the two
+    // rounds of four loads run one after the other.
+    static_assert(S::PL > 256 && S::PL <= 512, "two staging rounds per plane");
     auto stage = [&](int kp, int slot) {
         kp = max(0, min(kp, g.nz - 1));
-        for (int e = (int)threadIdx.x; e < S::PL; e += 256) {
-            const int ei = e % S::SY, ej = e / S::SY;
-            const int gi = max(0, min(i0 - 1 + ei, g.nx - 1)), gj = max(0, min(j0 - 1 + ej, g.ny - 1));
-            const int64_t q = gi + (int64_t)g.nx * gj + plane * kp;
+        if constexpr (STG) {
+            TIN v[2][4];
 #pragma unroll
-            for (int f = 0; f < 4; ++f) lds[(f * 3 + slot) * S::PL + e] = (double)x[f * n1 + q];
+            for (int u = 0; u < 2; ++u) {
+                const int e = (int)threadIdx.x + 256 * u;
+                if (u == 0 || e < S::PL) {
+                    const int ei = e % S::SY, ej = e / S::SY;
+                    const int gi = max(0, min(i0 - 1 + ei, g.nx - 1)), gj = max(0, min(j0 - 1 + ej, g.ny - 1));
+                    const int64_t q = gi + (int64_t)g.nx * gj + plane * kp;
+#pragma unroll
+                    for (int f = 0; f < 4; ++f) v[u][f] = x[f * n1 + q];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int e = (int)threadIdx.x + 256 * u;
+                if (u == 0 || e < S::PL) {
+#pragma unroll
+                    for (int f = 0; f < 4; ++f) lds[(f * 3 + slot) * S::PL + e] = (double)v[u][f];
+                }
+            }
+        } else {
+            for (int e = (int)threadIdx.x; e < S::PL; e += 256) {
+                const int ei = e % S::SY, ej = e / S::SY;
+                const int gi = max(0, min(i0 - 1 + ei, g.nx - 1)), gj = max(0, min(j0 - 1 + ej, g.ny - 1));
+                const int64_t q = gi + (int64_t)g.nx * gj + plane * kp;
+#pragma unroll
+                for (int f = 0; f < 4; ++f) lds[(f * 3 + slot) * S::PL + e] = (double)x[f * n1 + q];
+            }
         }
     };
 
@@ -535,25 +567,38 @@ __device__ __forceinline__ void op_column(const Grid &g, const ZTiling &t, unsig
         }
     }
     const int dxm = ic > 0 ? -1 : 0, dym = jc > 0 ? -g.nx : 0;
+    // Class indices of the node and its -x / -y neighbours, one plane ahead.
+    unsigned c0 = 0, cx = 0, cy = 0;
+    if constexpr (CLS) {
+        const int64_t p = pc + plane * kb;
+        c0 = c.cls[p]; cx = c.cls[p + dxm]; cy = c.cls[p + dym];
+    }
     FaceCarry fc{0.0, 0.0, 0.0, 0.0, 0.0};
     for (int kk = kb; kk < ke; ++kk) {
         const int r = kk - kb;
         const int sm = r % 3, s0 = (r + 1) % 3, sp = (r + 2) % 3;
-        stage(kk + 1, sp);
-        __syncthreads();
         const int64_t p = pc + plane * kk;
+        // The coefficients of plane kk (and the class indices of plane kk+1)
+        // are requested before the plane-(kk+1) staging loads, so their
+        // latencies overlap.
         typename std::conditional<CLS, ZmClassView, ZmCView>::type cv;
         if constexpr (CLS) {
-            const double *t0 = c.ctab + 6 * (int)c.cls[p];
-            const double *tx_ = c.ctab + 6 * (int)c.cls[p + dxm];
-            const double *ty_ = c.ctab + 6 * (int)c.cls[p + dym];
+            const double *t0 = c.ctab + 6 * c0;
+            const double *tx_ = c.ctab + 6 * cx;
+            const double *ty_ = c.ctab + 6 * cy;
             cv = ZmClassView{t0[0], t0[1], t0[2], t0[3], t0[4], t0[5], imxz, imyz, imzz, szz,
                              tx_[0], tx_[1], tx_[2], tx_[3], ty_[0], ty_[1], ty_[2], ty_[4]};
+            if (kk + 1 < ke) {
+                const int64_t pn = p + plane;
+                c0 = c.cls[pn]; cx = c.cls[pn + dxm]; cy = c.cls[pn + dym];
+            }
         } else {
             cv = ZmCView{c.im[0][p], c.im[1][p], c.im[2][p], c.sg[0][p], c.sg[1][p], c.sg[2][p],
                          imxz, imyz, imzz, szz,
                          c.im[0] + p, c.im[1] + p, c.im[2] + p, c.sg[0] + p, c.sg[1] + p, dxm, dym};
         }
+        stage(kk + 1, sp);
+        __syncthreads();
         const LdsView<TX> xv{lds, {sm * S::PL, s0 * S::PL, sp * S::PL}, ctr};
         if (MODE == OP_VA) {
             if (live && free_v(g, i, j, kk)) {
@@ -594,7 +639,11 @@ template <typename TIN, typename TOUT, int MODE, int TX, bool CLS>
 __global__ void __launch_bounds__(256) k_operator(Grid g, ZTiling t, CoeffsD c, OpK k, const TIN *__restrict__ x,
                                                   const double *__restrict__ b, TOUT *__restrict__ out) {
     __shared__ double lds[4 * 3 * ZShape<TX>::PL];
-    op_column<TIN, TOUT, MODE, TX, CLS>(g, t, xcd_tile(blockIdx.x, gridDim.x), c, k, x, b, out, lds);
+    // The unrolled staging is used where the kernel still fits in 128 VGPRs
+    // (four waves per SIMD) with it: the class-table apply on 32-wide
+    // columns.  Elsewhere it costs a wave per SIMD and runs slower.
+    constexpr bool STG = MODE == OP_APPLY && TX == 32 && CLS;
+    op_column<TIN, TOUT, MODE, TX, CLS, STG>(g, t, xcd_tile(blockIdx.x, gridDim.x), c, k, x, b, out, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -607,9 +656,14 @@ __global__ void __launch_bounds__(256) k_operator(Grid g, ZTiling t, CoeffsD c, 
 // Every loop bound is workgroup-uniform and every workgroup leaves after its
 // range, so the grid always drains.
 // ---------------------------------------------------------------------------
+// TX = 0: the column width and coefficient form are read per design (any mix
+// in one launch).  TX = 32 / 64: every design of the launch uses that width
+// and, per CLS, the class table or the streamed arrays; only that variant is
+// compiled in, so the kernel needs the registers and LDS of one variant.
+template <int TX, bool CLS>
 __global__ void __launch_bounds__(256) k_operator_batch(const BatchItem *__restrict__ items, int nitems,
                                                         unsigned total) {
-    __shared__ double lds[4 * 3 * ZPL_MAX];
+    __shared__ double lds[4 * 3 * (TX ? ZShape<TX ? TX : 64>::PL : ZPL_MAX)];
     const unsigned xcd = blockIdx.x & 7u, per = gridDim.x >> 3, l = blockIdx.x >> 3;
     const unsigned lo = (unsigned)(((uint64_t)total * xcd) >> 3), hi = (unsigned)(((uint64_t)total * (xcd + 1)) >> 3);
     int d = 0;
@@ -617,7 +671,9 @@ __global__ void __launch_bounds__(256) k_operator_batch(const BatchItem *__restr
         while (d + 1 < nitems && items[d + 1].col0 <= w) ++d;     // w only grows: forward scan
         const BatchItem &it = items[d];
         const unsigned col = w - it.col0;
-        if (it.c.cls) {
+        if constexpr (TX != 0) {
+            op_column<double, double, OP_APPLY, TX, CLS>(it.g, it.t, col, it.c, it.k, it.x, nullptr, it.y, lds);
+        } else if (it.c.cls) {
             if (it.t.tx == 32)
                 op_column<double, double, OP_APPLY, 32, true>(it.g, it.t, col, it.c, it.k, it.x, nullptr, it.y, lds);
             else
@@ -808,12 +864,20 @@ static void launch_operator(const Grid &g, const CoeffsD &c, const OpParams &op,
 
 // blocks: persistent grid size (rounded up to a multiple of 8 XCDs), or 0 for
 // one workgroup per column.
-void launch_operator_batch(const BatchItem *items_dev, int nitems, unsigned total, int blocks, hipStream_t s) {
+// tx, cls: the column width and coefficient form every design of the launch
+// shares, or tx = 0 when they differ between designs.
+void launch_operator_batch(const BatchItem *items_dev, int nitems, unsigned total, int blocks, int tx, int cls,
+                           hipStream_t s) {
     unsigned g = blocks > 0 ? (unsigned)blocks : total;
     g = (g + 7u) & ~7u;
     if (g > ((total + 7u) & ~7u)) g = (total + 7u) & ~7u;
     if (g < 8u) g = 8u;
-    hipLaunchKernelGGL(k_operator_batch, dim3(g), dim3(256), 0, s, items_dev, nitems, total);
+    const dim3 grid(g), block(256);
+    if (tx == 32 && cls) hipLaunchKernelGGL((k_operator_batch<32, true>), grid, block, 0, s, items_dev, nitems, total);
+    else if (tx == 32) hipLaunchKernelGGL((k_operator_batch<32, false>), grid, block, 0, s, items_dev, nitems, total);
+    else if (tx == 64 && cls) hipLaunchKernelGGL((k_operator_batch<64, true>), grid, block, 0, s, items_dev, nitems, total);
+    else if (tx == 64) hipLaunchKernelGGL((k_operator_batch<64, false>), grid, block, 0, s, items_dev, nitems, total);
+    else hipLaunchKernelGGL((k_operator_batch<0, false>), grid, block, 0, s, items_dev, nitems, total);
 }
 
 void launch_coeffs(const Grid &g, const uint16_t *slot, const double *mat, const Coeffs &c, hipStream_t s) {

@@ -726,6 +726,7 @@ struct qms_batch {
     std::vector<BatchItem> items;
     unsigned total = 0;
     int blocks = 0;            // persistent grid size, 0 = one workgroup per column
+    int tx = 0, cls = 0;       // column width / class-table use shared by all items (tx 0: mixed)
     std::string err;
     ~qms_batch() {
         if (s) hipStreamSynchronize(s);
@@ -746,6 +747,7 @@ int bfail(qms_batch *b, const std::exception &e, int code) {
 // use each design's own dt.  Waits for every handle's stream first.
 void batch_items(qms_batch *b, const double *v, int transient) {
     b->total = 0;
+    b->tx = -1;
     for (size_t d = 0; d < b->hs.size(); ++d) {
         qms_handle *h = b->hs[d];
         check(hipStreamSynchronize(h->s), "sync");
@@ -761,6 +763,9 @@ void batch_items(qms_batch *b, const double *v, int transient) {
         it.y = h->w;
         it.col0 = b->total;
         it.ncol = ncol;
+        const int icls = it.c.cls ? 1 : 0;
+        if (b->tx == -1) { b->tx = it.t.tx; b->cls = icls; }
+        else if (b->tx != it.t.tx || b->cls != icls) b->tx = 0;
         if ((uint64_t)b->total + ncol > 0xffffffffull) throw std::runtime_error("batch too large");
         b->total += ncol;
     }
@@ -771,7 +776,7 @@ void batch_items(qms_batch *b, const double *v, int transient) {
 
 void batch_launch(qms_batch *b, int mode, const double *v, int transient) {
     if (mode == 0) {
-        launch_operator_batch(b->items_dev, (int)b->items.size(), b->total, b->blocks, b->s);
+        launch_operator_batch(b->items_dev, (int)b->items.size(), b->total, b->blocks, b->tx, b->cls, b->s);
     } else {
         for (size_t d = 0; d < b->hs.size(); ++d) {
             qms_handle *h = b->hs[d];

@@ -59,14 +59,16 @@ def cpu_baseline():
             "dof_steps_per_s": d.dof / t}
 
 
-SHAPES = ((64, 2), (64, 3), (64, 4), (64, 6), (64, 8), (32, 4))     # (column width, z-chunk)
+SHAPES = ((64, 4), (64, 6), (64, 8), (32, 4), (32, 6), (32, 8), (32, 12), (32, 16))   # (column width, z-chunk)
 
 
-def traffic_from_profile(kernel: str):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
-    summary (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by what the
-    counter reads for the plain copy kernel of known size), or None."""
-    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}.txt")
+def traffic_from_profile(kernel: str, shape: str):
+    """HBM bytes per launch of `kernel` at column shape `shape` ("32x8": column
+    width x z-chunk) from the committed rocprofv3 PMC summary (FETCH_SIZE and
+    WRITE_SIZE per dispatch, each scaled by what the counter reads for the
+    plain copy kernel of known size), or None when no summary of that shape
+    is committed."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{shape}.txt")
     if not os.path.exists(path):
         return None, None
     kv = {}
@@ -106,7 +108,7 @@ def batch_config4(reps: int, ndesigns: int = 64):
         # best shape of each design on its own, then one launch per design
         for s_, vd in zip(ss, v):
             best = None
-            for cw_, kz_ in SHAPES + ((32, 2), (32, 8)):
+            for cw_, kz_ in SHAPES + ((64, 2), (32, 2)):
                 s_.set_colwidth(cw_)
                 s_.set_zchunk(kz_)
                 s_.time_kernel(native.K_APPLY, 2, v=vd)
@@ -198,7 +200,7 @@ def main():
     except native.QmsError as e:
         status = str(e)
     s.close()
-    traffic, tsrc = traffic_from_profile("k_operator_apply_config3_128" + ("_classes" if ncls else ""))
+    traffic, tsrc = traffic_from_profile("k_operator_apply_config3_128" + ("_classes" if ncls else ""), f"{cw}x{kz}")
     line = {
         "metric": "A-V time steps/s at 128^3, mixed vs FP64 @1e-8 rel err; served p50/p99 latency",
         "value": None,
