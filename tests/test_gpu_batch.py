@@ -78,7 +78,7 @@ def test_batch_mixed_zchunks_and_tuner(batch_setup):
         for k, s in enumerate(ss):
             assert np.array_equal(s.get_work(), ref[k]), k
         kz, cw, blocks = b.tune(v, reps=2)
-        assert all(c in (2, 3, 4, 6, 8) for c in kz) and all(c in (32, 64) for c in cw)
+        assert all(c in (2, 3, 4, 6, 8, 16) for c in kz) and all(c in (32, 64) for c in cw)
         assert blocks in (0, 1024, 2048, 4096, 8192)
         b.apply(v)
         for k, s in enumerate(ss):
@@ -88,6 +88,31 @@ def test_batch_mixed_zchunks_and_tuner(batch_setup):
         assert ms_b > 0 and ms_s > 0 and by == by2 > 0
     for s in ss:
         s.use_coef_classes(True)
+
+
+@pytest.mark.parametrize("width,classes", [(32, True), (32, False), (64, True), (64, False)])
+def test_batch_uniform_shape_bit_identical(batch_setup, width, classes):
+    """Designs sharing one column width and coefficient form run in the
+    batch kernel compiled for that variant alone; its rows are the same bits
+    as the single-design launches."""
+    ds, ss, xs, v = batch_setup
+    for k, s in enumerate(ss):
+        s.set_colwidth(width)
+        s.set_zchunk((3, 16)[k % 2])
+        s.use_coef_classes(classes)
+    try:
+        ref = _single(ss, xs, v, True)
+        with native.Batch(ss) as b:
+            for blocks in (0, 1024):
+                b.set_blocks(blocks)
+                b.apply(v)
+                for k, s in enumerate(ss):
+                    assert np.array_equal(s.get_work(), ref[k]), (width, classes, blocks, k)
+    finally:
+        for s in ss:
+            s.set_colwidth(64)
+            s.set_zchunk(4)
+            s.use_coef_classes(True)
 
 
 def test_batch_member_matches_oracle(batch_setup):
