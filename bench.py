@@ -20,8 +20,9 @@ The `batch` object measures the config-4 workload (64 uneven designs): the
 operator of every design in one persistent batched launch against one
 launch per design.  The CPU baseline is the oracle (CSR assembly +
 equilibrated SuperLU factor/solve, the restated reference algorithm) timed
-on one core over one ramp step of a 24^3-node design drawn from the same
-box distribution.
+on one core over ramp steps 1-3 of a 24^3-node design drawn from the same
+box distribution.  `operator_sizes` repeats the apply at 32^3, 64^3, 96^3 and
+160^3 nodes, each at its fastest column shape.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--zchunk KZ --colwidth CW] [--no-cpu-baseline] [--no-batch]
 """
@@ -47,15 +48,17 @@ def cpu_baseline():
     coef = O.coeffs(p)
     M, b = O.assemble(p, coef, O.steady_velocity(p))
     f = O.unpack(p, O.DirectSolver(M).solve(b))
-    v = O.step_velocity(p, 1)
+    nsteps = 3
     t0 = time.perf_counter()
-    M, b = O.assemble(p, coef, v, p.dt, f)      # per-step CSR rebuild (QMS_3D.f90:1452-2191)
-    x = O.DirectSolver(M).solve(b)              # factor + solve (phase 23: velocity changed)
-    O.unpack(p, x)
-    t = time.perf_counter() - t0
+    for n in range(1, nsteps + 1):
+        v = O.step_velocity(p, n)
+        M, b = O.assemble(p, coef, v, p.dt, f)      # per-step CSR rebuild (QMS_3D.f90:1452-2191)
+        x = O.DirectSolver(M).solve(b)              # factor + solve (phase 23: the velocity changed)
+        f = O.unpack(p, x)
+    t = (time.perf_counter() - t0) / nsteps
     return {"value": 1.0 / t, "unit": "steps/s", "cores": 1, "kind": "port",
-            "sample": f"one ramp step of a 24^3-node design ({d.dof} DoF, 8 boxes, v ramp to 5 m/s): "
-                      f"CSR assembly + row-equilibrated SuperLU factor and solve, {t:.2f} s",
+            "sample": f"ramp steps 1-3 of a 24^3-node design ({d.dof} DoF, 8 boxes, v ramp to 5 m/s): "
+                      f"CSR assembly + row-equilibrated SuperLU factor and solve, {t:.2f} s per step",
             "dof_steps_per_s": d.dof / t}
 
 
@@ -128,6 +131,47 @@ def batch_config4(reps: int, ndesigns: int = 64):
             "batched_ms": ms_b, "sequential_tuned_ms": ms_s, "sequential_default_ms": seq0,
             "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS, "dof_applies_per_s": dof / (ms_b * 1e-3),
             "speedup_vs_sequential": ms_s / ms_b}
+
+
+def size_scan(reps: int, sizes=(32, 64, 96, 160)):
+    """The same apply (config-3 box layout and v = 5 m/s rows, spacing scaled
+    to the size) at other node counts per axis: config 1/2 sizes and the
+    largest served size, each at the fastest column shape of the sweep (a
+    small design alone cannot fill the GPU: 32^3 nodes are 128 columns of
+    32 x 8 nodes even one plane deep).  Algorithmic bytes and timing as for
+    the headline."""
+    import numpy as np
+    from alpineqms_amd import configs, native, program
+    from alpineqms_amd.qms_io import material_slots
+    out = {}
+    for n in sizes:
+        d = configs.config3(N=n, ramp=True)
+        inp = d.inputs
+        slots, props = material_slots(inp, d.geom)
+        s = native.Solver(inp.cells, inp.h, inp.dt, inp.bc_vector(), slots, props, restart=1)
+        nx, ny, nz = d.nodes
+        s.set_fields(np.random.default_rng(n).normal(size=(4, nz, ny, nx)))
+        v = program.step_velocity(inp.traj, inp.dt, inp.steps)
+        best = None
+        for cw, kz in SHAPES + ((64, 1), (64, 2), (32, 1), (32, 2)):
+            s.set_colwidth(cw)
+            s.set_zchunk(kz)
+            s.time_kernel(native.K_APPLY, 2, v=v)
+            ms = s.time_kernel(native.K_APPLY, 5, v=v)[0]
+            if best is None or ms < best[0]:
+                best = (ms, cw, kz)
+        s.set_colwidth(best[1])
+        s.set_zchunk(best[2])
+        s.time_kernel(native.K_APPLY, 3, v=v)
+        ms, nbytes = s.time_kernel(native.K_APPLY, reps, v=v)
+        s.close()
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        out[str(n)] = {"dof": d.dof, "column": best[1], "zchunk": best[2], "ms_per_launch": ms,
+                       "algorithmic_bytes": nbytes, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS,
+                       "dof_applies_per_s": d.dof / (ms * 1e-3)}
+        print(f"bench: apply at {n}^3 nodes (column {best[1]}, z-chunk {best[2]}): {ms * 1e3:.1f} us, "
+              f"{gbs / HBM_PEAK_GBS:.2f} of HBM peak", file=sys.stderr, flush=True)
+    return out
 
 
 def main():
@@ -234,6 +278,7 @@ def main():
                                            "achieved_GBs": bytes_st / (ms_st * 1e-3) / 1e9,
                                            "frac": bytes_st / (ms_st * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "kernels": others,
+        "operator_sizes": size_scan(a.steps),
         "batch": None if a.no_batch else batch_config4(a.steps),
         "cpu_baseline": None if a.no_cpu_baseline else cpu_baseline(),
     }
