@@ -8,7 +8,6 @@
 namespace qms {
 
 static constexpr int RB = 256;          // threads per reduction block
-static constexpr int CH = 8;            // vectors per multi-dot chunk
 
 int reduce_blocks(int64_t n) {
     int64_t b = (n + RB * 8 - 1) / (RB * 8);
@@ -24,85 +23,135 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Block-wide sum of CH accumulators; thread 0 ends with the totals in acc.
-__device__ __forceinline__ void block_sum(double acc[CH], double *sh /* [RB/64][CH] */) {
+// Block-wide sum of K accumulators; thread 0 ends with the totals in acc.
+template <int K>
+__device__ __forceinline__ void block_sum(double acc[K], double *sh /* [RB/64][K] */) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    // one accumulator at a time: interleaving K butterflies would hold K sets
+    // of temporaries and cut the kernel's occupancy
 #pragma unroll
-    for (int c = 0; c < CH; ++c) acc[c] = wave_sum(acc[c]);
-    if (lane == 0)
-        for (int c = 0; c < CH; ++c) sh[wid * CH + c] = acc[c];
+    for (int c = 0; c < K; ++c) {
+        acc[c] = wave_sum(acc[c]);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int c = 0; c < K; ++c) sh[wid * K + c] = acc[c];
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int c = 0; c < CH; ++c) {
+#pragma unroll
+        for (int c = 0; c < K; ++c) {
             double s = 0.0;
-            for (int w = 0; w < RB / 64; ++w) s += sh[w * CH + c];
+            for (int w = 0; w < RB / 64; ++w) s += sh[w * K + c];
             acc[c] = s;
         }
     }
 }
 
-// partial[i*nblk + blk] = sum over this block's elements of V_i . w, i in [c0, c0+CH)
+// Basis vectors are read through pointers loaded from memory; marking them
+// global (rather than generic) lets every vector load be a global load from
+// a wave-uniform base.
+typedef const __attribute__((address_space(1))) double *gcptr;
+typedef __attribute__((address_space(1))) double *gptr;
+
+// Element offsets: I = uint32_t when every byte offset fits 32 bits (then all
+// loads of an element share one 32-bit offset on top of wave-uniform base
+// addresses), int64_t otherwise.
+template <typename P, typename I>
+__device__ __forceinline__ auto &at(P p, I off) {
+    return *(P)((__attribute__((address_space(1))) char *)p + off);
+}
+
+// partial[i*nblk + blk] = sum over this block's elements of V_i . w for the
+// up-to-K vectors i in [c0, c0 + min(K, nv - c0)).  Every element of w is
+// read once per chunk; the K loads of an element are independent and
+// unconditional (slots past the chunk re-read vector c0, from cache, and
+// their sums are dropped).
+template <int K, typename I>
 __global__ void __launch_bounds__(RB) k_multidot(const double *const *__restrict__ V, int nv, int c0,
                                                  const double *__restrict__ w, int64_t n,
                                                  double *__restrict__ partial) {
-    __shared__ double sh[(RB / 64) * CH];
-    double acc[CH];
+    __shared__ double sh[(RB / 64) * K];
+    double acc[K];
+    gcptr vp[K];
+    const int cn = min(K, nv - c0);
 #pragma unroll
-    for (int c = 0; c < CH; ++c) acc[c] = 0.0;
-    const int cn = min(CH, nv - c0);
-    const double *vp[CH];
-#pragma unroll
-    for (int c = 0; c < CH; ++c) vp[c] = (c < cn) ? V[c0 + c] : nullptr;
-    for (int64_t e = (int64_t)blockIdx.x * RB + threadIdx.x; e < n; e += (int64_t)gridDim.x * RB) {
-        const double we = w[e];
-#pragma unroll
-        for (int c = 0; c < CH; ++c)
-            if (c < cn) acc[c] += vp[c][e] * we;
+    for (int c = 0; c < K; ++c) {
+        acc[c] = 0.0;
+        vp[c] = (gcptr)V[c0 + (c < cn ? c : 0)];
     }
-    block_sum(acc, sh);
-    if (threadIdx.x == 0)
-        for (int c = 0; c < cn; ++c) partial[(int64_t)(c0 + c) * gridDim.x + blockIdx.x] = acc[c];
+    const gcptr wg = (gcptr)w;
+    const I nb = (I)n * (I)sizeof(double), step = (I)gridDim.x * RB * (I)sizeof(double);
+    for (I off = ((I)blockIdx.x * RB + threadIdx.x) * (I)sizeof(double); off < nb; off += step) {
+        const double we = at(wg, off);
+#pragma unroll
+        for (int c = 0; c < K; ++c) acc[c] = fma(at(vp[c], off), we, acc[c]);
+    }
+    block_sum<K>(acc, sh);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int c = 0; c < K; ++c)
+            if (c < cn) partial[(int64_t)(c0 + c) * gridDim.x + blockIdx.x] = acc[c];
+    }
 }
 
 __global__ void k_reduce_partials(const double *__restrict__ partial, int nblk, double *__restrict__ out) {
-    __shared__ double sh[(RB / 64) * CH];
-    double acc[CH];
-#pragma unroll
-    for (int c = 0; c < CH; ++c) acc[c] = 0.0;
+    __shared__ double sh[RB / 64];
+    double acc[1] = {0.0};
     const double *p = partial + (int64_t)blockIdx.x * nblk;
     for (int b = threadIdx.x; b < nblk; b += RB) acc[0] += p[b];
-    block_sum(acc, sh);
+    block_sum<1>(acc, sh);
     if (threadIdx.x == 0) out[blockIdx.x] = acc[0];
 }
 
-// w -= sum_i h_i V_i; partial[blk] = sum of w^2 over the block
-__global__ void __launch_bounds__(RB) k_orth_update(double *const *__restrict__ V, int nv,
+// w -= sum_{i in [c0, c0 + min(K, nv - c0))} h_i V_i, one chunk of up to K
+// basis vectors per pass over w (the pointers and coefficients of the chunk
+// sit in scalar registers; slots past the chunk re-read vector c0 with a
+// zero coefficient).  The pass that ends the basis also writes
+// partial[blk] = sum of w^2 over the block.  Chunks run in order, so the
+// subtractions happen in basis order whatever the chunking.
+template <int K, typename I>
+__global__ void __launch_bounds__(RB) k_orth_update(double *const *__restrict__ V, int nv, int c0,
                                                     const double *__restrict__ h, double *__restrict__ w, int64_t n,
                                                     double *__restrict__ partial) {
-    __shared__ double sh[(RB / 64) * CH];
-    double acc[CH];
+    __shared__ double sh[RB / 64];
+    gcptr vp[K];
+    double hc[K];
+    const int cn = min(K, nv - c0);
 #pragma unroll
-    for (int c = 0; c < CH; ++c) acc[c] = 0.0;
-    for (int64_t e = (int64_t)blockIdx.x * RB + threadIdx.x; e < n; e += (int64_t)gridDim.x * RB) {
-        double we = w[e];
-        for (int i = 0; i < nv; ++i) we -= h[i] * V[i][e];
-        w[e] = we;
+    for (int c = 0; c < K; ++c) {
+        vp[c] = (gcptr)V[c0 + (c < cn ? c : 0)];
+        hc[c] = c < cn ? h[c0 + c] : 0.0;
+    }
+    const bool last = c0 + cn == nv;
+    double acc[1] = {0.0};
+    const gptr wg = (gptr)w;
+    const I nb = (I)n * (I)sizeof(double), step = (I)gridDim.x * RB * (I)sizeof(double);
+    for (I off = ((I)blockIdx.x * RB + threadIdx.x) * (I)sizeof(double); off < nb; off += step) {
+        double ve[K];
+#pragma unroll
+        for (int c = 0; c < K; ++c) ve[c] = at(vp[c], off);
+        double we = at(wg, off);
+#pragma unroll
+        for (int c = 0; c < K; ++c) we -= hc[c] * ve[c];
+        at(wg, off) = we;
         acc[0] += we * we;
     }
-    block_sum(acc, sh);
-    if (threadIdx.x == 0) partial[blockIdx.x] = acc[0];
+    if (last) {
+        block_sum<1>(acc, sh);
+        if (threadIdx.x == 0) partial[blockIdx.x] = acc[0];
+    }
 }
 
 __global__ void __launch_bounds__(RB) k_norm2(const double *__restrict__ w, int64_t n, double *__restrict__ partial) {
-    __shared__ double sh[(RB / 64) * CH];
-    double acc[CH];
-#pragma unroll
-    for (int c = 0; c < CH; ++c) acc[c] = 0.0;
+    __shared__ double sh[RB / 64];
+    double acc[1] = {0.0};
     for (int64_t e = (int64_t)blockIdx.x * RB + threadIdx.x; e < n; e += (int64_t)gridDim.x * RB) {
         const double we = w[e];
         acc[0] += we * we;
     }
-    block_sum(acc, sh);
+    block_sum<1>(acc, sh);
     if (threadIdx.x == 0) partial[blockIdx.x] = acc[0];
 }
 
@@ -132,17 +181,37 @@ void launch_copy(double *dst, const double *src, int64_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dst, src, n);
 }
 
+static bool offsets32(int64_t n) { return n < ((int64_t)1 << 28); }   // byte offsets of n doubles (+ one grid stride) < 2^32
+
 void launch_multidot(const double *const *V, int nv, const double *w, int64_t n, double *partial, int nblk,
                      hipStream_t s) {
-    for (int c0 = 0; c0 < nv; c0 += CH)
-        hipLaunchKernelGGL(k_multidot, dim3(nblk), dim3(RB), 0, s, V, nv, c0, w, n, partial);
+    // chunks of up to 16 vectors; the narrower kernel when the chunk fits it
+    for (int c0 = 0; c0 < nv; c0 += 16) {
+        const int cn = nv - c0 < 16 ? nv - c0 : 16;
+        if (offsets32(n)) {
+            if (cn <= 8) hipLaunchKernelGGL((k_multidot<8, uint32_t>), dim3(nblk), dim3(RB), 0, s, V, nv, c0, w, n, partial);
+            else hipLaunchKernelGGL((k_multidot<16, uint32_t>), dim3(nblk), dim3(RB), 0, s, V, nv, c0, w, n, partial);
+        } else {
+            if (cn <= 8) hipLaunchKernelGGL((k_multidot<8, int64_t>), dim3(nblk), dim3(RB), 0, s, V, nv, c0, w, n, partial);
+            else hipLaunchKernelGGL((k_multidot<16, int64_t>), dim3(nblk), dim3(RB), 0, s, V, nv, c0, w, n, partial);
+        }
+    }
 }
 void launch_reduce_partials(const double *partial, int nv, int nblk, double *out, hipStream_t s) {
     hipLaunchKernelGGL(k_reduce_partials, dim3(nv), dim3(RB), 0, s, partial, nblk, out);
 }
 void launch_orth_update(double *const *V, int nv, const double *h, double *w, int64_t n, double *partial, int nblk,
                         hipStream_t s) {
-    hipLaunchKernelGGL(k_orth_update, dim3(nblk), dim3(RB), 0, s, V, nv, h, w, n, partial);
+    for (int c0 = 0; c0 < nv; c0 += 16) {
+        const int cn = nv - c0 < 16 ? nv - c0 : 16;
+        if (offsets32(n)) {
+            if (cn <= 8) hipLaunchKernelGGL((k_orth_update<8, uint32_t>), dim3(nblk), dim3(RB), 0, s, V, nv, c0, h, w, n, partial);
+            else hipLaunchKernelGGL((k_orth_update<16, uint32_t>), dim3(nblk), dim3(RB), 0, s, V, nv, c0, h, w, n, partial);
+        } else {
+            if (cn <= 8) hipLaunchKernelGGL((k_orth_update<8, int64_t>), dim3(nblk), dim3(RB), 0, s, V, nv, c0, h, w, n, partial);
+            else hipLaunchKernelGGL((k_orth_update<16, int64_t>), dim3(nblk), dim3(RB), 0, s, V, nv, c0, h, w, n, partial);
+        }
+    }
 }
 void launch_norm2(const double *w, int64_t n, double *partial, int nblk, hipStream_t s) {
     hipLaunchKernelGGL(k_norm2, dim3(nblk), dim3(RB), 0, s, w, n, partial);

@@ -266,6 +266,16 @@ double norm_dev(qms_handle *h, const double *w) {
     return sqrt(h->hsmall[0]);
 }
 
+// Classical Gram-Schmidt of h->w against basis vectors V[0..nv-1]: the dots
+// land in dsmall[0..nv-1], w is updated in place and |w|^2 lands in
+// dsmall[nv] (one fused pass for the dots, one for the update).
+void gram_schmidt(qms_handle *h, int nv) {
+    launch_multidot((const double *const *)h->dV, nv, h->w, h->n, h->partial, h->nblk, h->s);
+    launch_reduce_partials(h->partial, nv, h->nblk, h->dsmall, h->s);
+    launch_orth_update(h->dV, nv, h->dsmall, h->w, h->n, h->partial, h->nblk, h->s);
+    launch_reduce_partials(h->partial, 1, h->nblk, h->dsmall + nv, h->s);
+}
+
 // Restarted FGMRES with FP64 residual recomputation at each restart.
 template <typename T>
 int fgmres(qms_handle *h, const OpParams &op, int tr) {
@@ -304,11 +314,7 @@ int fgmres(qms_handle *h, const OpParams &op, int tr) {
         for (int j = 0; j < m && total < h->cfg.max_iters; ++j) {
             precond<T>(h, op, tr, D, h->V[j], h->Z[j]);
             launch_apply(h->g, op_coeffs(h), op, h->Z[j], h->w, s);
-            // classical Gram-Schmidt, one fused pass for the dots, one for the update
-            launch_multidot((const double *const *)h->dV, j + 1, h->w, n, h->partial, h->nblk, s);
-            launch_reduce_partials(h->partial, j + 1, h->nblk, h->dsmall, s);
-            launch_orth_update(h->dV, j + 1, h->dsmall, h->w, n, h->partial, h->nblk, s);
-            launch_reduce_partials(h->partial, 1, h->nblk, h->dsmall + (j + 1), s);
+            gram_schmidt(h, j + 1);
             check(hipMemcpyAsync(h->hsmall, h->dsmall, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, s), "d2h");
             check(hipStreamSynchronize(s), "sync");
             for (int i = 0; i <= j; ++i) H[(size_t)i * m + j] = h->hsmall[i];
@@ -638,7 +644,7 @@ int qms_set_colwidth(qms_handle *h, int nodes) {
 
 int qms_time_kernel(qms_handle *h, int which, const double v[3], int reps, double *ms_per_launch,
                     double *bytes_per_launch) {
-    if (!h || reps < 1 || !ms_per_launch || !bytes_per_launch || which < 0 || which > 6) return QMS_ERR_ARG;
+    if (!h || reps < 1 || !ms_per_launch || !bytes_per_launch || which < 0 || which > 7) return QMS_ERR_ARG;
     QMS_TRY(h)
     const int tr = h->cfg.dt > 0 ? 1 : 0;
     const double v0[3] = {0, 0, 0};
@@ -665,6 +671,8 @@ int qms_time_kernel(qms_handle *h, int which, const double v[3], int reps, doubl
             launch_hfield(h->g, op_coeffs(h), h->x, h->tmp, 0, h->s);
         } else if (which == 6) {
             launch_resid(h->g, op_coeffs(h), op, h->x, h->b, h->w, h->s);
+        } else if (which == 7) {
+            gram_schmidt(h, h->cfg.restart + 1);
         } else {
             const Fine f = h->fine(tr ? h->cfg.dt : 0.0);
             const MGParams mp = mgp(h);
@@ -698,9 +706,34 @@ int qms_time_kernel(qms_handle *h, int which, const double v[3], int reps, doubl
     case 3: *bytes_per_launch = 16.0 * N; break;                             // read + write of the state
     case 4: *bytes_per_launch = 8.0 * (3.0 * N1 + 3.0 * N1 + 4.0 * N1); break; // A^n, sigma, b
     case 5: *bytes_per_launch = 8.0 * (6.0 * N1 + 3.0 * NC); break;          // A, 1/Mu, H
+    case 7: {                                                                 // dots: m+1 basis + w; update: m+1 basis + w read + w written
+        const double m1 = (double)(h->cfg.restart + 1);
+        *bytes_per_launch = 8.0 * N * ((m1 + 1.0) + (m1 + 2.0));
+        break;
+    }
     case 1: *bytes_per_launch = s * (3.0 * 3.0 * N1 + 6.0 * N1); break;
     default: *bytes_per_launch = s * (3.0 * N1 + 3.0 * N1); break;
     }
+    return QMS_OK;
+    QMS_CATCH(h)
+}
+
+int qms_gram_schmidt(qms_handle *h, int nv, const double *basis, const double *w, double *hcoef, double *wout) {
+    if (!h || !basis || !w || !hcoef || !wout) return QMS_ERR_ARG;
+    if (nv < 1 || nv > h->cfg.restart + 1) {
+        h->err = "qms_gram_schmidt: nv must be in [1, restart + 1]";
+        return QMS_ERR_ARG;
+    }
+    QMS_TRY(h)
+    const int64_t n = h->n;
+    for (int i = 0; i < nv; ++i)
+        check(hipMemcpyAsync(h->V[i], basis + (size_t)i * n, sizeof(double) * n, hipMemcpyHostToDevice, h->s), "h2d");
+    check(hipMemcpyAsync(h->w, w, sizeof(double) * n, hipMemcpyHostToDevice, h->s), "h2d");
+    gram_schmidt(h, nv);
+    check(hipMemcpyAsync(h->hsmall, h->dsmall, sizeof(double) * (nv + 1), hipMemcpyDeviceToHost, h->s), "d2h");
+    check(hipMemcpyAsync(wout, h->w, sizeof(double) * n, hipMemcpyDeviceToHost, h->s), "d2h");
+    check(hipStreamSynchronize(h->s), "sync");
+    memcpy(hcoef, h->hsmall, sizeof(double) * (nv + 1));
     return QMS_OK;
     QMS_CATCH(h)
 }

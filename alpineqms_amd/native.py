@@ -27,13 +27,13 @@ QMS_MIXED = 1
 EXPORTS = (
     "qms_config_default", "qms_create", "qms_steady", "qms_step", "qms_run",
     "qms_get_fields", "qms_set_fields", "qms_get_h", "qms_apply", "qms_residual", "qms_rhs",
-    "qms_get_stats", "qms_set_zchunk", "qms_set_colwidth", "qms_use_coef_classes", "qms_time_kernel", "qms_get_work", "qms_destroy", "qms_last_error",
+    "qms_get_stats", "qms_set_zchunk", "qms_set_colwidth", "qms_use_coef_classes", "qms_time_kernel", "qms_gram_schmidt", "qms_get_work", "qms_destroy", "qms_last_error",
     "qms_batch_create", "qms_batch_set_blocks", "qms_batch_apply", "qms_batch_tune", "qms_batch_time",
     "qms_batch_last_error", "qms_batch_destroy",
 )
 
 # qms_time_kernel selectors
-K_APPLY, K_SMOOTH_A, K_SMOOTH_V, K_COPY, K_RHS, K_HFIELD, K_RESID = range(7)
+K_APPLY, K_SMOOTH_A, K_SMOOTH_V, K_COPY, K_RHS, K_HFIELD, K_RESID, K_GRAM_SCHMIDT = range(8)
 
 
 class QmsError(RuntimeError):
@@ -106,6 +106,7 @@ def load(path: Optional[str] = None):
     L.qms_set_zchunk.argtypes = [H, ctypes.c_int]
     L.qms_time_kernel.argtypes = [H, ctypes.c_int, d, ctypes.c_int, d, d]
     L.qms_get_work.argtypes = [H, d]
+    L.qms_gram_schmidt.argtypes = [H, ctypes.c_int, d, d, d, d]
     L.qms_residual.argtypes = [H, d, ctypes.c_double, d, d, d]
     L.qms_batch_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
     L.qms_batch_set_blocks.argtypes = [H, ctypes.c_int]
@@ -293,6 +294,18 @@ class Solver:
             vp = _dp(vv)
         self._check(self._L.qms_time_kernel(self._h, which, vp, reps, ctypes.byref(ms), ctypes.byref(by)))
         return ms.value, by.value
+
+    def gram_schmidt(self, basis: np.ndarray, w: np.ndarray):
+        """One Gram-Schmidt step of the solver's FGMRES on the GPU: returns
+        (h, w') with h[i] = V_i . w for the nv basis vectors, w' = w - sum h_i
+        V_i and h[nv] = w' . w'."""
+        nv = basis.shape[0]
+        bb = np.ascontiguousarray(basis, dtype=np.float64)
+        ww = np.ascontiguousarray(w, dtype=np.float64)
+        hc = np.empty(nv + 1, dtype=np.float64)
+        out = np.empty_like(ww)
+        self._check(self._L.qms_gram_schmidt(self._h, int(nv), _dp(bb), _dp(ww), _dp(hc), _dp(out)))
+        return hc, out
 
     def get_work(self) -> np.ndarray:
         """The last operator output D^-1 M x (4 node-sized arrays)."""

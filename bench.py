@@ -15,7 +15,9 @@ operator inside every Krylov iteration) with the rows of the last ramp step
 (transient, v = 5 m/s), against the HBM roofline, with HIP events on the
 stream it is launched on; its column shape and z-chunk depth (planes per
 workgroup) are picked from a sweep.  Beside it: the FP64 residual, the step right-hand side, the
-cell-centred H and a plain copy kernel of the same state (calibration).
+cell-centred H, a plain copy kernel of the same state (calibration) and
+the FGMRES Gram-Schmidt step against 31 basis vectors (multi-dot,
+reduction, update, norm).
 The `batch` object measures the config-4 workload (64 uneven designs): the
 operator of every design in one persistent batched launch against one
 launch per design.  The CPU baseline is the oracle (CSR assembly +
@@ -233,7 +235,8 @@ def main():
     s.use_coef_classes(True)
     others = {}
     for name, which in (("k_operator_resid", native.K_RESID), ("k_rhs", native.K_RHS),
-                        ("k_hfield", native.K_HFIELD), ("k_copy", native.K_COPY)):
+                        ("k_hfield", native.K_HFIELD), ("k_copy", native.K_COPY),
+                        ("gram_schmidt_31", native.K_GRAM_SCHMIDT)):
         mk, bk = timed(which)
         gbs = bk / (mk * 1e-3) / 1e9
         others[name] = {"ms_per_launch": mk, "algorithmic_bytes": bk, "achieved_GBs": gbs,

@@ -147,12 +147,23 @@ int qms_set_colwidth(qms_handle *h, int nodes);
  * which: 0 fused operator apply (FP64), 1 level-0 A-block smoother sweep,
  * 2 level-0 V-block smoother sweep (precision of the handle's multigrid),
  * 3 plain copy of the 4-field state (calibration of byte counters),
- * 4 step right-hand side, 5 cell-centred H, 6 FP64 residual b - M x.
+ * 4 step right-hand side, 5 cell-centred H, 6 FP64 residual b - M x,
+ * 7 Gram-Schmidt step of FGMRES against restart + 1 basis vectors (the
+ * largest of a restart cycle: multi-dot, reduction, update, norm).
  * Operator bytes are the workload's: x, y (and b) plus the six FP64
  * coefficient arrays, also when the kernel reads the class table instead.
  * No reference counterpart (measurement only). */
 int qms_time_kernel(qms_handle *h, int which, const double v[3], int reps, double *ms_per_launch,
                     double *bytes_per_launch);
+
+/* One classical Gram-Schmidt step of the solver's FGMRES, as it runs inside
+ * every Krylov iteration (the build's replacement of PARDISO's solve phase,
+ * QMS_3D.f90:1314-1317, :2193-2203; PARDISO itself is closed): with the nv
+ * vectors of `basis` (nv x 4 node-sized arrays, 1 <= nv <= restart + 1) and
+ * w, computes h_i = V_i . w, then w <- w - sum_i h_i V_i and h_nv = w . w,
+ * with the fused multi-dot, two-stage reduction and update kernels.
+ * Writes h[0..nv] and the updated w.  Exposed for parity tests. */
+int qms_gram_schmidt(qms_handle *h, int nv, const double *basis, const double *w, double *hcoef, double *wout);
 
 /* Copy the handle's work vector (4 node-sized arrays) to the host: the last
  * operator output D^-1 M x written by qms_apply, qms_time_kernel or

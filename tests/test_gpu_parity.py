@@ -163,7 +163,8 @@ def test_operator_largest_served_size():
 def test_time_kernel_selectors():
     d = configs.small_box_design(16, 3, nbox=2, v=1.0, ramp=False)
     with solver_for(d) as s:
-        for which in (native.K_APPLY, native.K_COPY, native.K_RHS, native.K_HFIELD, native.K_RESID):
+        for which in (native.K_APPLY, native.K_COPY, native.K_RHS, native.K_HFIELD, native.K_RESID,
+                      native.K_GRAM_SCHMIDT):
             ms, nbytes = s.time_kernel(which, 3, v=[1.0, 0.0, 0.0])
             assert ms > 0 and nbytes > 0, which
 
