@@ -425,10 +425,11 @@ __device__ __forceinline__ bool tile_node(const Grid &g, const Tiling &t, int &i
 
 // ---------------------------------------------------------------------------
 // z-marching operator.  A workgroup owns a TX x (256/TX) column of nodes
-// (TX = 64 or 32) and walks it through kz consecutive planes.  The four fields of planes k-1, k, k+1
-// (with a one-node halo in x and y) sit in an LDS ring of three slots, so
-// each field value is read from HBM once per column instead of once per
-// stencil use; the material coefficients of plane k are read per node (their
+// (TX = 64 or 32) and walks it through kz consecutive planes.  The four
+// fields of planes k-1, k, k+1 (with a one-node halo in x and y) sit in an
+// LDS ring of three slots, so each field value is read from HBM once per
+// column instead of once per stencil use.  The material coefficients of
+// plane k are read per node, from the class table or the six arrays (their
 // -x and -y neighbours hit L1), and the plane-(k-1) values they need are
 // carried in registers from the previous plane.
 // ---------------------------------------------------------------------------
