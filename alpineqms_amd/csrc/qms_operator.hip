@@ -582,6 +582,18 @@ __device__ __forceinline__ void op_column(const Grid &g, const ZTiling &t, unsig
         // The coefficients of plane kk (and the class indices of plane kk+1)
         // are requested before the plane-(kk+1) staging loads, so their
         // latencies overlap.
+        // Residual with the class table: b of the node's four rows is
+        // requested with the coefficients, so its latency overlaps the
+        // staging (19% faster at 128^3).  With streamed coefficients the
+        // registers that takes would cost a wave per SIMD, so b is read when
+        // the rows are written.
+        double bq[4] = {0.0, 0.0, 0.0, 0.0};
+        if constexpr (MODE == OP_RESID && CLS) {
+            if (live) {
+#pragma unroll
+                for (int f = 0; f < 4; ++f) bq[f] = b[f * n1 + p];
+            }
+        }
         typename std::conditional<CLS, ZmClassView, ZmCView>::type cv;
         if constexpr (CLS) {
             const double *t0 = c.ctab + 6 * c0;
@@ -616,10 +628,10 @@ __device__ __forceinline__ void op_column(const Grid &g, const ZTiling &t, unsig
                     const int64_t q = f * n1 + p;
                     double rr;
                     if (free_f(g, f, i, j, kk)) {
-                        rr = ((MODE == OP_RESID) ? b[q] - mx : mx) * recip(d);
+                        rr = ((MODE == OP_RESID) ? (CLS ? bq[f] : b[q]) - mx : mx) * recip(d);
                     } else if (exists_f(g, f, i, j, kk)) {
                         const double xq = xv.get(f, 0, 0, 0);
-                        rr = (MODE == OP_RESID) ? b[q] - xq : xq;
+                        rr = (MODE == OP_RESID) ? (CLS ? bq[f] : b[q]) - xq : xq;
                     } else {
                         rr = 0.0;
                     }

@@ -26,7 +26,8 @@ on one core over ramp steps 1-3 of a 24^3-node design drawn from the same
 box distribution.  `operator_sizes` repeats the apply at 32^3, 64^3, 96^3 and
 160^3 nodes, each at its fastest column shape.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--zchunk KZ --colwidth CW] [--no-cpu-baseline] [--no-batch]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--zchunk KZ --colwidth CW] [--no-cpu-baseline]
+       [--no-batch] [--no-sizes] [--only-batch]
 """
 import argparse
 import json
@@ -188,6 +189,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the config-4 batched operator measurement")
     ap.add_argument("--only-batch", action="store_true", help="print only the config-4 batched measurement")
+    ap.add_argument("--no-sizes", action="store_true", help="skip the apply at other node counts")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py runs on one GPU only: every design here fits one MI355X, and the multi-GPU "
@@ -281,7 +283,7 @@ def main():
                                            "achieved_GBs": bytes_st / (ms_st * 1e-3) / 1e9,
                                            "frac": bytes_st / (ms_st * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "kernels": others,
-        "operator_sizes": size_scan(a.steps),
+        "operator_sizes": None if a.no_sizes else size_scan(a.steps),
         "batch": None if a.no_batch else batch_config4(a.steps),
         "cpu_baseline": None if a.no_cpu_baseline else cpu_baseline(),
     }
