@@ -369,11 +369,11 @@ int solve(qms_handle *h, const OpParams &op, int tr) {
     return fgmres<double>(h, op, tr);
 }
 
-// The multigrid-preconditioned solve faulted the GPU in its first hierarchy
-// build on the two GPU runs made so far, with no cause found in this code (a
-// CPU build of the same sources runs the same tests clean under
-// AddressSanitizer).  Until the cause is shown, the solve entry points refuse
-// to run unless QMS_ENABLE_SOLVE=1 is set explicitly.
+// The multigrid-preconditioned solve has faulted the GPU in its first
+// hierarchy build, with no cause found in this code (a CPU build of the same
+// sources runs the same tests clean under AddressSanitizer).  Until the cause
+// is shown, the solve entry points refuse to run unless QMS_ENABLE_SOLVE=1 is
+// set explicitly.
 bool solve_enabled(qms_handle *h) {
     const char *e = getenv("QMS_ENABLE_SOLVE");
     if (e && e[0] == '1') return true;

@@ -7,7 +7,7 @@ ramped to 5 m/s over 200 steps of 2 us), the configuration the headline
 metric "A-V time steps/s at 128^3" is quoted on.
 
 Printed: one JSON line.  The time-step rate needs the multigrid solve path,
-which is disabled in this build (it faulted the GPU during its first
+which is disabled in the library for now (it faulted the GPU during its first
 hierarchy build, cause not identified), so `value` is null and says so.
 What is measured on the GPU: the fused A-V operator kernel at 128^3 (the
 matrix-free replacement of the reference's per-step CSR assembly and the
@@ -193,7 +193,7 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py runs on one GPU only: every design here fits one MI355X, and the multi-GPU "
-              "path (whole designs per GPU, no inter-GPU traffic) is not part of this build", file=sys.stderr)
+              "path (whole designs per GPU, no inter-GPU traffic) is not implemented", file=sys.stderr)
         return 2
     import numpy as np
     from alpineqms_amd import configs, native, program

@@ -157,7 +157,7 @@ int qms_time_kernel(qms_handle *h, int which, const double v[3], int reps, doubl
                     double *bytes_per_launch);
 
 /* One classical Gram-Schmidt step of the solver's FGMRES, as it runs inside
- * every Krylov iteration (the build's replacement of PARDISO's solve phase,
+ * every Krylov iteration (the library's replacement of PARDISO's solve phase,
  * QMS_3D.f90:1314-1317, :2193-2203; PARDISO itself is closed): with the nv
  * vectors of `basis` (nv x 4 node-sized arrays, 1 <= nv <= restart + 1) and
  * w, computes h_i = V_i . w, then w <- w - sum_i h_i V_i and h_nv = w . w,

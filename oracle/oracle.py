@@ -188,7 +188,7 @@ def hfield(p: Problem, coef: np.ndarray, fields: np.ndarray) -> np.ndarray:
 
 def bfield(p: Problem, fields: np.ndarray) -> np.ndarray:
     """Cell-centred B: the formula of ``:1353-1372`` with every 1/Mu equal to 1
-    (the reference writes only H; B is an addition of this build)."""
+    (the reference writes only H; B is an additional output of the library)."""
     Nx, Ny, Nz = p.nodes
     ones = np.ones((6, Nz, Ny, Nx), dtype=np.float64)
     return hfield(p, ones, fields)
