@@ -76,6 +76,17 @@ def test_operator_matches_csr_ragged_grid():
     _operator_check(d, np.array([-2.0, 0.5, 1.25]), True, (4, 7, 2, 30), seed=13)
 
 
+@pytest.mark.parametrize("cells", [(5, 5, 5), (5, 6, 7), (5, 40, 6), (70, 5, 5)])
+def test_operator_matches_csr_smallest_and_thin_grids(cells):
+    """The smallest grid the reference accepts (5 cells per axis: a 2-layer
+    Dirichlet shell around 2 free nodes) and slabs one column wide or one
+    z-chunk deep: columns and chunks larger than the design, mostly idle
+    lanes, every row near the shell."""
+    d = configs.small_box_design(0, 3 + sum(cells), nbox=1, v=1.0, ramp=False, gradient=True, cells=cells)
+    for transient in (False, True):
+        _operator_check(d, np.array([0.8, -1.1, 0.3]), transient, (1, 2, 64), seed=sum(cells))
+
+
 def test_operator_matches_csr_config2_full_size():
     """64^3 nodes, the full size of the 8-box design at 0.5 mm (1,036,288 DoF),
     against the oracle's CSR (19.4M nonzeros)."""
