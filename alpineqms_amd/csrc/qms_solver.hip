@@ -67,8 +67,8 @@ struct qms_handle {
     double *partial = nullptr, *dsmall = nullptr;
     double *hsmall = nullptr;   // pinned
     int nblk = 1;
-    int zchunk = 4;             // planes per operator workgroup
-    int colw = 64;              // operator column width (nodes)
+    int zchunk = 4;             // planes per operator workgroup (set by default_shape)
+    int colw = 32;              // operator column width (nodes)
 
     // preconditioner work vectors (4*n1 in MG precision)
     void *rT = nullptr, *zT = nullptr, *tT = nullptr;
@@ -192,6 +192,18 @@ CoeffsD op_coeffs(const qms_handle *h) {
     CoeffsD c = h->c64;
     if (h->cls && h->use_cls) { c.cls = h->cls; c.ctab = h->ctab; }
     return c;
+}
+
+// Default operator shape: 32 x 8 columns, and the deepest z-chunk of
+// 1, 2, 4, 8, 16 planes that leaves at least 768 workgroup columns (three
+// per CU of 256), so the launch still fills the GPU.
+void default_shape(qms_handle *h) {
+    h->colw = 32;
+    const int64_t cols2d = (int64_t)((h->g.nx + 31) / 32) * ((h->g.ny + 7) / 8);
+    int kz = 1;
+    for (int k : {2, 4, 8, 16})
+        if (cols2d * ((h->g.nz + k - 1) / k) >= 768) kz = k;
+    h->zchunk = kz;
 }
 
 OpParams make_op(const qms_handle *h, const double v[3], double dt, int tr) {
@@ -462,6 +474,7 @@ int qms_create(const qms_config *cfg, const uint16_t *cell_slot, const double *s
     launch_coeffs(g, h->slot, h->mat, h->coef, h->s);
     build_classes(h.get());
     h->n = 4 * g.n1;
+    default_shape(h.get());
     h->x = h->dalloc<double>(h->n);
     h->b = h->dalloc<double>(h->n);
     h->r = h->dalloc<double>(h->n);

@@ -280,7 +280,7 @@ class Solver:
         self._check(self._L.qms_use_coef_classes(self._h, 1 if on else 0))
 
     def set_colwidth(self, nodes: int):
-        """Operator column width in nodes (64 or 32)."""
+        """Operator column width in nodes (32, the default, or 64)."""
         self._check(self._L.qms_set_colwidth(self._h, int(nodes)))
 
     def time_kernel(self, which: int, reps: int, v=None):

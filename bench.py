@@ -185,7 +185,7 @@ def main():
     ap.add_argument("--nodes", type=int, default=128)
     ap.add_argument("--zchunk", type=int, default=0,
                     help="planes per operator workgroup (0: sweep column shapes and z-chunks)")
-    ap.add_argument("--colwidth", type=int, default=64, help="operator column width with --zchunk (64 or 32)")
+    ap.add_argument("--colwidth", type=int, default=32, help="operator column width with --zchunk (32 or 64)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the config-4 batched operator measurement")
     ap.add_argument("--only-batch", action="store_true", help="print only the config-4 batched measurement")

@@ -120,11 +120,13 @@ int qms_residual(qms_handle *h, const double v[3], double dt, const double *x, c
 
 int qms_get_stats(const qms_handle *h, qms_stats *st);
 
-/* Planes per operator workgroup (1..64, default 4): each 256-thread
- * workgroup of the operator kernel owns a 64 x 4 column of nodes and marches
- * it through this many z-planes, staging the fields through LDS.  A tuning
- * knob with no counterpart in the reference (whose assembly loop is serial,
- * QMS_3D.f90:448-1293). */
+/* Planes per operator workgroup (1..64): each 256-thread workgroup of the
+ * operator kernel owns a column of nodes (qms_set_colwidth) and marches it
+ * through this many z-planes, staging the fields through LDS.  qms_create
+ * picks the deepest of 1, 2, 4, 8, 16 planes that still gives at least 768
+ * columns (three per CU), the fastest choice measured at 32^3..160^3 nodes.
+ * A tuning knob with no counterpart in the reference (whose assembly loop is
+ * serial, QMS_3D.f90:448-1293). */
 int qms_set_zchunk(qms_handle *h, int planes);
 
 /* Operator coefficients from the class table (on != 0, the default when
@@ -135,10 +137,9 @@ int qms_set_zchunk(qms_handle *h, int planes);
  * Performance knob, no reference counterpart. */
 int qms_use_coef_classes(qms_handle *h, int on);
 
-/* Operator column width in nodes (64, the default, or 32): a workgroup owns
- * 64 x 4 or 32 x 8 nodes of each plane it marches through; 32 fits designs
- * whose x extent is not near a multiple of 64.  Tuning knob, no reference
- * counterpart. */
+/* Operator column width in nodes (32, the default, or 64): a workgroup owns
+ * 32 x 8 or 64 x 4 nodes of each plane it marches through.  Tuning knob, no
+ * reference counterpart. */
 int qms_set_colwidth(qms_handle *h, int nodes);
 
 /* Times `reps` launches of one kernel on the handle's stream with HIP events
