@@ -38,6 +38,22 @@ def test_config_defaults_without_gpu():
     assert c.tol == 1e-12 and c.mg_omega == 0.7 and c.mg_alpha == 1.8
 
 
+@pytest.mark.parametrize("cells,h,restart", [((4, 5, 5), 5e-4, 30), ((5, 5, 5), 0.0, 30), ((5, 5, 5), 5e-4, 0),
+                                                ((5, 5, 4097), 5e-4, 30)])
+def test_create_rejects_invalid_configs_before_touching_the_gpu(cells, h, restart):
+    """qms_create checks its arguments (cells per axis in [5, 4096], positive
+    steps, restart in [1, 64]) and returns QMS_ERR_ARG with a message before
+    any device call, so this runs without a GPU."""
+    if not os.path.exists(native.LIB_PATH):
+        pytest.skip("libqms.so not built")
+    import numpy as np
+    nc = cells[0] * cells[1] * cells[2]
+    with pytest.raises(native.QmsError) as e:
+        native.Solver(cells, (h, h, h), 1e-6, np.zeros(12), np.zeros(nc, dtype=np.uint16),
+                      np.array([[1.0, 1.0, 1.0, 0.0, 0.0, 0.0]]), restart=restart)
+    assert e.value.code == native.QMS_ERR_ARG and "qms_create" in str(e.value)
+
+
 @pytest.mark.parametrize("traj", [
     [(5.0, 0.0, 0), (0.0, 0.0, 0), (-1.0, 0.0, 0)],
     [(0.0, 1e5, 40), (2.0, 0.0, 3), (1.0, 3.0, 2)],
