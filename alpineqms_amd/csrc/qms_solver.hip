@@ -38,6 +38,28 @@ void check(hipError_t e, const char *what) {
 
 std::string g_last_error;
 
+// Mean milliseconds per call of f over `reps` calls on stream s, after one
+// untimed call, with HIP events (released on every exit path).
+double time_on(hipStream_t s, int reps, const std::function<void()> &f) {
+    struct Events {
+        hipEvent_t e[2] = {nullptr, nullptr};
+        ~Events() {
+            for (hipEvent_t x : e)
+                if (x) hipEventDestroy(x);
+        }
+    } ev;
+    check(hipEventCreate(&ev.e[0]), "event");
+    check(hipEventCreate(&ev.e[1]), "event");
+    f();
+    check(hipEventRecord(ev.e[0], s), "event");
+    for (int r = 0; r < reps; ++r) f();
+    check(hipEventRecord(ev.e[1], s), "event");
+    check(hipEventSynchronize(ev.e[1]), "event");
+    float ms = 0;
+    check(hipEventElapsedTime(&ms, ev.e[0], ev.e[1]), "event");
+    return ms / reps;
+}
+
 }  // namespace
 
 struct qms_handle {
@@ -670,9 +692,6 @@ int qms_time_kernel(qms_handle *h, int which, const double v[3], int reps, doubl
         if (!solve_enabled(h)) return QMS_ERR_STATE;
         if (mixed) ensure_mg<float>(h, tr); else ensure_mg<double>(h, tr);
     }
-    hipEvent_t e0, e1;
-    check(hipEventCreate(&e0), "event");
-    check(hipEventCreate(&e1), "event");
     auto launch = [&]() {
         if (which == 0) {
             launch_apply(h->g, op_coeffs(h), op, h->x, h->w, h->s);
@@ -698,16 +717,7 @@ int qms_time_kernel(qms_handle *h, int which, const double v[3], int reps, doubl
             }
         }
     };
-    launch();
-    check(hipEventRecord(e0, h->s), "event");
-    for (int r = 0; r < reps; ++r) launch();
-    check(hipEventRecord(e1, h->s), "event");
-    check(hipEventSynchronize(e1), "event");
-    float ms = 0;
-    check(hipEventElapsedTime(&ms, e0, e1), "event");
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    *ms_per_launch = ms / reps;
+    *ms_per_launch = time_on(h->s, reps, launch);
     // Algorithmic bytes: each input and output array streamed once.
     const double N = (double)n, N1 = (double)n1, NC = (double)nc;
     // Operator: the workload's algorithmic bytes, x and y (and b) plus the six
@@ -832,21 +842,6 @@ void batch_launch(qms_batch *b, int mode, const double *v, int transient) {
     }
 }
 
-double time_on(hipStream_t s, int reps, const std::function<void()> &f) {
-    hipEvent_t e0, e1;
-    check(hipEventCreate(&e0), "event");
-    check(hipEventCreate(&e1), "event");
-    f();
-    check(hipEventRecord(e0, s), "event");
-    for (int r = 0; r < reps; ++r) f();
-    check(hipEventRecord(e1, s), "event");
-    check(hipEventSynchronize(e1), "event");
-    float ms = 0;
-    check(hipEventElapsedTime(&ms, e0, e1), "event");
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    return ms / reps;
-}
 
 }  // namespace
 
