@@ -69,11 +69,22 @@ struct BatchItem {
     OpK k;
     const double *x;
     double *y;
-    unsigned col0, ncol;      // first workgroup column in the batch, column count
 };
+// One segment of a batched launch: planes [kb, ke) of workgroup column `col`
+// (x fastest, then y, as in ZTiling) of design d.  16 bytes, one scalar load.
+struct BatchSeg {
+    unsigned d, col;
+    int kb, ke;
+};
+// slots workgroups (a multiple of 8); the workgroup of slot s runs segments
+// [slot_off[s], slot_off[s+1]) in order, where workgroup b has slot
+// (b % 8) * (slots / 8) + b / 8, so each XCD runs one contiguous slot range.
 // tx, cls: column width and coefficient form shared by every design, or tx = 0.
-void launch_operator_batch(const BatchItem *items_dev, int nitems, unsigned total, int blocks, int tx, int cls,
-                           hipStream_t s);
+void launch_operator_batch(const BatchItem *items_dev, const BatchSeg *segs_dev, const unsigned *slot_off_dev,
+                           unsigned slots, int tx, int cls, hipStream_t s);
+// Workgroups of that kernel variant resident on the device at once (a
+// multiple of 8).
+int batch_resident_blocks(int tx, int cls);
 
 enum { OP_RESID = 0, OP_APPLY = 1, OP_VA = 2 };
 

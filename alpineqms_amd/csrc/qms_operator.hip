@@ -491,19 +491,42 @@ struct ZmClassView {
     __device__ __forceinline__ double SZ(int, int, int dk) const { return dk ? szz : sz0; }
 };
 
-// One workgroup column `tid` of one design.  Every thread of the workgroup
-// must call this (it synchronises the workgroup); `lds` holds 4*3*ZPL_MAX
-// doubles.  STG selects the staging form (see `stage`); the results do not
-// depend on it.
-template <typename TIN, typename TOUT, int MODE, int TX, bool CLS, bool STG = false>
-__device__ __forceinline__ void op_column(const Grid &g, const ZTiling &t, unsigned tid, const CoeffsD &c,
-                                          const OpK &k, const TIN *__restrict__ x, const double *__restrict__ b,
-                                          TOUT *__restrict__ out, double *lds) {
+// Planes [kb, ke) of the workgroup column (tx, ty) of a design's tiling.
+struct ColSpan {
+    unsigned tx, ty;
+    int kb, ke;
+};
+// Workgroup column `tid` of a single-design launch: x fastest, then y, then
+// the z-chunk.
+__device__ __forceinline__ ColSpan zspan(const Grid &g, const ZTiling &t, unsigned tid) {
     const unsigned tx = tid % (unsigned)t.ntx, r1 = tid / (unsigned)t.ntx;
     const unsigned ty = r1 % (unsigned)t.nty, tz = r1 / (unsigned)t.nty;
+    const int kb = (int)tz * t.kz;
+    return ColSpan{tx, ty, kb, min(kb + t.kz, g.nz)};
+}
+
+// One workgroup column span of one design.  Every thread of the workgroup
+// must call this (it synchronises the workgroup); `lds` holds 4*3*ZPL_MAX
+// doubles.  The rows of a plane do not depend on where the span starts (the
+// carried plane-(kb-1) values are recomputed by the same expressions), so any
+// split of a column into spans gives the same bits.  STG selects the staging
+// form (see `stage`); the results do not depend on it.
+template <typename TIN, typename TOUT, int MODE, int TX, bool CLS, bool STG = false>
+__device__ __forceinline__ void op_column(const Grid &gin, const ColSpan cs, const CoeffsD &c, const OpK &k,
+                                          const TIN *__restrict__ x, const double *__restrict__ b,
+                                          TOUT *__restrict__ out, double *lds) {
+    // The node counts, held in scalar registers for the whole z-march: where
+    // the grid comes from memory (the batched kernel) the compiler would
+    // otherwise reload them in every plane, and each reload's wait also
+    // waits for the outstanding LDS reads.
+    Grid g = gin;
+    g.nx = __builtin_amdgcn_readfirstlane(gin.nx);
+    g.ny = __builtin_amdgcn_readfirstlane(gin.ny);
+    g.nz = __builtin_amdgcn_readfirstlane(gin.nz);
+    g.n1 = (int64_t)g.nx * g.ny * g.nz;
     using S = ZShape<TX>;
-    const int i0 = (int)tx * TX, j0 = (int)ty * S::TY;
-    const int kb = (int)tz * t.kz, ke = min(kb + t.kz, g.nz);
+    const int i0 = (int)cs.tx * TX, j0 = (int)cs.ty * S::TY;
+    const int kb = cs.kb, ke = cs.ke;
     const int li = (int)(threadIdx.x % TX), lj = (int)(threadIdx.x / TX);
     const int i = i0 + li, j = j0 + lj;
     const int ic = min(i, g.nx - 1), jc = min(j, g.ny - 1);
@@ -512,9 +535,6 @@ __device__ __forceinline__ void op_column(const Grid &g, const ZTiling &t, unsig
     const int64_t n1 = g.n1;
     const int64_t plane = (int64_t)g.nx * g.ny;
 
-    // Stage plane kp into LDS slot `slot`: all loads of the plane are issued
-    // before the first LDS write, so one memory latency is exposed per plane.
-    static_assert(S::PL > 256 && S::PL <= 512, "two staging rounds per plane");
     // Stage plane kp into LDS slot `slot`.  STG: all eight loads of a thread
     // are issued before its first LDS write (one exposed memory latency per
     // plane, at the cost of the registers that hold them); otherwise the two
@@ -656,46 +676,66 @@ __global__ void __launch_bounds__(256) k_operator(Grid g, ZTiling t, CoeffsD c, 
     // (four waves per SIMD) with it: the class-table apply on 32-wide
     // columns.  Elsewhere it costs a wave per SIMD and runs slower.
     constexpr bool STG = MODE == OP_APPLY && TX == 32 && CLS;
-    op_column<TIN, TOUT, MODE, TX, CLS, STG>(g, t, xcd_tile(blockIdx.x, gridDim.x), c, k, x, b, out, lds);
+    op_column<TIN, TOUT, MODE, TX, CLS, STG>(g, zspan(g, t, xcd_tile(blockIdx.x, gridDim.x)), c, k, x, b, out, lds);
 }
 
 // ---------------------------------------------------------------------------
 // Batched operator: y_d = D_d^-1 M_d x_d for many independent designs in one
-// persistent launch.  The work list is the concatenation of every design's
-// workgroup columns (design d owns columns [col0, col0 + ncol)).  The grid
-// is sized to the machine; the columns are split into 8 contiguous ranges,
-// one per XCD, and the workgroups of an XCD stride through its range, so
-// neighbouring columns of a design run together and share that XCD's L2.
-// Every loop bound is workgroup-uniform and every workgroup leaves after its
-// range, so the grid always drains.
+// persistent launch.  The host splits every design's workgroup columns into
+// z-spans (segments) and deals them to the workgroups so that each runs about
+// the same number of planes, and so that the workgroups of one XCD run
+// neighbouring columns of a design at the same time and share its L2 (the
+// x/y halo of a column is its neighbours' data).  Each workgroup runs its own
+// list of segments; every loop bound is workgroup-uniform, so the grid
+// always drains.
 // ---------------------------------------------------------------------------
 // TX = 0: the column width and coefficient form are read per design (any mix
 // in one launch).  TX = 32 / 64: every design of the launch uses that width
 // and, per CLS, the class table or the streamed arrays; only that variant is
 // compiled in, so the kernel needs the registers and LDS of one variant.
+// Every variant is held to 128 VGPRs (four waves per SIMD); the mixed one
+// would take 133 and run three.
+// The design's pointers, read through the constant address space: a generic
+// pointer loaded from constant memory is taken to address global memory, so
+// every access through it is a global (not flat) instruction.  Flat
+// instructions also count against the LDS wait counter, so every LDS wait of
+// the z-march would wait for the HBM loads as well.
+typedef const __attribute__((address_space(4))) BatchItem ConstItem;
+__device__ __forceinline__ CoeffsD item_coeffs(ConstItem &it) {
+    CoeffsD c;
+    for (int q = 0; q < 3; ++q) { c.im[q] = it.c.im[q]; c.sg[q] = it.c.sg[q]; }
+    c.cls = it.c.cls;
+    c.ctab = it.c.ctab;
+    return c;
+}
+
 template <int TX, bool CLS>
-__global__ void __launch_bounds__(256) k_operator_batch(const BatchItem *__restrict__ items, int nitems,
-                                                        unsigned total) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) k_operator_batch(const BatchItem *__restrict__ items,
+                                                        const BatchSeg *__restrict__ segs,
+                                                        const unsigned *__restrict__ slot_off) {
     __shared__ double lds[4 * 3 * (TX ? ZShape<TX ? TX : 64>::PL : ZPL_MAX)];
-    const unsigned xcd = blockIdx.x & 7u, per = gridDim.x >> 3, l = blockIdx.x >> 3;
-    const unsigned lo = (unsigned)(((uint64_t)total * xcd) >> 3), hi = (unsigned)(((uint64_t)total * (xcd + 1)) >> 3);
-    int d = 0;
-    for (unsigned w = lo + l; w < hi; w += per) {
-        while (d + 1 < nitems && items[d + 1].col0 <= w) ++d;     // w only grows: forward scan
-        const BatchItem &it = items[d];
-        const unsigned col = w - it.col0;
+    const unsigned slot = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+    const unsigned e1 = slot_off[slot + 1];
+    for (unsigned e = slot_off[slot]; e < e1; ++e) {
+        const BatchSeg sg = segs[e];
+        const BatchItem &it = items[sg.d];
+        ConstItem &ci = ((ConstItem *)items)[sg.d];
+        const ColSpan cs{sg.col % (unsigned)it.t.ntx, sg.col / (unsigned)it.t.ntx, sg.kb, sg.ke};
+        const CoeffsD c = item_coeffs(ci);
+        const double *x = ci.x;
+        double *y = ci.y;
         if constexpr (TX != 0) {
-            op_column<double, double, OP_APPLY, TX, CLS>(it.g, it.t, col, it.c, it.k, it.x, nullptr, it.y, lds);
-        } else if (it.c.cls) {
+            op_column<double, double, OP_APPLY, TX, CLS>(it.g, cs, c, it.k, x, nullptr, y, lds);
+        } else if (c.cls) {
             if (it.t.tx == 32)
-                op_column<double, double, OP_APPLY, 32, true>(it.g, it.t, col, it.c, it.k, it.x, nullptr, it.y, lds);
+                op_column<double, double, OP_APPLY, 32, true>(it.g, cs, c, it.k, x, nullptr, y, lds);
             else
-                op_column<double, double, OP_APPLY, 64, true>(it.g, it.t, col, it.c, it.k, it.x, nullptr, it.y, lds);
+                op_column<double, double, OP_APPLY, 64, true>(it.g, cs, c, it.k, x, nullptr, y, lds);
         } else {
             if (it.t.tx == 32)
-                op_column<double, double, OP_APPLY, 32, false>(it.g, it.t, col, it.c, it.k, it.x, nullptr, it.y, lds);
+                op_column<double, double, OP_APPLY, 32, false>(it.g, cs, c, it.k, x, nullptr, y, lds);
             else
-                op_column<double, double, OP_APPLY, 64, false>(it.g, it.t, col, it.c, it.k, it.x, nullptr, it.y, lds);
+                op_column<double, double, OP_APPLY, 64, false>(it.g, cs, c, it.k, x, nullptr, y, lds);
         }
     }
 }
@@ -875,22 +915,35 @@ static void launch_operator(const Grid &g, const CoeffsD &c, const OpParams &op,
     }
 }
 
-// blocks: persistent grid size (rounded up to a multiple of 8 XCDs), or 0 for
-// one workgroup per column.
+// slots: the grid size, a multiple of 8 (one workgroup per slot of the plan).
 // tx, cls: the column width and coefficient form every design of the launch
 // shares, or tx = 0 when they differ between designs.
-void launch_operator_batch(const BatchItem *items_dev, int nitems, unsigned total, int blocks, int tx, int cls,
-                           hipStream_t s) {
-    unsigned g = blocks > 0 ? (unsigned)blocks : total;
-    g = (g + 7u) & ~7u;
-    if (g > ((total + 7u) & ~7u)) g = (total + 7u) & ~7u;
-    if (g < 8u) g = 8u;
-    const dim3 grid(g), block(256);
-    if (tx == 32 && cls) hipLaunchKernelGGL((k_operator_batch<32, true>), grid, block, 0, s, items_dev, nitems, total);
-    else if (tx == 32) hipLaunchKernelGGL((k_operator_batch<32, false>), grid, block, 0, s, items_dev, nitems, total);
-    else if (tx == 64 && cls) hipLaunchKernelGGL((k_operator_batch<64, true>), grid, block, 0, s, items_dev, nitems, total);
-    else if (tx == 64) hipLaunchKernelGGL((k_operator_batch<64, false>), grid, block, 0, s, items_dev, nitems, total);
-    else hipLaunchKernelGGL((k_operator_batch<0, false>), grid, block, 0, s, items_dev, nitems, total);
+void launch_operator_batch(const BatchItem *items_dev, const BatchSeg *segs_dev, const unsigned *slot_off_dev,
+                           unsigned slots, int tx, int cls, hipStream_t s) {
+    const dim3 grid(slots), block(256);
+    if (tx == 32 && cls) hipLaunchKernelGGL((k_operator_batch<32, true>), grid, block, 0, s, items_dev, segs_dev, slot_off_dev);
+    else if (tx == 32) hipLaunchKernelGGL((k_operator_batch<32, false>), grid, block, 0, s, items_dev, segs_dev, slot_off_dev);
+    else if (tx == 64 && cls) hipLaunchKernelGGL((k_operator_batch<64, true>), grid, block, 0, s, items_dev, segs_dev, slot_off_dev);
+    else if (tx == 64) hipLaunchKernelGGL((k_operator_batch<64, false>), grid, block, 0, s, items_dev, segs_dev, slot_off_dev);
+    else hipLaunchKernelGGL((k_operator_batch<0, false>), grid, block, 0, s, items_dev, segs_dev, slot_off_dev);
+}
+
+// Workgroups of the batched kernel variant (tx, cls as for the launch) that
+// the device holds at once: its occupancy per CU times the CU count, rounded
+// down to a multiple of 8.
+int batch_resident_blocks(int tx, int cls) {
+    int dev = 0, ncu = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 1024;
+    const void *f = tx == 32 && cls ? (const void *)&k_operator_batch<32, true>
+                  : tx == 32        ? (const void *)&k_operator_batch<32, false>
+                  : tx == 64 && cls ? (const void *)&k_operator_batch<64, true>
+                  : tx == 64        ? (const void *)&k_operator_batch<64, false>
+                                    : (const void *)&k_operator_batch<0, false>;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, f, 256, 0) != hipSuccess || per < 1) return 1024;
+    const int r = per * ncu / 8 * 8;
+    return r >= 8 ? r : 8;
 }
 
 void launch_coeffs(const Grid &g, const uint16_t *slot, const double *mat, const Coeffs &c, hipStream_t s) {

@@ -20,6 +20,7 @@
 #include <array>
 #include <unordered_map>
 #include "qms_kernels.h"
+#include "qms_plan.h"
 #include "../../include/qms.h"
 
 using namespace qms;
@@ -780,13 +781,20 @@ struct qms_batch {
     hipStream_t s = nullptr;
     BatchItem *items_dev = nullptr;
     std::vector<BatchItem> items;
-    unsigned total = 0;
-    int blocks = 0;            // persistent grid size, 0 = one workgroup per column
+    int blocks = 0;            // persistent grid size, 0 = one workgroup per natural piece
     int tx = 0, cls = 0;       // column width / class-table use shared by all items (tx 0: mixed)
+    // the schedule on the device, and the inputs it was planned for
+    BatchSeg *segs_dev = nullptr;
+    unsigned *off_dev = nullptr;
+    size_t segs_cap = 0, off_cap = 0;
+    unsigned slots = 0;
+    std::vector<int> plan_key;
     std::string err;
     ~qms_batch() {
         if (s) hipStreamSynchronize(s);
         if (items_dev) hipFree(items_dev);
+        if (segs_dev) hipFree(segs_dev);
+        if (off_dev) hipFree(off_dev);
         if (s) hipStreamDestroy(s);
     }
 };
@@ -799,10 +807,48 @@ int bfail(qms_batch *b, const std::exception &e, int code) {
     return code;
 }
 
+static_assert(sizeof(PlanSeg) == sizeof(BatchSeg), "host and device segment layouts differ");
+
+// Plans the launch for the designs' current tilings and grid size and
+// uploads the schedule, unless it is the one already on the device.
+void batch_plan(qms_batch *b) {
+    std::vector<int> key{b->blocks};
+    std::vector<PlanDesign> pd;
+    for (const BatchItem &it : b->items) {
+        const unsigned ncol = (unsigned)it.t.ntx * (unsigned)it.t.nty;
+        pd.push_back(PlanDesign{ncol, it.g.nz, it.t.kz});
+        key.insert(key.end(), {(int)ncol, it.g.nz, it.t.kz});
+    }
+    if (key == b->plan_key) return;
+    const BatchPlan p = plan_batch(pd, (unsigned)b->blocks);
+    if ((uint64_t)p.segs.size() >= 0xffffffffull) throw std::runtime_error("batch too large");
+    if (p.segs.size() > b->segs_cap) {
+        if (b->segs_dev) { check(hipStreamSynchronize(b->s), "sync"); check(hipFree(b->segs_dev), "hipFree"); }
+        b->segs_dev = nullptr;
+        b->segs_cap = 0;
+        check(hipMalloc(&b->segs_dev, sizeof(BatchSeg) * p.segs.size()), "hipMalloc");
+        b->segs_cap = p.segs.size();
+    }
+    if (p.off.size() > b->off_cap) {
+        if (b->off_dev) { check(hipStreamSynchronize(b->s), "sync"); check(hipFree(b->off_dev), "hipFree"); }
+        b->off_dev = nullptr;
+        b->off_cap = 0;
+        check(hipMalloc(&b->off_dev, sizeof(unsigned) * p.off.size()), "hipMalloc");
+        b->off_cap = p.off.size();
+    }
+    if (!p.segs.empty())
+        check(hipMemcpyAsync(b->segs_dev, p.segs.data(), sizeof(BatchSeg) * p.segs.size(), hipMemcpyHostToDevice,
+                             b->s), "h2d");
+    check(hipMemcpyAsync(b->off_dev, p.off.data(), sizeof(unsigned) * p.off.size(), hipMemcpyHostToDevice, b->s),
+          "h2d");
+    check(hipStreamSynchronize(b->s), "sync");
+    b->slots = p.slots;
+    b->plan_key = key;
+}
+
 // Work items for rows at per-design velocities v[3d..3d+2]; transient rows
 // use each design's own dt.  Waits for every handle's stream first.
 void batch_items(qms_batch *b, const double *v, int transient) {
-    b->total = 0;
     b->tx = -1;
     for (size_t d = 0; d < b->hs.size(); ++d) {
         qms_handle *h = b->hs[d];
@@ -817,22 +863,19 @@ void batch_items(qms_batch *b, const double *v, int transient) {
         it.k = make_opk(h->g, op);
         it.x = h->x;
         it.y = h->w;
-        it.col0 = b->total;
-        it.ncol = ncol;
         const int icls = it.c.cls ? 1 : 0;
         if (b->tx == -1) { b->tx = it.t.tx; b->cls = icls; }
         else if (b->tx != it.t.tx || b->cls != icls) b->tx = 0;
-        if ((uint64_t)b->total + ncol > 0xffffffffull) throw std::runtime_error("batch too large");
-        b->total += ncol;
     }
     check(hipMemcpyAsync(b->items_dev, b->items.data(), sizeof(BatchItem) * b->items.size(), hipMemcpyHostToDevice,
                          b->s), "h2d");
     check(hipStreamSynchronize(b->s), "sync");
+    batch_plan(b);
 }
 
 void batch_launch(qms_batch *b, int mode, const double *v, int transient) {
     if (mode == 0) {
-        launch_operator_batch(b->items_dev, (int)b->items.size(), b->total, b->blocks, b->tx, b->cls, b->s);
+        launch_operator_batch(b->items_dev, b->segs_dev, b->off_dev, b->slots, b->tx, b->cls, b->s);
     } else {
         for (size_t d = 0; d < b->hs.size(); ++d) {
             qms_handle *h = b->hs[d];
@@ -894,7 +937,7 @@ int qms_batch_tune(qms_batch *b, const double *v, int transient, int reps, int *
                    int *blocks) {
     if (!b || !v || reps < 1) return QMS_ERR_ARG;
     try {
-        static const int KZ[] = {2, 3, 4, 6, 8};
+        static const int KZ[] = {1, 2, 3, 4, 6, 8};
         static const int CW[] = {64, 32};
         for (size_t d = 0; d < b->hs.size(); ++d) {
             qms_handle *h = b->hs[d];
@@ -917,36 +960,39 @@ int qms_batch_tune(qms_batch *b, const double *v, int transient, int reps, int *
         }
         // The per-design choice favours many small columns (a lone small
         // design must fill the GPU); in a batch the designs fill it together,
-        // so also try one shape for all designs and keep what runs fastest.
+        // so also try one shape for all designs, each with one workgroup per
+        // natural piece and with the balanced persistent schedule at one and
+        // two times the resident workgroup count, and keep the fastest.
         std::vector<int> bkz(b->hs.size()), bcw(b->hs.size());
         for (size_t d = 0; d < b->hs.size(); ++d) { bkz[d] = b->hs[d]->zchunk; bcw[d] = b->hs[d]->colw; }
-        batch_items(b, v, transient);
-        double best_shape = time_on(b->s, reps, [&]() { batch_launch(b, 0, v, transient); });
-        int ucw = 0, ukz = 0;
-        for (int cw : CW) {
-            for (int kz : {2, 4, 8, 16}) {
-                for (qms_handle *h : b->hs) { h->zchunk = kz; h->colw = cw; }
-                batch_items(b, v, transient);
-                const double ms = time_on(b->s, reps, [&]() { batch_launch(b, 0, v, transient); });
-                if (ms < best_shape) { best_shape = ms; ucw = cw; ukz = kz; }
+        double best = 1e30;
+        int ucw = 0, ukz = 0, bestb = 0;
+        auto trial = [&](int cw, int kz) {
+            for (size_t d = 0; d < b->hs.size(); ++d) {
+                b->hs[d]->zchunk = kz ? kz : bkz[d];
+                b->hs[d]->colw = kz ? cw : bcw[d];
             }
-        }
+            b->blocks = 0;
+            batch_items(b, v, transient);
+            const int res = batch_resident_blocks(b->tx, b->cls);
+            for (int bl : {0, res, 2 * res}) {
+                b->blocks = bl;
+                batch_plan(b);
+                const double ms = time_on(b->s, reps, [&]() { batch_launch(b, 0, v, transient); });
+                if (ms < best) { best = ms; ucw = cw; ukz = kz; bestb = bl; }
+            }
+        };
+        trial(0, 0);
+        for (int cw : CW)
+            for (int kz : {2, 4, 8, 16, 32, 64}) trial(cw, kz);
         for (size_t d = 0; d < b->hs.size(); ++d) {
             b->hs[d]->zchunk = ukz ? ukz : bkz[d];
-            b->hs[d]->colw = ucw ? ucw : bcw[d];
+            b->hs[d]->colw = ukz ? ucw : bcw[d];
             if (zchunks) zchunks[d] = b->hs[d]->zchunk;
             if (colwidths) colwidths[d] = b->hs[d]->colw;
         }
-        batch_items(b, v, transient);
-        static const int BL[] = {0, 1024, 2048, 4096, 8192};
-        double best = 1e30;
-        int bestb = 0;
-        for (int bl : BL) {
-            b->blocks = bl;
-            const double ms = time_on(b->s, reps, [&]() { batch_launch(b, 0, v, transient); });
-            if (ms < best) { best = ms; bestb = bl; }
-        }
         b->blocks = bestb;
+        batch_items(b, v, transient);
         if (blocks) *blocks = bestb;
     } catch (const HipError &e) {
         return bfail(b, e, e.code);
@@ -969,6 +1015,44 @@ int qms_batch_time(qms_batch *b, int mode, const double *v, int transient, int r
         return bfail(b, e, e.code);
     } catch (const std::exception &e) {
         return bfail(b, e, QMS_ERR_ARG);
+    }
+    return QMS_OK;
+}
+
+int qms_batch_plan(int n, const int *ncols, const int *nz, const int *band, int blocks, unsigned *segs,
+                   int seg_cap, unsigned *slot_off, int off_cap, int *nsegs, int *nslots) {
+    if (n < 1 || !ncols || !nz || !band || blocks < 0 || !nsegs || !nslots) {
+        g_last_error = "qms_batch_plan: invalid argument";
+        return QMS_ERR_ARG;
+    }
+    try {
+        std::vector<PlanDesign> pd;
+        for (int d = 0; d < n; ++d) {
+            if (ncols[d] < 1 || nz[d] < 1 || band[d] < 1) {
+                g_last_error = "qms_batch_plan: columns, planes and z-chunk must be positive";
+                return QMS_ERR_ARG;
+            }
+            pd.push_back(PlanDesign{(unsigned)ncols[d], nz[d], band[d]});
+        }
+        const BatchPlan p = plan_batch(pd, (unsigned)blocks);
+        if (p.segs.size() > 0x7fffffffu) { g_last_error = "qms_batch_plan: too many segments"; return QMS_ERR_ARG; }
+        *nsegs = (int)p.segs.size();
+        *nslots = (int)p.slots;
+        if (!segs || !slot_off) return QMS_OK;
+        if ((size_t)seg_cap < p.segs.size() || (size_t)off_cap < p.off.size()) {
+            g_last_error = "qms_batch_plan: output buffers too small";
+            return QMS_ERR_ARG;
+        }
+        for (size_t e = 0; e < p.segs.size(); ++e) {
+            segs[4 * e] = p.segs[e].d;
+            segs[4 * e + 1] = p.segs[e].col;
+            segs[4 * e + 2] = (unsigned)p.segs[e].kb;
+            segs[4 * e + 3] = (unsigned)p.segs[e].ke;
+        }
+        for (size_t s = 0; s < p.off.size(); ++s) slot_off[s] = p.off[s];
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return QMS_ERR_ARG;
     }
     return QMS_OK;
 }

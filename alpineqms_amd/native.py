@@ -29,7 +29,7 @@ EXPORTS = (
     "qms_get_fields", "qms_set_fields", "qms_get_h", "qms_apply", "qms_residual", "qms_rhs",
     "qms_get_stats", "qms_set_zchunk", "qms_set_colwidth", "qms_use_coef_classes", "qms_time_kernel", "qms_gram_schmidt", "qms_get_work", "qms_destroy", "qms_last_error",
     "qms_batch_create", "qms_batch_set_blocks", "qms_batch_apply", "qms_batch_tune", "qms_batch_time",
-    "qms_batch_last_error", "qms_batch_destroy",
+    "qms_batch_plan", "qms_batch_last_error", "qms_batch_destroy",
 )
 
 # qms_time_kernel selectors
@@ -116,6 +116,9 @@ def load(path: Optional[str] = None):
     L.qms_set_colwidth.argtypes = [H, ctypes.c_int]
     L.qms_use_coef_classes.argtypes = [H, ctypes.c_int]
     L.qms_batch_time.argtypes = [H, ctypes.c_int, d, ctypes.c_int, ctypes.c_int, d, d]
+    ip = ctypes.POINTER(ctypes.c_int)
+    up = ctypes.POINTER(ctypes.c_uint)
+    L.qms_batch_plan.argtypes = [ctypes.c_int, ip, ip, ip, ctypes.c_int, up, ctypes.c_int, up, ctypes.c_int, ip, ip]
     L.qms_batch_last_error.argtypes = [H]
     L.qms_batch_last_error.restype = ctypes.c_char_p
     L.qms_batch_destroy.argtypes = [H]
@@ -313,6 +316,29 @@ class Solver:
         out = np.empty((4, nz, ny, nx), dtype=np.float64)
         self._check(self._L.qms_get_work(self._h, _dp(out)))
         return out
+
+
+def batch_plan(ncols: Sequence[int], nz: Sequence[int], band: Sequence[int], blocks: int):
+    """Schedule of a batched launch, computed on the host (no GPU call).
+
+    Returns (segments, slot_off): segments is an (S, 4) array of (design,
+    column, first plane, end plane); the workgroup of slot s runs rows
+    slot_off[s]:slot_off[s+1] in order."""
+    L = load()
+    n = len(ncols)
+    a = [(ctypes.c_int * n)(*map(int, x)) for x in (ncols, nz, band)]
+    ns, nsl = ctypes.c_int(), ctypes.c_int()
+    rc = L.qms_batch_plan(n, a[0], a[1], a[2], int(blocks), None, 0, None, 0, ctypes.byref(ns), ctypes.byref(nsl))
+    if rc != QMS_OK:
+        raise QmsError(rc, L.qms_last_error(None).decode())
+    segs = np.zeros((ns.value, 4), dtype=np.uint32)
+    off = np.zeros(nsl.value + 1, dtype=np.uint32)
+    up = ctypes.POINTER(ctypes.c_uint)
+    rc = L.qms_batch_plan(n, a[0], a[1], a[2], int(blocks), segs.ctypes.data_as(up), ns.value,
+                          off.ctypes.data_as(up), nsl.value + 1, ctypes.byref(ns), ctypes.byref(nsl))
+    if rc != QMS_OK:
+        raise QmsError(rc, L.qms_last_error(None).decode())
+    return segs, off
 
 
 class Batch:

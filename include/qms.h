@@ -178,7 +178,10 @@ int qms_get_work(qms_handle *h, double *out);
  * and owns one HIP stream. */
 typedef struct qms_batch qms_batch;
 int qms_batch_create(qms_handle *const *handles, int n, qms_batch **out);
-/* Persistent grid size of the batched launch (0: one workgroup per column). */
+/* Grid size of the batched launch.  0: one workgroup per natural piece (a
+ * design's column over one z-chunk).  blocks > 0: that many persistent
+ * workgroups (rounded up to a multiple of 8), each running a list of column
+ * z-spans of about equal cost (qms_batch_plan). */
 int qms_batch_set_blocks(qms_batch *b, int blocks);
 /* w_d = D_d^-1 M_d(v_d) x_d for every design d (x_d: the handle's state,
  * v: 3n velocities; transient != 0: transient rows with each design's dt).
@@ -186,17 +189,31 @@ int qms_batch_set_blocks(qms_batch *b, int blocks);
  * for bit to a single-design qms_apply of the same rows. */
 int qms_batch_apply(qms_batch *b, const double *v, int transient);
 /* Autotuner: times each design's operator at column widths {64, 32} x
- * z-chunks {2,3,4,6,8} and keeps the fastest (qms_set_colwidth,
+ * z-chunks {1,2,3,4,6,8} and keeps the fastest (qms_set_colwidth,
  * qms_set_zchunk); then times the batched launch with those per-design
- * shapes and with one shape for all designs ({64, 32} x {2, 4, 8, 16}) and keeps
- * the fastest; then times grid sizes {per column, 1024, 2048, 4096, 8192}.  zchunks[n], colwidths[n]
- * and blocks (any may be NULL) receive the choices. */
+ * shapes and with one shape for all designs ({64, 32} x z-chunks
+ * {2, 4, 8, 16, 32, 64}), each at grid sizes {one workgroup per natural piece,
+ * R, 2R} where R is the number of workgroups the device holds at once, and
+ * keeps the fastest.  zchunks[n], colwidths[n] and blocks (any may be NULL)
+ * receive the choices. */
 int qms_batch_tune(qms_batch *b, const double *v, int transient, int reps, int *zchunks, int *colwidths,
                    int *blocks);
 /* Mean time of one batched launch (mode 0) or of the n single-design
  * launches it replaces (mode 1), and the algorithmic bytes they move. */
 int qms_batch_time(qms_batch *b, int mode, const double *v, int transient, int reps, double *ms_per_launch,
                    double *bytes_per_launch);
+/* The schedule of a batched launch, computed on the host with no GPU call
+ * (for inspection and tests).  Design d has ncols[d] workgroup columns per
+ * plane, nz[d] planes and natural z-chunk band[d]; blocks as for
+ * qms_batch_set_blocks.  Writes *nslots (the grid size) and *nsegs.  When
+ * segs and slot_off are non-null and seg_cap >= *nsegs, off_cap >= *nslots + 1,
+ * also writes the segments (4 values each: design, column, first plane, end
+ * plane) and the slot offsets: the workgroup of slot s runs segments
+ * slot_off[s] .. slot_off[s+1]-1 in order, and workgroup b has slot
+ * (b % 8) * (*nslots / 8) + b / 8.  Returns QMS_ERR_ARG on invalid input or
+ * short buffers. */
+int qms_batch_plan(int n, const int *ncols, const int *nz, const int *band, int blocks, unsigned *segs,
+                   int seg_cap, unsigned *slot_off, int off_cap, int *nsegs, int *nslots);
 const char *qms_batch_last_error(const qms_batch *b);
 void qms_batch_destroy(qms_batch *b);
 
