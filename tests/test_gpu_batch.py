@@ -134,3 +134,38 @@ def test_batch_member_matches_oracle(batch_setup):
     got = O.pack(p, w)
     scale = (abs(M) @ np.abs(xp)) / np.abs(Dg)
     assert np.max(np.abs(got - ref) / scale) < 1e-13
+
+
+def test_batch_of_smallest_and_thin_designs():
+    """Designs at the smallest size the reference accepts (5 cells per axis)
+    and slabs one column wide or a few planes deep, batched with one larger
+    design: z-chunks deeper than a design, segments of one or two planes,
+    and most of the persistent workgroups idle.  Every design's rows are
+    the same bits as its own launch, for every grid size."""
+    shapes = [(5, 5, 5), (5, 31, 6), (40, 5, 5), (6, 7, 70), (45, 29, 37)]
+    ds = [configs.small_box_design(0, 7 + k, nbox=1, v=1.0, ramp=False, gradient=True, cells=c)
+          for k, c in enumerate(shapes)]
+    ss = [_solver(d) for d in ds]
+    rng = np.random.default_rng(43)
+    try:
+        xs = []
+        for s in ss:
+            nx, ny, nz = s.nodes
+            x = rng.normal(size=(4, nz, ny, nx))
+            s.set_fields(x)
+            xs.append(x)
+        v = rng.normal(size=(len(ss), 3))
+        for kz in (1, 3, 64):
+            for s in ss:
+                s.set_colwidth(32)
+                s.set_zchunk(kz)
+            ref = _single(ss, xs, v, True)
+            with native.Batch(ss) as b:
+                for blocks in (0, 8, 40, 1024):
+                    b.set_blocks(blocks)
+                    b.apply(v)
+                    for k, s in enumerate(ss):
+                        assert np.array_equal(s.get_work(), ref[k]), (kz, blocks, shapes[k])
+    finally:
+        for s in ss:
+            s.close()
