@@ -79,6 +79,8 @@ struct BatchSeg {
 // slots workgroups (a multiple of 8); the workgroup of slot s runs segments
 // [slot_off[s], slot_off[s+1]) in order, where workgroup b has slot
 // (b % 8) * (slots / 8) + b / 8, so each XCD runs one contiguous slot range.
+// slot_off_dev null: slots = the segment count, one workgroup per segment,
+// each XCD running one contiguous range of segments.
 // tx, cls: column width and coefficient form shared by every design, or tx = 0.
 void launch_operator_batch(const BatchItem *items_dev, const BatchSeg *segs_dev, const unsigned *slot_off_dev,
                            unsigned slots, int tx, int cls, hipStream_t s);

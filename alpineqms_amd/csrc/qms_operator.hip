@@ -709,6 +709,31 @@ __device__ __forceinline__ CoeffsD item_coeffs(ConstItem &it) {
     return c;
 }
 
+// One segment of a batched launch.  STG as for op_column.
+template <int TX, bool CLS, bool STG>
+__device__ __forceinline__ void batch_segment(const BatchItem *__restrict__ items, const BatchSeg &sg, double *lds) {
+    const BatchItem &it = items[sg.d];
+    ConstItem &ci = ((ConstItem *)items)[sg.d];
+    const ColSpan cs{sg.col % (unsigned)it.t.ntx, sg.col / (unsigned)it.t.ntx, sg.kb, sg.ke};
+    const CoeffsD c = item_coeffs(ci);
+    const double *x = ci.x;
+    double *y = ci.y;
+    if constexpr (TX != 0) {
+        op_column<double, double, OP_APPLY, TX, CLS, STG>(it.g, cs, c, it.k, x, nullptr, y, lds);
+    } else if (c.cls) {
+        if (it.t.tx == 32)
+            op_column<double, double, OP_APPLY, 32, true>(it.g, cs, c, it.k, x, nullptr, y, lds);
+        else
+            op_column<double, double, OP_APPLY, 64, true>(it.g, cs, c, it.k, x, nullptr, y, lds);
+    } else {
+        if (it.t.tx == 32)
+            op_column<double, double, OP_APPLY, 32, false>(it.g, cs, c, it.k, x, nullptr, y, lds);
+        else
+            op_column<double, double, OP_APPLY, 64, false>(it.g, cs, c, it.k, x, nullptr, y, lds);
+    }
+}
+
+// Persistent form: slots workgroups, each running its list of segments.
 template <int TX, bool CLS>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) k_operator_batch(const BatchItem *__restrict__ items,
                                                         const BatchSeg *__restrict__ segs,
@@ -716,28 +741,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))
     __shared__ double lds[4 * 3 * (TX ? ZShape<TX ? TX : 64>::PL : ZPL_MAX)];
     const unsigned slot = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
     const unsigned e1 = slot_off[slot + 1];
-    for (unsigned e = slot_off[slot]; e < e1; ++e) {
-        const BatchSeg sg = segs[e];
-        const BatchItem &it = items[sg.d];
-        ConstItem &ci = ((ConstItem *)items)[sg.d];
-        const ColSpan cs{sg.col % (unsigned)it.t.ntx, sg.col / (unsigned)it.t.ntx, sg.kb, sg.ke};
-        const CoeffsD c = item_coeffs(ci);
-        const double *x = ci.x;
-        double *y = ci.y;
-        if constexpr (TX != 0) {
-            op_column<double, double, OP_APPLY, TX, CLS>(it.g, cs, c, it.k, x, nullptr, y, lds);
-        } else if (c.cls) {
-            if (it.t.tx == 32)
-                op_column<double, double, OP_APPLY, 32, true>(it.g, cs, c, it.k, x, nullptr, y, lds);
-            else
-                op_column<double, double, OP_APPLY, 64, true>(it.g, cs, c, it.k, x, nullptr, y, lds);
-        } else {
-            if (it.t.tx == 32)
-                op_column<double, double, OP_APPLY, 32, false>(it.g, cs, c, it.k, x, nullptr, y, lds);
-            else
-                op_column<double, double, OP_APPLY, 64, false>(it.g, cs, c, it.k, x, nullptr, y, lds);
-        }
-    }
+    for (unsigned e = slot_off[slot]; e < e1; ++e) batch_segment<TX, CLS, false>(items, segs[e], lds);
+}
+
+// One segment per workgroup (the hardware dispatcher balances the load):
+// with no loop around the z-march, the 32-wide class-table variant fits the
+// unrolled staging in 128 VGPRs, as k_operator does.  Segment order is
+// XCD-aware as in k_operator.
+template <int TX, bool CLS>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) k_operator_batch1(const BatchItem *__restrict__ items,
+                                                         const BatchSeg *__restrict__ segs) {
+    __shared__ double lds[4 * 3 * (TX ? ZShape<TX ? TX : 64>::PL : ZPL_MAX)];
+    batch_segment<TX, CLS, TX == 32 && CLS>(items, segs[xcd_tile(blockIdx.x, gridDim.x)], lds);
 }
 
 // Row diagonal D for every entry (1 on fixed rows, 0 where no unknown exists).
@@ -915,12 +930,21 @@ static void launch_operator(const Grid &g, const CoeffsD &c, const OpParams &op,
     }
 }
 
-// slots: the grid size, a multiple of 8 (one workgroup per slot of the plan).
+// slots: the grid size.  With slot offsets, a multiple of 8: one persistent
+// workgroup per slot of the plan.  Without (null), one workgroup per segment.
 // tx, cls: the column width and coefficient form every design of the launch
 // shares, or tx = 0 when they differ between designs.
 void launch_operator_batch(const BatchItem *items_dev, const BatchSeg *segs_dev, const unsigned *slot_off_dev,
                            unsigned slots, int tx, int cls, hipStream_t s) {
     const dim3 grid(slots), block(256);
+    if (!slot_off_dev) {
+        if (tx == 32 && cls) hipLaunchKernelGGL((k_operator_batch1<32, true>), grid, block, 0, s, items_dev, segs_dev);
+        else if (tx == 32) hipLaunchKernelGGL((k_operator_batch1<32, false>), grid, block, 0, s, items_dev, segs_dev);
+        else if (tx == 64 && cls) hipLaunchKernelGGL((k_operator_batch1<64, true>), grid, block, 0, s, items_dev, segs_dev);
+        else if (tx == 64) hipLaunchKernelGGL((k_operator_batch1<64, false>), grid, block, 0, s, items_dev, segs_dev);
+        else hipLaunchKernelGGL((k_operator_batch1<0, false>), grid, block, 0, s, items_dev, segs_dev);
+        return;
+    }
     if (tx == 32 && cls) hipLaunchKernelGGL((k_operator_batch<32, true>), grid, block, 0, s, items_dev, segs_dev, slot_off_dev);
     else if (tx == 32) hipLaunchKernelGGL((k_operator_batch<32, false>), grid, block, 0, s, items_dev, segs_dev, slot_off_dev);
     else if (tx == 64 && cls) hipLaunchKernelGGL((k_operator_batch<64, true>), grid, block, 0, s, items_dev, segs_dev, slot_off_dev);

@@ -787,7 +787,7 @@ struct qms_batch {
     BatchSeg *segs_dev = nullptr;
     unsigned *off_dev = nullptr;
     size_t segs_cap = 0, off_cap = 0;
-    unsigned slots = 0;
+    unsigned slots = 0, nsegs = 0;
     std::vector<int> plan_key;
     std::string err;
     ~qms_batch() {
@@ -843,6 +843,7 @@ void batch_plan(qms_batch *b) {
           "h2d");
     check(hipStreamSynchronize(b->s), "sync");
     b->slots = p.slots;
+    b->nsegs = (unsigned)p.segs.size();
     b->plan_key = key;
 }
 
@@ -875,7 +876,10 @@ void batch_items(qms_batch *b, const double *v, int transient) {
 
 void batch_launch(qms_batch *b, int mode, const double *v, int transient) {
     if (mode == 0) {
-        launch_operator_batch(b->items_dev, b->segs_dev, b->off_dev, b->slots, b->tx, b->cls, b->s);
+        if (b->blocks == 0)   // one workgroup per segment
+            launch_operator_batch(b->items_dev, b->segs_dev, nullptr, b->nsegs, b->tx, b->cls, b->s);
+        else
+            launch_operator_batch(b->items_dev, b->segs_dev, b->off_dev, b->slots, b->tx, b->cls, b->s);
     } else {
         for (size_t d = 0; d < b->hs.size(); ++d) {
             qms_handle *h = b->hs[d];
