@@ -792,7 +792,9 @@ __global__ void __launch_bounds__(256) k_rhs(Grid g, Tiling t, CoeffsD c, BC bc,
                 r = bc_value(g, bc, k.v, f, i, j, kk);
             } else if (k.tr) {
                 // sigma streamed from the arrays: here the class-table gathers
-                // measured slower than the coalesced array reads
+                // (one node per thread, so each gather's latency is exposed)
+                // measured slower than the coalesced array reads, 31 against
+                // 27 us at 128^3
                 if (f < 3) {
                     r = (c.sg[f][p] * k.idt) * aprev[f * n1 + p];
                 } else {
