@@ -34,6 +34,8 @@ EXPORTS = (
 
 # qms_time_kernel selectors
 K_APPLY, K_SMOOTH_A, K_SMOOTH_V, K_COPY, K_RHS, K_HFIELD, K_RESID, K_GRAM_SCHMIDT = range(8)
+# or-ed into a selector: the kernel timed with cold caches
+K_COLD = 256
 
 
 class QmsError(RuntimeError):
@@ -288,7 +290,8 @@ class Solver:
 
     def time_kernel(self, which: int, reps: int, v=None):
         """Mean milliseconds and algorithmic bytes per launch of one kernel
-        (selectors K_*), timed with HIP events on the handle's stream."""
+        (selectors K_*; or-ed with K_COLD, with cold caches), timed with HIP
+        events on the handle's stream."""
         ms = ctypes.c_double()
         by = ctypes.c_double()
         vp = None

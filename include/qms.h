@@ -151,9 +151,17 @@ int qms_set_colwidth(qms_handle *h, int nodes);
  * 4 step right-hand side, 5 cell-centred H, 6 FP64 residual b - M x,
  * 7 Gram-Schmidt step of FGMRES against restart + 1 basis vectors (the
  * largest of a restart cycle: multi-dot, reduction, update, norm).
+ * which | QMS_TIME_COLD: the same kernel with cold caches: each launch
+ * follows a read of a 640 MB buffer that evicts its arrays from the L2s and
+ * the 256 MB Infinity Cache, as the Krylov basis does between two launches
+ * inside a solve; the kernel's time is that of the (read, kernel) pairs less
+ * that of the reads alone (allocates that buffer for the call).  Without it,
+ * launches run back to back and a working set below 256 MB can stay in the
+ * Infinity Cache between them.
  * Operator bytes are the workload's: x, y (and b) plus the six FP64
  * coefficient arrays, also when the kernel reads the class table instead.
  * No reference counterpart (measurement only). */
+#define QMS_TIME_COLD 256
 int qms_time_kernel(qms_handle *h, int which, const double v[3], int reps, double *ms_per_launch,
                     double *bytes_per_launch);
 
