@@ -155,7 +155,8 @@ int qms_set_colwidth(qms_handle *h, int nodes);
  * follows a read of a 640 MB buffer that evicts its arrays from the L2s and
  * the 256 MB Infinity Cache, as the Krylov basis does between two launches
  * inside a solve; the kernel's time is that of the (read, kernel) pairs less
- * that of the reads alone (allocates that buffer for the call).  Without it,
+ * that of the reads alone (allocates that buffer for the call; for kernels
+ * far shorter than the read, about 0.1 ms, the difference is noisy).  Without it,
  * launches run back to back and a working set below 256 MB can stay in the
  * Infinity Cache between them.
  * Operator bytes are the workload's: x, y (and b) plus the six FP64
