@@ -18,6 +18,7 @@
 #include <memory>
 #include <functional>
 #include <array>
+#include <algorithm>
 #include <unordered_map>
 #include "qms_kernels.h"
 #include "qms_plan.h"
@@ -696,17 +697,24 @@ int qms_time_kernel(qms_handle *h, int which, const double v[3], int reps, doubl
         if (!solve_enabled(h)) return QMS_ERR_STATE;
         if (mixed) ensure_mg<float>(h, tr); else ensure_mg<double>(h, tr);
     }
+    // The arrays a launch reads (x, b) and writes (w): the handle's own, or
+    // one of the copies the cold timing rotates through.
+    struct Arrays {
+        const double *x, *b;
+        double *w;
+    };
+    Arrays cur{h->x, h->b, which == 5 ? h->tmp : h->w};
     auto launch = [&]() {
         if (which == 0) {
-            launch_apply(h->g, op_coeffs(h), op, h->x, h->w, h->s);
+            launch_apply(h->g, op_coeffs(h), op, cur.x, cur.w, h->s);
         } else if (which == 3) {
-            launch_copy(h->w, h->x, n, h->s);
+            launch_copy(cur.w, cur.x, n, h->s);
         } else if (which == 4) {
-            launch_rhs(h->g, op_coeffs(h), h->bc, op, h->x, h->w, h->s);
+            launch_rhs(h->g, op_coeffs(h), h->bc, op, cur.x, cur.w, h->s);
         } else if (which == 5) {
-            launch_hfield(h->g, op_coeffs(h), h->x, h->tmp, 0, h->s);
+            launch_hfield(h->g, op_coeffs(h), cur.x, cur.w, 0, h->s);
         } else if (which == 6) {
-            launch_resid(h->g, op_coeffs(h), op, h->x, h->b, h->w, h->s);
+            launch_resid(h->g, op_coeffs(h), op, cur.x, cur.b, cur.w, h->s);
         } else if (which == 7) {
             gram_schmidt(h, h->cfg.restart + 1);
         } else {
@@ -721,27 +729,41 @@ int qms_time_kernel(qms_handle *h, int which, const double v[3], int reps, doubl
             }
         }
     };
-    if (cold) {
+    if (cold && which != 7) {
         // Cold caches, as inside a solve, where the Krylov basis streams
-        // through the caches between two launches of a kernel: each launch
-        // follows a read-only pass over a buffer larger than the L2s and the
-        // 256 MB Infinity Cache together, which evicts the kernel's arrays.
-        // The kernel's time is that of the pairs less that of the passes
-        // alone, both back to back, so no event sits between the pass and
-        // the kernel.
-        struct Flush {
-            double *p = nullptr;
-            ~Flush() {
-                if (p) hipFree(p);
+        // through the caches between two launches of a kernel: the launches
+        // rotate through sets of copies of the kernel's arrays, enough sets
+        // that the others' traffic since a set's last use (at least 640 MB)
+        // exceeds the L2s and the 256 MB Infinity Cache together.  Launches
+        // run back to back and are timed as a whole, like the warm figure.
+        // (The Gram-Schmidt step streams 4 GB and is cold as it is.)
+        const int64_t per = 16 * n;                              // x read, w written, bytes
+        const int nsets = (int)std::min<int64_t>(64, 1 + ((int64_t)640 << 20) / per + 1);
+        struct Sets {
+            std::vector<double *> p;
+            ~Sets() {
+                for (double *q : p) hipFree(q);
             }
-        } f;
-        const int64_t nflush = (int64_t)80 << 20;                 // 640 MB of doubles
-        check(hipMalloc(&f.p, sizeof(double) * nflush), "hipMalloc");
-        check(hipMemsetAsync(f.p, 0, sizeof(double) * nflush, h->s), "hipMemsetAsync");
-        auto flush = [&]() { launch_norm2(f.p, nflush, h->partial, h->nblk, h->s); };
-        const double both = time_on(h->s, reps, [&]() { flush(); launch(); });
-        const double pass = time_on(h->s, reps, flush);
-        *ms_per_launch = both - pass;
+        } sets;
+        std::vector<Arrays> arr{cur};
+        for (int k = 1; k < nsets; ++k) {
+            double *q[3] = {nullptr, nullptr, nullptr};
+            for (double *&qq : q) {
+                check(hipMalloc(&qq, sizeof(double) * n), "hipMalloc");
+                sets.p.push_back(qq);
+            }
+            check(hipMemcpyAsync(q[0], h->x, sizeof(double) * n, hipMemcpyDeviceToDevice, h->s), "d2d");
+            check(hipMemcpyAsync(q[1], h->b, sizeof(double) * n, hipMemcpyDeviceToDevice, h->s), "d2d");
+            arr.push_back(Arrays{q[0], q[1], q[2]});
+        }
+        int next = 0;
+        auto rotate = [&]() {
+            cur = arr[next];
+            next = (next + 1) % nsets;
+            launch();
+        };
+        for (int k = 0; k < nsets; ++k) rotate();               // every set once, untimed
+        *ms_per_launch = time_on(h->s, reps, rotate);
     } else {
         *ms_per_launch = time_on(h->s, reps, launch);
     }

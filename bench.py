@@ -18,11 +18,11 @@ workgroup) are picked from a sweep.  Beside it: the FP64 residual, the step righ
 cell-centred H, a plain copy kernel of the same state (calibration) and
 the FGMRES Gram-Schmidt step against 31 basis vectors (multi-dot,
 reduction, update, norm).
-Every kernel figure is timed with cold caches: each launch follows a read of
-a 640 MB buffer that evicts the kernel's arrays from the L2s and the 256 MB
-Infinity Cache, as the Krylov basis does between two launches inside a
-solve, and the kernel's time is that of the (read, kernel) pairs less that of
-the reads alone.  Back-to-back launches (reported beside,
+Every kernel figure is timed with cold caches, as inside a solve, where the
+Krylov basis streams through the caches between two launches of a kernel:
+back-to-back launches rotate through copies of the kernel's arrays, enough
+that 640 MB of other arrays (more than the L2s and the 256 MB Infinity Cache
+hold) pass between two uses of one copy.  Back-to-back launches (reported beside,
 as `warm_repeated`) can keep a working set below 256 MB, such as the apply's
 142 MB at 128^3, in the Infinity Cache and run faster than HBM allows.
 The `batch` object measures the config-4 workload (64 uneven designs): the
@@ -293,8 +293,8 @@ def main():
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": tsrc,
                      "algorithmic_bytes": nbytes, "ms_per_launch": ms, "launches": a.steps,
-                     "caches": "cold: each launch follows a 640 MB read that evicts the L2s and the Infinity "
-                               "Cache; time of (read, apply) pairs less that of the reads",
+                     "caches": "cold: back-to-back launches rotate through copies of x and y, 640 MB of "
+                               "other arrays between two uses of one copy (L2s + 256 MB Infinity Cache evicted)",
                      "warm_repeated": dict(rate(ms_warm, nbytes), launches=a.steps)},
         "shape_sweep_ms": {f"{c}x{z}": m[0] for (c, z), m in sweep.items()},
         "shape_sweep_warm_ms": {f"{c}x{z}": m[2] for (c, z), m in sweep.items()},

@@ -151,14 +151,16 @@ int qms_set_colwidth(qms_handle *h, int nodes);
  * 4 step right-hand side, 5 cell-centred H, 6 FP64 residual b - M x,
  * 7 Gram-Schmidt step of FGMRES against restart + 1 basis vectors (the
  * largest of a restart cycle: multi-dot, reduction, update, norm).
- * which | QMS_TIME_COLD: the same kernel with cold caches: each launch
- * follows a read of a 640 MB buffer that evicts its arrays from the L2s and
- * the 256 MB Infinity Cache, as the Krylov basis does between two launches
- * inside a solve; the kernel's time is that of the (read, kernel) pairs less
- * that of the reads alone (allocates that buffer for the call; for kernels
- * far shorter than the read, about 0.1 ms, the difference is noisy).  Without it,
- * launches run back to back and a working set below 256 MB can stay in the
- * Infinity Cache between them.
+ * which | QMS_TIME_COLD: the same kernel with cold caches, as inside a solve
+ * where the Krylov basis streams through the caches between two launches:
+ * the launches rotate through copies of the kernel's arrays, enough that at
+ * least 640 MB of other arrays pass between two uses of one copy (more than
+ * the L2s and the 256 MB Infinity Cache hold), back to back and timed as a
+ * whole (allocates the copies for the call; up to 64 of them, so a design
+ * under about 10 MB per copy stays cached).  Without it, launches reuse the
+ * handle's arrays and a working set below 256 MB can stay in the Infinity
+ * Cache between them.  The Gram-Schmidt step (7) streams its 4 GB basis
+ * either way.
  * Operator bytes are the workload's: x, y (and b) plus the six FP64
  * coefficient arrays, also when the kernel reads the class table instead.
  * No reference counterpart (measurement only). */

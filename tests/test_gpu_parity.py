@@ -178,10 +178,9 @@ def test_time_kernel_selectors():
                       native.K_GRAM_SCHMIDT):
             ms, nbytes = s.time_kernel(which, 3, v=[1.0, 0.0, 0.0])
             assert ms > 0 and nbytes > 0, which
-            # cold caches: the same bytes; for a kernel this small the time is
-            # a difference of two ~0.1 ms measurements, so only its presence is checked
+            # cold caches (rotating copies of the arrays): the same bytes
             ms_c, nbytes_c = s.time_kernel(which | native.K_COLD, 2, v=[1.0, 0.0, 0.0])
-            assert np.isfinite(ms_c) and nbytes_c == nbytes, which
+            assert ms_c > 0 and nbytes_c == nbytes, which
         with pytest.raises(native.QmsError):
             s.time_kernel(8, 1)
 
