@@ -729,14 +729,15 @@ int qms_time_kernel(qms_handle *h, int which, const double v[3], int reps, doubl
             }
         }
     };
-    if (cold && which != 7) {
+    if (cold && (which == 0 || (which >= 3 && which <= 6))) {
         // Cold caches, as inside a solve, where the Krylov basis streams
         // through the caches between two launches of a kernel: the launches
         // rotate through sets of copies of the kernel's arrays, enough sets
         // that the others' traffic since a set's last use (at least 640 MB)
         // exceeds the L2s and the 256 MB Infinity Cache together.  Launches
         // run back to back and are timed as a whole, like the warm figure.
-        // (The Gram-Schmidt step streams 4 GB and is cold as it is.)
+        // (The Gram-Schmidt step streams 4 GB and is cold as it is; the
+        // smoother sweeps are timed as they are.)
         const int64_t per = 16 * n;                              // x read, w written, bytes
         const int nsets = (int)std::min<int64_t>(64, 1 + ((int64_t)640 << 20) / per + 1);
         struct Sets {

@@ -160,7 +160,7 @@ int qms_set_colwidth(qms_handle *h, int nodes);
  * under about 10 MB per copy stays cached).  Without it, launches reuse the
  * handle's arrays and a working set below 256 MB can stay in the Infinity
  * Cache between them.  The Gram-Schmidt step (7) streams its 4 GB basis
- * either way.
+ * either way; the smoother sweeps (1, 2) are timed as without the flag.
  * Operator bytes are the workload's: x, y (and b) plus the six FP64
  * coefficient arrays, also when the kernel reads the class table instead.
  * No reference counterpart (measurement only). */
