@@ -107,3 +107,32 @@ def test_plan_rejects_bad_input():
         native.batch_plan([4, 0], [10, 10], [4, 4], 64)
     with pytest.raises(native.QmsError):
         native.batch_plan([4], [10], [4], -1)
+
+
+def test_plan_properties_hold_for_any_batch():
+    """Property check over generated batches: exact cover of every column's
+    planes, grid size a multiple of 8, and, with persistent workgroups, no
+    workgroup more than a split's cost above its XCD's mean."""
+    hyp = pytest.importorskip("hypothesis")
+    st = hyp.strategies
+
+    design = st.tuples(st.integers(1, 30), st.integers(6, 90), st.sampled_from([1, 2, 3, 4, 7, 8, 16, 32, 64]))
+
+    @hyp.settings(max_examples=60, deadline=None)
+    @hyp.given(st.lists(design, min_size=1, max_size=9), st.sampled_from([0, 8, 16, 40, 256, 1024]))
+    def check(ds, blocks):
+        ncols, nz, band = (list(v) for v in zip(*ds))
+        segs, off = native.batch_plan(ncols, nz, band, blocks)
+        nslots = len(off) - 1
+        assert nslots % 8 == 0 and off[-1] == len(segs)
+        for c in _cover(segs, ncols, nz):
+            assert np.all(c == 1)
+        if blocks:
+            per = nslots // 8
+            loads = _loads(segs, off)
+            for x in range(8):
+                lx = loads[x * per:(x + 1) * per]
+                if lx.sum():
+                    assert lx.max() <= math.ceil(lx.mean()) + 2 * MIN_SPAN + SEG_START
+
+    check()
