@@ -869,6 +869,7 @@ void batch_plan(qms_batch *b) {
         key.insert(key.end(), {(int)ncol, it.g.nz, it.t.kz});
     }
     if (key == b->plan_key) return;
+    b->plan_key.clear();           // no plan on the device until this one is uploaded
     const BatchPlan p = plan_batch(pd, (unsigned)b->blocks);
     if ((uint64_t)p.segs.size() >= 0xffffffffull) throw std::runtime_error("batch too large");
     if (p.segs.size() > b->segs_cap) {
@@ -925,6 +926,7 @@ void batch_items(qms_batch *b, const double *v, int transient) {
 
 void batch_launch(qms_batch *b, int mode, const double *v, int transient) {
     if (mode == 0) {
+        if (b->plan_key.empty() || !b->segs_dev || !b->off_dev) throw std::runtime_error("batch: no schedule on the device");
         if (b->blocks == 0)   // one workgroup per segment
             launch_operator_batch(b->items_dev, b->segs_dev, nullptr, b->nsegs, b->tx, b->cls, b->s);
         else
