@@ -741,11 +741,13 @@ int qms_time_kernel(qms_handle *h, int which, const double v[3], int reps, doubl
         const int64_t per = 16 * n;                              // x read, w written, bytes
         const int nsets = (int)std::min<int64_t>(64, 1 + ((int64_t)640 << 20) / per + 1);
         struct Sets {
+            hipStream_t s;
             std::vector<double *> p;
             ~Sets() {
+                hipStreamSynchronize(s);          // no launch may still use a copy (also on an error exit)
                 for (double *q : p) hipFree(q);
             }
-        } sets;
+        } sets{h->s, {}};
         std::vector<Arrays> arr{cur};
         for (int k = 1; k < nsets; ++k) {
             double *q[3] = {nullptr, nullptr, nullptr};
