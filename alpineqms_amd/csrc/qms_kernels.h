@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <functional>
 #include "qms_grid.h"
 
 namespace qms {
@@ -100,66 +101,102 @@ void launch_resid(const Grid &g, const CoeffsD &c, const OpParams &op, const dou
 void launch_apply(const Grid &g, const CoeffsD &c, const OpParams &op, const double *x, double *y, hipStream_t s);
 void launch_va_f32(const Grid &g, const CoeffsD &c, const OpParams &op, const float *x, float *y, hipStream_t s);
 void launch_va_f64(const Grid &g, const CoeffsD &c, const OpParams &op, const double *x, double *y, hipStream_t s);
+// y = D^-1 M x for an FP32-stored x (FP64 arithmetic and output)
+void launch_apply_f32in(const Grid &g, const CoeffsD &c, const OpParams &op, const float *x, double *y, hipStream_t s);
 void launch_hfield(const Grid &g, const CoeffsD &c, const double *x, double *H, int unit_mu, hipStream_t s);
 
 // ---------------------------------------------------------------------------
-// Multigrid hierarchy (one per block: the coupled A block or the V block).
-// Level 0 is evaluated on the fly from the material coefficients; levels
-// >= 1 hold explicit Galerkin stencils in difference form:
+// Multigrid hierarchy of one diagonal block of the preconditioner (the
+// coupled A block or the V block).  Level 0 is evaluated on the fly from the
+// coefficient arrays; levels >= 1 hold explicit Galerkin stencils in
+// difference form:
 //   (A x)_p = e_p x_p + sum_d c_{p,d} (x_p - x_{p+d}) + sum_s a_{p,s} y_{p+o_s}
 // where d runs over the 6 axis neighbours of the same component and s over
 // the 4+4 cross-component couplings (A block only).
 // ---------------------------------------------------------------------------
-template <typename T>
-struct Level {
+constexpr int MG_MAX_LEVELS = 16;
+
+struct LevelDims {
     int nx, ny, nz;
     int64_t n1;
-    int ncomp;            // 3 (A block) or 1 (V block)
-    T *cond;              // [ncomp][6][n1] conductances (-x,+x,-y,+y,-z,+z)
-    T *exc;               // [ncomp][n1] excess diagonal
-    T *cross;             // [ncomp][8][n1] cross-component coefficients (A block)
-    T *dinv;              // [ncomp][n1] 1/diag, 0 on inactive rows
-    T *x, *b, *r, *t;     // [ncomp][n1] work vectors (x,b unused on level 0)
 };
 
-struct MGParams {
-    int nu;               // pre/post smoothing sweeps
-    double omega;         // Jacobi damping
-    double alpha;         // coarse-correction scaling
+// Grid constants of the level-0 rows, in the working precision.
+template <typename T>
+struct FineConst {
+    T ih2[3];        // 1/h_a^2
+    T ixy, ixz, iyz; // 1/(h_a h_b)
+    T idt;           // 1/dt (0: steady, no mass term)
+    T sigfloor;      // sigma floor of the V Laplacian
+};
+
+// Device coefficient arrays of one precision: 1/Mu_{x,y,z}, Sigma_{x,y,z}.
+template <typename T>
+struct CoeffsT {
+    const T *im[3];
+    const T *sg[3];
+};
+// Host copies of the FP64 coefficient arrays (node-sized).
+struct HostCoef {
+    const double *im[3];
+    const double *sg[3];
+};
+
+template <typename T>
+struct MgLevel {
+    LevelDims d{0, 0, 0, 0};
+    int ncomp = 0;             // 3 (A block) or 1 (V block)
+    T *cond = nullptr;         // [ncomp][6][n1] conductances (-x,+x,-y,+y,-z,+z)
+    T *exc = nullptr;          // [ncomp][n1] excess diagonal
+    T *cross = nullptr;        // [ncomp][8][n1] cross-component coefficients (A block)
+    T *dinv = nullptr;         // [ncomp][n1] 1/diag, 0 on inactive rows
+    T *x = nullptr, *b = nullptr, *r = nullptr, *t = nullptr;   // [ncomp][n1] (x, b unused on level 0)
 };
 
 template <typename T>
 struct Hierarchy {
-    int block;            // 0: A block (3 components), 1: V block
-    int nlev;             // number of levels (level 0 on the fly)
-    Level<T> lev[16];
-    // coarsest-level dense inverse
-    int ncoarse;          // active unknowns on the coarsest level
-    int *coarse_idx;      // [ncoarse] -> comp*n1 + p on the coarsest level
-    double *coarse_inv;   // [ncoarse*ncoarse] row-major FP64
+    int block = 0;             // 0: A block (3 components), 1: V block
+    int nlev = 0;
+    Grid g{};
+    const T *im[3] = {nullptr, nullptr, nullptr};
+    const T *sg[3] = {nullptr, nullptr, nullptr};
+    FineConst<T> fk{};
+    MgLevel<T> lev[MG_MAX_LEVELS];
+    int ncoarse = 0;           // active unknowns of the coarsest level
+    int *coarse_idx = nullptr; // [ncoarse] -> comp*n1 + p on the coarsest level
+    double *coarse_inv = nullptr;   // [ncoarse*ncoarse] row-major FP64 inverse
 };
 
-// Level-0 data shared by both blocks.
-struct Fine {
-    Grid g;
-    CoeffsF c32;
-    CoeffsD c64;
-    double dt;            // 0 for steady (no mass term)
+struct MGParams {
+    int nu;          // pre/post smoothing sweeps
+    double omega;    // Jacobi damping
+    double alpha;    // coarse-correction scaling
 };
 
+// Device allocations of the hierarchy go through the owner (which frees them).
+struct MgAlloc {
+    std::function<void *(size_t)> alloc;
+    hipStream_t stream;
+};
+
+FineConst<double> fine_const(const Grid &g, double dt);
+// Builds the coarse levels on the host in FP64 from the host coefficients and
+// uploads them in precision T; dt = 0 builds the steady operator.
 template <typename T>
-void mg_build(const Fine &f, Hierarchy<T> &h, hipStream_t s);
+void mg_setup(const Grid &g, const HostCoef &hc, const CoeffsT<T> &dc, double dt, int block, const MgAlloc &al,
+              Hierarchy<T> &h);
+// x0 <- one V-cycle applied to b0 (x0, b0: [ncomp][n1] of level 0).
 template <typename T>
-void mg_vcycle(const Fine &f, Hierarchy<T> &h, const MGParams &mp, T *x0, const T *b0, hipStream_t s);
+void mg_vcycle(Hierarchy<T> &h, const MGParams &mp, T *x0, const T *b0, hipStream_t s);
 // One level-0 smoother sweep (for kernel timing).
 template <typename T>
-void mg_fine_sweep(const Fine &f, Hierarchy<T> &h, const MGParams &mp, const T *x, const T *b, T *xo, hipStream_t s);
+void mg_fine_sweep(Hierarchy<T> &h, const MGParams &mp, const T *x, const T *b, T *xo, hipStream_t s);
+template <typename T>
+void launch_fill(T *x, int64_t n, T v, hipStream_t s);
 
-// conversions / vector kernels
+// conversions / vector kernels at the Krylov / multigrid boundary
 void launch_d2t(const double *src, const double *scale, float *dst, int64_t n, hipStream_t s);
 void launch_d2t(const double *src, const double *scale, double *dst, int64_t n, hipStream_t s);
-void launch_t2d(const float *src, double *dst, int64_t n, hipStream_t s);
-void launch_t2d(const double *src, double *dst, int64_t n, hipStream_t s);
 void launch_sub(float *a, const float *b, int64_t n, hipStream_t s);
 void launch_sub(double *a, const double *b, int64_t n, hipStream_t s);
 
@@ -171,6 +208,15 @@ void launch_orth_update(double *const *V, int nv, const double *h, double *w, in
                         hipStream_t s);
 void launch_scale(double *dst, const double *src, double a, int64_t n, hipStream_t s);
 void launch_axpy_list(double *x, const double *const *Z, int nz, const double *y_dev, int64_t n, hipStream_t s);
+void launch_axpy_list(double *x, const float *const *Z, int nz, const double *y_dev, int64_t n, hipStream_t s);
+void launch_scale_dev(double *dst, const double *src, const double *a_dev, int64_t n, hipStream_t s);
+// FGMRES cycle scalars on the device (st[KS_*])
+enum { KS_BNORM = 0, KS_TOL = 1, KS_INVH = 2, KS_EST = 3, KS_K = 4, KS_ITERS = 5, KS_COUNT = 8 };
+void launch_cycle_start(int m, double beta, double bnorm, double tol, double *g, double *st, hipStream_t s);
+void launch_givens(int j, int m, const double *hcol, double *H, double *cs, double *sn, double *g, double *st,
+                   hipStream_t s);
+void launch_backsub(int k, int m, const double *H, const double *g, double *y, hipStream_t s);
+void launch_sqrt_to(const double *src, double *dst, hipStream_t s);
 void launch_norm2(const double *w, int64_t n, double *partial, int nblk, hipStream_t s);
 int reduce_blocks(int64_t n);
 void launch_copy(double *dst, const double *src, int64_t n, hipStream_t s);

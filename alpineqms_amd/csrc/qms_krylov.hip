@@ -162,14 +162,92 @@ __global__ void k_scale(double *__restrict__ dst, const double *__restrict__ src
     dst[e] = src[e] * a;
 }
 
-// x += sum_i y_i Z_i
-__global__ void k_axpy_list(double *__restrict__ x, const double *const *__restrict__ Z, int nz,
-                            const double *__restrict__ y, int64_t n) {
+// x += sum_i y_i Z_i (Z in the preconditioner's precision; FP64 sums)
+template <typename TZ>
+__global__ void __launch_bounds__(256) k_axpy_list(double *__restrict__ x, const TZ *const *__restrict__ Z, int nz,
+                                                   const double *__restrict__ y, int64_t n) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
     double s = x[e];
-    for (int i = 0; i < nz; ++i) s += y[i] * Z[i][e];
+    for (int i = 0; i < nz; ++i) s += y[i] * (double)Z[i][e];
     x[e] = s;
+}
+
+// dst = src * a[0] (a device scalar, so the launch needs no host value)
+__global__ void __launch_bounds__(256) k_scale_dev(double *__restrict__ dst, const double *__restrict__ src,
+                                                   const double *__restrict__ a, int64_t n) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    dst[e] = src[e] * a[0];
+}
+
+// ---------------------------------------------------------------------------
+// FGMRES bookkeeping on the device (one thread): the Hessenberg matrix H
+// ((m+1) x m, row-major), the Givens rotations cs/sn and the rotated
+// right-hand side g live in device memory, so an iteration needs no host
+// round trip.  st holds the cycle's scalars (KS_* below).
+// ---------------------------------------------------------------------------
+// A new cycle from a residual of norm beta: g = beta e_1, no iteration done.
+__global__ void k_cycle_start(int m, double beta, double bnorm, double tol, double *__restrict__ g,
+                              double *__restrict__ st) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int i = 0; i <= m; ++i) g[i] = 0.0;
+    g[0] = beta;
+    st[KS_BNORM] = bnorm;
+    st[KS_TOL] = tol;
+    st[KS_INVH] = 0.0;
+    st[KS_EST] = bnorm > 0 ? beta / bnorm : beta;
+    st[KS_K] = 0.0;
+    st[KS_ITERS] = 0.0;
+}
+
+// Iteration j: hcol[0..j] are the Gram-Schmidt coefficients of w against
+// V_0..V_j and hcol[j+1] = |w|^2 after the orthogonalisation.  Stores column j
+// of H, rotates it, extends the rotated right-hand side, records the residual
+// estimate |g[j+1]| / bnorm and, the first time it is below tol/2 (or the
+// Krylov space is exhausted), the iteration count k = j+1 of the solution.
+// st[KS_INVH] = 1/|w| scales w into V_{j+1}.
+__global__ void k_givens(int j, int m, const double *__restrict__ hcol, double *__restrict__ H,
+                         double *__restrict__ cs, double *__restrict__ sn, double *__restrict__ g,
+                         double *__restrict__ st) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int i = 0; i <= j; ++i) H[(int64_t)i * m + j] = hcol[i];
+    const double hn = sqrt(fmax(hcol[j + 1], 0.0));
+    for (int i = 0; i < j; ++i) {
+        const double a = H[(int64_t)i * m + j], b = H[(int64_t)(i + 1) * m + j];
+        H[(int64_t)i * m + j] = cs[i] * a + sn[i] * b;
+        H[(int64_t)(i + 1) * m + j] = -sn[i] * a + cs[i] * b;
+    }
+    const double a = H[(int64_t)j * m + j];
+    const double d = hypot(a, hn);
+    cs[j] = d > 0 ? a / d : 1.0;
+    sn[j] = d > 0 ? hn / d : 0.0;
+    H[(int64_t)j * m + j] = d;
+    g[j + 1] = -sn[j] * g[j];
+    g[j] = cs[j] * g[j];
+    const double bnorm = st[KS_BNORM];
+    const double est = fabs(g[j + 1]) / (bnorm > 0 ? bnorm : 1.0);
+    st[KS_EST] = est;
+    st[KS_ITERS] = (double)(j + 1);
+    if (st[KS_K] == 0.0 && (est <= 0.5 * st[KS_TOL] || hn == 0.0 || d == 0.0)) st[KS_K] = (double)(j + 1);
+    st[KS_INVH] = hn > 0 ? 1.0 / hn : 0.0;
+}
+
+// y = R^-1 g over the first k columns (R: the rotated upper-triangular H).
+__global__ void k_backsub(int k, int m, const double *__restrict__ H, const double *__restrict__ g,
+                          double *__restrict__ y) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int i = k - 1; i >= 0; --i) {
+        double t = g[i];
+        for (int c = i + 1; c < k; ++c) t -= H[(int64_t)i * m + c] * y[c];
+        const double r = H[(int64_t)i * m + i];
+        y[i] = r != 0.0 ? t / r : 0.0;
+    }
+}
+
+// dst = sqrt(max(src, 0)) (a reduced sum of squares to a norm, on the device)
+__global__ void k_sqrt_to(const double *__restrict__ src, double *__restrict__ dst) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) dst[0] = sqrt(fmax(src[0], 0.0));
 }
 
 // Plain streaming copy (calibration reference for HBM byte counters).
@@ -220,7 +298,26 @@ void launch_scale(double *dst, const double *src, double a, int64_t n, hipStream
     hipLaunchKernelGGL(k_scale, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dst, src, a, n);
 }
 void launch_axpy_list(double *x, const double *const *Z, int nz, const double *y_dev, int64_t n, hipStream_t s) {
-    hipLaunchKernelGGL(k_axpy_list, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, Z, nz, y_dev, n);
+    hipLaunchKernelGGL(k_axpy_list<double>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, Z, nz, y_dev, n);
+}
+void launch_axpy_list(double *x, const float *const *Z, int nz, const double *y_dev, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_axpy_list<float>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, Z, nz, y_dev, n);
+}
+void launch_scale_dev(double *dst, const double *src, const double *a_dev, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_scale_dev, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dst, src, a_dev, n);
+}
+void launch_cycle_start(int m, double beta, double bnorm, double tol, double *g, double *st, hipStream_t s) {
+    hipLaunchKernelGGL(k_cycle_start, dim3(1), dim3(64), 0, s, m, beta, bnorm, tol, g, st);
+}
+void launch_givens(int j, int m, const double *hcol, double *H, double *cs, double *sn, double *g, double *st,
+                   hipStream_t s) {
+    hipLaunchKernelGGL(k_givens, dim3(1), dim3(64), 0, s, j, m, hcol, H, cs, sn, g, st);
+}
+void launch_backsub(int k, int m, const double *H, const double *g, double *y, hipStream_t s) {
+    hipLaunchKernelGGL(k_backsub, dim3(1), dim3(64), 0, s, k, m, H, g, y);
+}
+void launch_sqrt_to(const double *src, double *dst, hipStream_t s) {
+    hipLaunchKernelGGL(k_sqrt_to, dim3(1), dim3(64), 0, s, src, dst);
 }
 
 }  // namespace qms

@@ -1,344 +1,361 @@
-// Geometric multigrid for the two diagonal blocks of the block-triangular
-// preconditioner:
-//   A block: K + Sigma/dt restricted to the free Ax, Ay, Az unknowns
-//            (curl-curl + component-wise gauge penalty + mass, no advection),
-//            coupled across the three components.
-//   V block: the sigma-floored Laplacian of the V rows (QMS_3D.f90:1237-1281).
-// Coarse spaces: 2x2x2 aggregation of each component's free unknowns
-// (coarse index = fine index >> 1), Galerkin coarse operators.  With this
-// aggregation the coarse operator keeps exactly the fine sparsity pattern
-// (7-point self stencil plus the 4+4 cross-component offsets), so every level
-// is stored as the same structured stencil.  All couplings are kept in
-// difference form with an explicit excess diagonal; in FP32 this preserves the
-// relative accuracy of the 1e-3 S/m background links next to 6e7 S/m
-// conductors.
+// Multigrid for the two diagonal blocks of the solver's block-triangular
+// preconditioner (the linear solve replaces PARDISO, QMS_3D.f90:1314-1317,
+// :2193-2203):
+//   A block: K + Sigma/dt on the free Ax, Ay, Az unknowns -- curl-curl plus
+//            the component-wise gauge penalty plus the mass term of the
+//            reference's A rows (QMS_3D.f90:476-607, :650-781, :824-955,
+//            :1504, :1681, :1857), without the frame-velocity terms; the three
+//            components stay coupled through the curl-curl cross terms.
+//   V block: the sigma-floored Laplacian of the V rows (:1237-1281).
+// Couplings to fixed (Dirichlet) unknowns are eliminated: the preconditioner
+// acts on corrections, which vanish there.
+//
+// Coarse spaces: 2x2x2 aggregation of each component's active unknowns
+// (coarse index = fine index >> 1), Galerkin coarse operators.  This
+// aggregation keeps the fine pattern (7-point self stencil plus the 4+4
+// cross-component offsets), so every coarse level is one structured stencil
+// in difference form:
+//   (A x)_p = e_p x_p + sum_d c_{p,d} (x_p - x_{p+d}) + sum_s a_{p,s} y_{p+o_s}
+// which keeps the relative accuracy of 1e-3 S/m background links next to
+// 6e7 S/m conductors in FP32.
+//
+// The coarse operators are built once per design on the host in FP64 (from the
+// FP64 Yee coefficients) and uploaded in the working precision; the device
+// runs only element-wise stencil kernels (smoother, residual, transfers) and a
+// one-workgroup dense solve on the coarsest level.  Level 0 is evaluated on
+// the fly from the coefficient arrays.
+//
+// Every stencil table below is a constexpr function of compile-time indices
+// inside fully unrolled loops, so no kernel indexes a private array at run
+// time (none needs scratch memory).
 #include <hip/hip_runtime.h>
 #include <math.h>
-#include <vector>
+#include <string.h>
+#include <algorithm>
 #include <stdexcept>
 #include <string>
+#include <vector>
 #include "qms_kernels.h"
 
 namespace qms {
 
-// Self-neighbour offsets: -x, +x, -y, +y, -z, +z
-__device__ __forceinline__ void self_off(int d, int &di, int &dj, int &dk) {
-    di = (d == 0) ? -1 : (d == 1) ? 1 : 0;
-    dj = (d == 2) ? -1 : (d == 3) ? 1 : 0;
-    dk = (d == 4) ? -1 : (d == 5) ? 1 : 0;
-}
+// ---------------------------------------------------------------------------
+// Stencil tables
+// ---------------------------------------------------------------------------
+// Self neighbour d of a node: -x, +x, -y, +y, -z, +z.
+QMS_HD constexpr int so_i(int d) { return d == 0 ? -1 : d == 1 ? 1 : 0; }
+QMS_HD constexpr int so_j(int d) { return d == 2 ? -1 : d == 3 ? 1 : 0; }
+QMS_HD constexpr int so_k(int d) { return d == 4 ? -1 : d == 5 ? 1 : 0; }
 
-// Cross-coupling target (component, offset) for component c and slot s.
-// Slots 0-3 couple to the first other component, 4-7 to the second, with the
-// offsets of the reference's rows (QMS_3D.f90:546-592, :727-770, :844-888).
-__device__ __forceinline__ void cross_off(int c, int s, int &tc, int &di, int &dj, int &dk) {
-    // table[c][s] = {tc, di, dj, dk}
-    const int T[3][8][4] = {
-        {{1, 0, -1, 0}, {1, 1, -1, 0}, {1, 0, 0, 0}, {1, 1, 0, 0}, {2, 0, 0, -1}, {2, 1, 0, -1}, {2, 0, 0, 0}, {2, 1, 0, 0}},
-        {{0, -1, 0, 0}, {0, 0, 0, 0}, {0, -1, 1, 0}, {0, 0, 1, 0}, {2, 0, 0, -1}, {2, 0, 1, -1}, {2, 0, 0, 0}, {2, 0, 1, 0}},
-        {{0, -1, 0, 0}, {0, 0, 0, 0}, {0, -1, 0, 1}, {0, 0, 0, 1}, {1, 0, -1, 0}, {1, 0, 0, 0}, {1, 0, -1, 1}, {1, 0, 0, 1}}};
-    tc = T[c][s][0]; di = T[c][s][1]; dj = T[c][s][2]; dk = T[c][s][3];
+// Cross-component coupling slot s of component c: slots 0-3 couple to the
+// first other component, 4-7 to the second, at the offsets where the
+// reference's rows read them (Ax row :546-592, Ay row :727-770, Az row
+// :844-888).  Encoded as tc*64 + (di+1)*16 + (dj+1)*4 + (dk+1).
+QMS_HD constexpr int xo_code(int c, int s) {
+    constexpr int T[3][8] = {
+        // Ax row: Ay at (0,-1,0) (1,-1,0) (0,0,0) (1,0,0); Az at (0,0,-1) (1,0,-1) (0,0,0) (1,0,0)
+        {1 * 64 + 1 * 16 + 0 * 4 + 1, 1 * 64 + 2 * 16 + 0 * 4 + 1, 1 * 64 + 1 * 16 + 1 * 4 + 1, 1 * 64 + 2 * 16 + 1 * 4 + 1,
+         2 * 64 + 1 * 16 + 1 * 4 + 0, 2 * 64 + 2 * 16 + 1 * 4 + 0, 2 * 64 + 1 * 16 + 1 * 4 + 1, 2 * 64 + 2 * 16 + 1 * 4 + 1},
+        // Ay row: Ax at (-1,0,0) (0,0,0) (-1,1,0) (0,1,0); Az at (0,0,-1) (0,1,-1) (0,0,0) (0,1,0)
+        {0 * 64 + 0 * 16 + 1 * 4 + 1, 0 * 64 + 1 * 16 + 1 * 4 + 1, 0 * 64 + 0 * 16 + 2 * 4 + 1, 0 * 64 + 1 * 16 + 2 * 4 + 1,
+         2 * 64 + 1 * 16 + 1 * 4 + 0, 2 * 64 + 1 * 16 + 2 * 4 + 0, 2 * 64 + 1 * 16 + 1 * 4 + 1, 2 * 64 + 1 * 16 + 2 * 4 + 1},
+        // Az row: Ax at (-1,0,0) (0,0,0) (-1,0,1) (0,0,1); Ay at (0,-1,0) (0,0,0) (0,-1,1) (0,0,1)
+        {0 * 64 + 0 * 16 + 1 * 4 + 1, 0 * 64 + 1 * 16 + 1 * 4 + 1, 0 * 64 + 0 * 16 + 1 * 4 + 2, 0 * 64 + 1 * 16 + 1 * 4 + 2,
+         1 * 64 + 1 * 16 + 0 * 4 + 1, 1 * 64 + 1 * 16 + 1 * 4 + 1, 1 * 64 + 1 * 16 + 0 * 4 + 2, 1 * 64 + 1 * 16 + 1 * 4 + 2}};
+    return T[c][s];
 }
+QMS_HD constexpr int xo_c(int c, int s) { return xo_code(c, s) >> 6; }
+QMS_HD constexpr int xo_i(int c, int s) { return ((xo_code(c, s) >> 4) & 3) - 1; }
+QMS_HD constexpr int xo_j(int c, int s) { return ((xo_code(c, s) >> 2) & 3) - 1; }
+QMS_HD constexpr int xo_k(int c, int s) { return (xo_code(c, s) & 3) - 1; }
 
-struct Dims {
-    int nx, ny, nz;
-    int64_t n1;
-};
-__device__ __forceinline__ int64_t didx(const Dims &d, int i, int j, int k) {
+QMS_HD int64_t lidx(const LevelDims &d, int i, int j, int k) {
     return (int64_t)i + (int64_t)d.nx * ((int64_t)j + (int64_t)d.ny * k);
 }
-__device__ __forceinline__ bool inside(const Dims &d, int i, int j, int k) {
+QMS_HD bool inside(const LevelDims &d, int i, int j, int k) {
     return i >= 0 && j >= 0 && k >= 0 && i < d.nx && j < d.ny && k < d.nz;
 }
 
 // ---------------------------------------------------------------------------
-// Level-0 providers: stencil rows computed on the fly from the material
-// coefficients, with couplings to fixed (Dirichlet) unknowns eliminated
-// (conductance moved to the excess diagonal, cross coefficient dropped) since
-// the preconditioner acts on corrections that vanish there.
+// Level-0 rows from the Yee coefficients (shared by the device kernels and
+// the host Galerkin build).  IM(a, di, dj, dk) is 1/Mu_a at the node offset,
+// SG(a) is Sigma_a at the node.
 // ---------------------------------------------------------------------------
 template <typename T>
-struct FineA {
+struct Row {
+    T cond[6];
+    T exc;
+    T cross[8];
+};
+
+// Coefficient access at node (i,j,k) of a node-sized array.
+template <typename T>
+struct CoefAt {
+    const T *im[3];
+    const T *sg[3];
+    int64_t p;
+    int sy, sz;
+    QMS_HD T IM(int a, int di, int dj, int dk) const { return im[a][p + di + (int64_t)dj * sy + (int64_t)dk * sz]; }
+    QMS_HD T SG(int a, int di, int dj, int dk) const { return sg[a][p + di + (int64_t)dj * sy + (int64_t)dk * sz]; }
+};
+
+// A-block row of component C at a free node (see the file comment), before
+// the Dirichlet elimination.  The values are the reference's coefficients:
+// Ax row :476-607 (+ :1504), Ay row :650-781 (+ :1681), Az row :824-955
+// (+ :1857).
+template <int C, typename T, typename CA>
+QMS_HD void row_a_raw(const CA &ca, const FineConst<T> &fk, Row<T> &r) {
+    if (C == 0) {
+        const T a = ca.IM(2, 0, -1, 0) * fk.ixy, b = ca.IM(2, 0, 0, 0) * fk.ixy;
+        const T e = ca.IM(1, 0, 0, -1) * fk.ixz, f = ca.IM(1, 0, 0, 0) * fk.ixz;
+        r.cond[0] = ca.IM(0, -1, 0, 0) * fk.ih2[0]; r.cond[1] = ca.IM(0, 0, 0, 0) * fk.ih2[0];
+        r.cond[2] = ca.IM(2, 0, -1, 0) * fk.ih2[1]; r.cond[3] = ca.IM(2, 0, 0, 0) * fk.ih2[1];
+        r.cond[4] = ca.IM(1, 0, 0, -1) * fk.ih2[2]; r.cond[5] = ca.IM(1, 0, 0, 0) * fk.ih2[2];
+        r.cross[0] = a; r.cross[1] = -a; r.cross[2] = -b; r.cross[3] = b;
+        r.cross[4] = e; r.cross[5] = -e; r.cross[6] = -f; r.cross[7] = f;
+        r.exc = ca.SG(0, 0, 0, 0) * fk.idt;
+    } else if (C == 1) {
+        const T a = ca.IM(2, -1, 0, 0) * fk.ixy, b = ca.IM(2, 0, 0, 0) * fk.ixy;
+        const T e = ca.IM(0, 0, 0, -1) * fk.iyz, f = ca.IM(0, 0, 0, 0) * fk.iyz;
+        r.cond[0] = ca.IM(2, -1, 0, 0) * fk.ih2[0]; r.cond[1] = ca.IM(2, 0, 0, 0) * fk.ih2[0];
+        r.cond[2] = ca.IM(1, 0, -1, 0) * fk.ih2[1]; r.cond[3] = ca.IM(1, 0, 0, 0) * fk.ih2[1];
+        r.cond[4] = ca.IM(0, 0, 0, -1) * fk.ih2[2]; r.cond[5] = ca.IM(0, 0, 0, 0) * fk.ih2[2];
+        r.cross[0] = a; r.cross[1] = -b; r.cross[2] = -a; r.cross[3] = b;
+        r.cross[4] = e; r.cross[5] = -e; r.cross[6] = -f; r.cross[7] = f;
+        r.exc = ca.SG(1, 0, 0, 0) * fk.idt;
+    } else {
+        const T a = ca.IM(1, -1, 0, 0) * fk.ixz, b = ca.IM(1, 0, 0, 0) * fk.ixz;
+        const T e = ca.IM(0, 0, -1, 0) * fk.iyz, f = ca.IM(0, 0, 0, 0) * fk.iyz;
+        r.cond[0] = ca.IM(1, -1, 0, 0) * fk.ih2[0]; r.cond[1] = ca.IM(1, 0, 0, 0) * fk.ih2[0];
+        r.cond[2] = ca.IM(0, 0, -1, 0) * fk.ih2[1]; r.cond[3] = ca.IM(0, 0, 0, 0) * fk.ih2[1];
+        r.cond[4] = ca.IM(2, 0, 0, -1) * fk.ih2[2]; r.cond[5] = ca.IM(2, 0, 0, 0) * fk.ih2[2];
+        r.cross[0] = a; r.cross[1] = -b; r.cross[2] = -a; r.cross[3] = b;
+        r.cross[4] = e; r.cross[5] = -f; r.cross[6] = -e; r.cross[7] = f;
+        r.exc = ca.SG(2, 0, 0, 0) * fk.idt;
+    }
+}
+
+// V-block row at a free node: floored-sigma Laplacian (:1237-1281).
+template <typename T, typename CA>
+QMS_HD void row_v_raw(const CA &ca, const FineConst<T> &fk, Row<T> &r) {
+    const T fl = fk.sigfloor;
+    auto S = [&](int a, int di, int dj, int dk) {
+        const T s = ca.SG(a, di, dj, dk);
+        return s > fl ? s : fl;
+    };
+    r.cond[0] = S(0, -1, 0, 0) * fk.ih2[0]; r.cond[1] = S(0, 0, 0, 0) * fk.ih2[0];
+    r.cond[2] = S(1, 0, -1, 0) * fk.ih2[1]; r.cond[3] = S(1, 0, 0, 0) * fk.ih2[1];
+    r.cond[4] = S(2, 0, 0, -1) * fk.ih2[2]; r.cond[5] = S(2, 0, 0, 0) * fk.ih2[2];
+    r.exc = (T)0;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) r.cross[s] = (T)0;
+}
+
+// Dirichlet elimination: a coupling to a fixed neighbour moves into the
+// excess diagonal (self) or is dropped (cross).
+template <int C, int NC, typename T>
+QMS_HD void eliminate(const Grid &g, int i, int j, int k, Row<T> &r) {
+    constexpr int F = NC == 1 ? FV : C;
+#pragma unroll
+    for (int d = 0; d < 6; ++d)
+        if (!free_f(g, F, i + so_i(d), j + so_j(d), k + so_k(d))) { r.exc += r.cond[d]; r.cond[d] = (T)0; }
+    if (NC == 3) {
+#pragma unroll
+        for (int s = 0; s < 8; ++s)
+            if (!free_f(g, xo_c(C, s), i + xo_i(C, s), j + xo_j(C, s), k + xo_k(C, s))) r.cross[s] = (T)0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Row providers for the device kernels
+// ---------------------------------------------------------------------------
+// Level 0: rows evaluated from the coefficient arrays.
+template <typename T, int NC>
+struct FineProv {
     Grid g;
     const T *im[3];
     const T *sg[3];
-    T inv_dt;       // 1/dt, 0 for steady
-    static constexpr int NC = 3;
-    __device__ __forceinline__ Dims dims() const { return Dims{g.nx, g.ny, g.nz, g.n1}; }
-    __device__ __forceinline__ bool active(int c, int i, int j, int k) const { return free_f(g, c, i, j, k); }
-    __device__ __forceinline__ T IM(int a, int i, int j, int k) const { return im[a][gidx(g, i, j, k)]; }
-    __device__ bool row(int c, int i, int j, int k, T cond[6], T &exc, T cross[8]) const {
-        if (!free_f(g, c, i, j, k)) return false;
-        const T hx2 = (T)(g.hx * g.hx), hy2 = (T)(g.hy * g.hy), hz2 = (T)(g.hz * g.hz);
-        const T hxy = (T)(g.hx * g.hy), hxz = (T)(g.hx * g.hz), hyz = (T)(g.hy * g.hz);
-        T cm;
-        if (c == 0) {
-            cond[0] = IM(0, i - 1, j, k) / hx2; cond[1] = IM(0, i, j, k) / hx2;
-            cond[2] = IM(2, i, j - 1, k) / hy2; cond[3] = IM(2, i, j, k) / hy2;
-            cond[4] = IM(1, i, j, k - 1) / hz2; cond[5] = IM(1, i, j, k) / hz2;
-            const T a = IM(2, i, j - 1, k) / hxy, b = IM(2, i, j, k) / hxy;
-            const T e = IM(1, i, j, k - 1) / hxz, f = IM(1, i, j, k) / hxz;
-            cross[0] = a; cross[1] = -a; cross[2] = -b; cross[3] = b;
-            cross[4] = e; cross[5] = -e; cross[6] = -f; cross[7] = f;
-            cm = sg[0][gidx(g, i, j, k)];
-        } else if (c == 1) {
-            cond[0] = IM(2, i - 1, j, k) / hx2; cond[1] = IM(2, i, j, k) / hx2;
-            cond[2] = IM(1, i, j - 1, k) / hy2; cond[3] = IM(1, i, j, k) / hy2;
-            cond[4] = IM(0, i, j, k - 1) / hz2; cond[5] = IM(0, i, j, k) / hz2;
-            const T a = IM(2, i - 1, j, k) / hxy, b = IM(2, i, j, k) / hxy;
-            const T e = IM(0, i, j, k - 1) / hyz, f = IM(0, i, j, k) / hyz;
-            cross[0] = a; cross[1] = -b; cross[2] = -a; cross[3] = b;
-            cross[4] = e; cross[5] = -e; cross[6] = -f; cross[7] = f;
-            cm = sg[1][gidx(g, i, j, k)];
-        } else {
-            cond[0] = IM(1, i - 1, j, k) / hx2; cond[1] = IM(1, i, j, k) / hx2;
-            cond[2] = IM(0, i, j - 1, k) / hy2; cond[3] = IM(0, i, j, k) / hy2;
-            cond[4] = IM(2, i, j, k - 1) / hz2; cond[5] = IM(2, i, j, k) / hz2;
-            const T a = IM(1, i - 1, j, k) / hxz, b = IM(1, i, j, k) / hxz;
-            const T e = IM(0, i, j - 1, k) / hyz, f = IM(0, i, j, k) / hyz;
-            cross[0] = a; cross[1] = -b; cross[2] = -a; cross[3] = b;
-            cross[4] = e; cross[5] = -f; cross[6] = -e; cross[7] = f;
-            cm = sg[2][gidx(g, i, j, k)];
-        }
-        exc = cm * inv_dt;
-#pragma unroll
-        for (int d = 0; d < 6; ++d) {
-            int di, dj, dk;
-            self_off(d, di, dj, dk);
-            if (!free_f(g, c, i + di, j + dj, k + dk)) { exc += cond[d]; cond[d] = (T)0; }
-        }
-#pragma unroll
-        for (int s = 0; s < 8; ++s) {
-            int tc, di, dj, dk;
-            cross_off(c, s, tc, di, dj, dk);
-            if (!free_f(g, tc, i + di, j + dj, k + dk)) cross[s] = (T)0;
-        }
+    FineConst<T> fk;
+    __device__ __forceinline__ LevelDims dims() const { return LevelDims{g.nx, g.ny, g.nz, g.n1}; }
+    template <int C>
+    __device__ __forceinline__ bool active(int i, int j, int k) const { return free_f(g, NC == 1 ? FV : C, i, j, k); }
+    template <int C>
+    __device__ __forceinline__ bool row(int i, int j, int k, Row<T> &r) const {
+        if (!active<C>(i, j, k)) return false;
+        const CoefAt<T> ca{{im[0], im[1], im[2]}, {sg[0], sg[1], sg[2]}, gidx(g, i, j, k), g.nx, g.nx * g.ny};
+        if (NC == 3) row_a_raw<C, T>(ca, fk, r);
+        else row_v_raw<T>(ca, fk, r);
+        eliminate<C, NC, T>(g, i, j, k, r);
         return true;
     }
 };
 
-template <typename T>
-struct FineV {
-    Grid g;
-    const T *sg[3];
-    static constexpr int NC = 1;
-    __device__ __forceinline__ Dims dims() const { return Dims{g.nx, g.ny, g.nz, g.n1}; }
-    __device__ __forceinline__ bool active(int, int i, int j, int k) const { return free_v(g, i, j, k); }
-    __device__ bool row(int, int i, int j, int k, T cond[6], T &exc, T *) const {
-        if (!free_v(g, i, j, k)) return false;
-        const T fl = (T)QMS_SIGFLOOR;
-        const T hx2 = (T)(g.hx * g.hx), hy2 = (T)(g.hy * g.hy), hz2 = (T)(g.hz * g.hz);
-        auto S = [&](int a, int ii, int jj, int kk) { const T s = sg[a][gidx(g, ii, jj, kk)]; return s > fl ? s : fl; };
-        cond[0] = S(0, i - 1, j, k) / hx2; cond[1] = S(0, i, j, k) / hx2;
-        cond[2] = S(1, i, j - 1, k) / hy2; cond[3] = S(1, i, j, k) / hy2;
-        cond[4] = S(2, i, j, k - 1) / hz2; cond[5] = S(2, i, j, k) / hz2;
-        exc = (T)0;
-#pragma unroll
-        for (int d = 0; d < 6; ++d) {
-            int di, dj, dk;
-            self_off(d, di, dj, dk);
-            if (!free_v(g, i + di, j + dj, k + dk)) { exc += cond[d]; cond[d] = (T)0; }
-        }
-        return true;
-    }
-};
-
-// Explicit-stencil provider for levels >= 1.
-template <typename T, int NCOMP>
-struct Expl {
-    Dims d;
-    const T *cond, *exc, *cross, *dinv;
-    static constexpr int NC = NCOMP;
-    __device__ __forceinline__ Dims dims() const { return d; }
-    __device__ __forceinline__ bool active(int c, int i, int j, int k) const {
-        return dinv[(int64_t)c * d.n1 + didx(d, i, j, k)] != (T)0;
-    }
-    __device__ bool row(int c, int i, int j, int k, T cnd[6], T &ex, T crs[8]) const {
-        const int64_t p = didx(d, i, j, k);
-        if (dinv[(int64_t)c * d.n1 + p] == (T)0) return false;
-#pragma unroll
-        for (int q = 0; q < 6; ++q) cnd[q] = cond[((int64_t)c * 6 + q) * d.n1 + p];
-        ex = exc[(int64_t)c * d.n1 + p];
-        if (NCOMP == 3) {
-#pragma unroll
-            for (int s = 0; s < 8; ++s) crs[s] = cross[((int64_t)c * 8 + s) * d.n1 + p];
-        }
-        return true;
-    }
-};
-
-// (A x) for component c at (i,j,k) given the row coefficients.
+// Levels >= 1: explicit arrays.
 template <typename T, int NC>
-__device__ __forceinline__ T apply_row(const Dims &d, const T *x, int c, int i, int j, int k, const T cond[6], T exc,
-                                       const T cross[8]) {
-    const int64_t p = didx(d, i, j, k);
-    const T x0 = x[(int64_t)c * d.n1 + p];
-    T y = exc * x0;
+struct ExplProv {
+    LevelDims d;
+    const T *cond, *exc, *cross, *dinv;
+    __device__ __forceinline__ LevelDims dims() const { return d; }
+    template <int C>
+    __device__ __forceinline__ bool active(int i, int j, int k) const {
+        return dinv[(int64_t)C * d.n1 + lidx(d, i, j, k)] != (T)0;
+    }
+    template <int C>
+    __device__ __forceinline__ bool row(int i, int j, int k, Row<T> &r) const {
+        const int64_t p = lidx(d, i, j, k);
+        if (dinv[(int64_t)C * d.n1 + p] == (T)0) return false;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) r.cond[q] = cond[((int64_t)C * 6 + q) * d.n1 + p];
+        r.exc = exc[(int64_t)C * d.n1 + p];
+        if (NC == 3) {
+#pragma unroll
+            for (int s = 0; s < 8; ++s) r.cross[s] = cross[((int64_t)C * 8 + s) * d.n1 + p];
+        }
+        return true;
+    }
+};
+
+// (A x) of component C at (i,j,k).  Only couplings with a non-zero
+// coefficient read their neighbour, and every read is checked against the
+// level's box.
+template <int C, int NC, typename T>
+__device__ __forceinline__ T apply_row(const LevelDims &d, const T *__restrict__ x, int i, int j, int k, const Row<T> &r,
+                                       T &diag) {
+    const T x0 = x[(int64_t)C * d.n1 + lidx(d, i, j, k)];
+    T y = r.exc * x0;
+    diag = r.exc;
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
-        if (cond[q] != (T)0) {
-            int di, dj, dk;
-            self_off(q, di, dj, dk);
-            y += cond[q] * (x0 - x[(int64_t)c * d.n1 + didx(d, i + di, j + dj, k + dk)]);
-        }
+        diag += r.cond[q];
+        const int ni = i + so_i(q), nj = j + so_j(q), nk = k + so_k(q);
+        if (r.cond[q] != (T)0 && inside(d, ni, nj, nk))
+            y += r.cond[q] * (x0 - x[(int64_t)C * d.n1 + lidx(d, ni, nj, nk)]);
     }
     if (NC == 3) {
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
-            if (cross[s] != (T)0) {
-                int tc, di, dj, dk;
-                cross_off(c, s, tc, di, dj, dk);
-                y += cross[s] * x[(int64_t)tc * d.n1 + didx(d, i + di, j + dj, k + dk)];
-            }
+            const int ni = i + xo_i(C, s), nj = j + xo_j(C, s), nk = k + xo_k(C, s);
+            if (r.cross[s] != (T)0 && inside(d, ni, nj, nk))
+                y += r.cross[s] * x[(int64_t)xo_c(C, s) * d.n1 + lidx(d, ni, nj, nk)];
         }
     }
     return y;
 }
 
-__device__ __forceinline__ bool node_of(const Dims &d, int64_t p, int &i, int &j, int &k) {
+template <typename T>
+__device__ __forceinline__ T row_diag(const Row<T> &r) {
+    T d = r.exc;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) d += r.cond[q];
+    return d;
+}
+
+__device__ __forceinline__ bool node_of(const LevelDims &d, int64_t p, int &i, int &j, int &k) {
     if (p >= d.n1) return false;
-    i = (int)(p % d.nx); j = (int)((p / d.nx) % d.ny); k = (int)(p / ((int64_t)d.nx * d.ny));
+    i = (int)(p % d.nx);
+    j = (int)((p / d.nx) % d.ny);
+    k = (int)(p / ((int64_t)d.nx * d.ny));
     return true;
 }
 
-// Damped Jacobi sweep: xo = x + w (b - A x)/diag; first sweep assumes x = 0.
-template <typename T, typename P>
-__global__ void __launch_bounds__(256) k_smooth(P pv, const T *__restrict__ x, const T *__restrict__ b,
-                                                T *__restrict__ xo, T omega, int first) {
-    const Dims d = pv.dims();
-    int i, j, k;
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (!node_of(d, p, i, j, k)) return;
-#pragma unroll
-    for (int c = 0; c < P::NC; ++c) {
-        T cnd[6], ex, crs[8];
-        const int64_t q = (int64_t)c * d.n1 + p;
-        if (!pv.row(c, i, j, k, cnd, ex, crs)) { xo[q] = (T)0; continue; }
-        T diag = ex;
-#pragma unroll
-        for (int s = 0; s < 6; ++s) diag += cnd[s];
-        if (first) {
-            xo[q] = omega * b[q] / diag;
-        } else {
-            const T ax = apply_row<T, P::NC>(d, x, c, i, j, k, cnd, ex, crs);
-            xo[q] = x[q] + omega * (b[q] - ax) / diag;
-        }
+// ---------------------------------------------------------------------------
+// Kernels (one thread per node, all components of the block)
+// ---------------------------------------------------------------------------
+// Damped Jacobi: xo = x + w (b - A x) / diag; FIRST: x = 0, xo = w b / diag.
+// Inactive rows get 0.
+template <int C, int NC, bool FIRST, typename T, typename P>
+__device__ __forceinline__ void smooth_comp(const P &pv, const LevelDims &d, int64_t p, int i, int j, int k,
+                                            const T *__restrict__ x, const T *__restrict__ b, T *__restrict__ xo,
+                                            T omega) {
+    Row<T> r;
+    const int64_t q = (int64_t)C * d.n1 + p;
+    if (!pv.template row<C>(i, j, k, r)) { xo[q] = (T)0; return; }
+    if (FIRST) {
+        xo[q] = omega * b[q] / row_diag(r);
+    } else {
+        T diag;
+        const T ax = apply_row<C, NC, T>(d, x, i, j, k, r, diag);
+        xo[q] = x[q] + omega * (b[q] - ax) / diag;
     }
 }
 
-template <typename T, typename P>
-__global__ void __launch_bounds__(256) k_resid(P pv, const T *__restrict__ x, const T *__restrict__ b,
-                                               T *__restrict__ r) {
-    const Dims d = pv.dims();
+template <typename T, typename P, int NC, bool FIRST>
+__global__ void __launch_bounds__(256) k_mg_smooth(P pv, const T *__restrict__ x, const T *__restrict__ b,
+                                                   T *__restrict__ xo, T omega) {
+    const LevelDims d = pv.dims();
     int i, j, k;
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (!node_of(d, p, i, j, k)) return;
-#pragma unroll
-    for (int c = 0; c < P::NC; ++c) {
-        T cnd[6], ex, crs[8];
-        const int64_t q = (int64_t)c * d.n1 + p;
-        if (!pv.row(c, i, j, k, cnd, ex, crs)) { r[q] = (T)0; continue; }
-        r[q] = b[q] - apply_row<T, P::NC>(d, x, c, i, j, k, cnd, ex, crs);
+    smooth_comp<0, NC, FIRST, T>(pv, d, p, i, j, k, x, b, xo, omega);
+    if (NC == 3) {
+        smooth_comp<1, NC, FIRST, T>(pv, d, p, i, j, k, x, b, xo, omega);
+        smooth_comp<2, NC, FIRST, T>(pv, d, p, i, j, k, x, b, xo, omega);
     }
 }
 
-// bc = P^T r (sum over the 2x2x2 children)
+template <int C, int NC, typename T, typename P>
+__device__ __forceinline__ void resid_comp(const P &pv, const LevelDims &d, int64_t p, int i, int j, int k,
+                                           const T *__restrict__ x, const T *__restrict__ b, T *__restrict__ r) {
+    Row<T> row;
+    const int64_t q = (int64_t)C * d.n1 + p;
+    if (!pv.template row<C>(i, j, k, row)) { r[q] = (T)0; return; }
+    T diag;
+    r[q] = b[q] - apply_row<C, NC, T>(d, x, i, j, k, row, diag);
+}
+
+template <typename T, typename P, int NC>
+__global__ void __launch_bounds__(256) k_mg_resid(P pv, const T *__restrict__ x, const T *__restrict__ b,
+                                                  T *__restrict__ r) {
+    const LevelDims d = pv.dims();
+    int i, j, k;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (!node_of(d, p, i, j, k)) return;
+    resid_comp<0, NC, T>(pv, d, p, i, j, k, x, b, r);
+    if (NC == 3) {
+        resid_comp<1, NC, T>(pv, d, p, i, j, k, x, b, r);
+        resid_comp<2, NC, T>(pv, d, p, i, j, k, x, b, r);
+    }
+}
+
+// bc = P^T r: the sum over the (up to) 2x2x2 children of each coarse node.
 template <typename T>
-__global__ void k_restrict(Dims fd, Dims cd, int ncomp, const T *__restrict__ r, T *__restrict__ bc) {
+__global__ void __launch_bounds__(256) k_mg_restrict(LevelDims fd, LevelDims cd, int ncomp, const T *__restrict__ r,
+                                                     T *__restrict__ bc) {
     int I, J, K;
-    const int64_t P = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (!node_of(cd, P, I, J, K)) return;
+    const int64_t pc = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (!node_of(cd, pc, I, J, K)) return;
     for (int c = 0; c < ncomp; ++c) {
         T s = (T)0;
-        for (int dk = 0; dk < 2; ++dk)
-            for (int dj = 0; dj < 2; ++dj)
-                for (int di = 0; di < 2; ++di) {
-                    const int i = 2 * I + di, j = 2 * J + dj, k = 2 * K + dk;
-                    if (inside(fd, i, j, k)) s += r[(int64_t)c * fd.n1 + didx(fd, i, j, k)];
-                }
-        bc[(int64_t)c * cd.n1 + P] = s;
+#pragma unroll
+        for (int ch = 0; ch < 8; ++ch) {
+            const int i = 2 * I + (ch & 1), j = 2 * J + ((ch >> 1) & 1), k = 2 * K + (ch >> 2);
+            if (inside(fd, i, j, k)) s += r[(int64_t)c * fd.n1 + lidx(fd, i, j, k)];
+        }
+        bc[(int64_t)c * cd.n1 + pc] = s;
     }
 }
 
-// x += alpha * P xc on active fine rows
-template <typename T, typename PV>
-__global__ void k_prolong(PV pv, Dims cd, const T *__restrict__ xc, T *__restrict__ x, T alpha) {
-    const Dims d = pv.dims();
+// x += alpha * P xc on the active fine rows.
+template <typename T, typename P, int NC>
+__global__ void __launch_bounds__(256) k_mg_prolong(P pv, LevelDims cd, const T *__restrict__ xc, T *__restrict__ x,
+                                                    T alpha) {
+    const LevelDims d = pv.dims();
     int i, j, k;
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (!node_of(d, p, i, j, k)) return;
-    const int64_t P = didx(cd, i >> 1, j >> 1, k >> 1);
-#pragma unroll
-    for (int c = 0; c < PV::NC; ++c)
-        if (pv.active(c, i, j, k)) x[(int64_t)c * d.n1 + p] += alpha * xc[(int64_t)c * cd.n1 + P];
-}
-
-// Galerkin coarse operator of level l (provider pv) into explicit arrays of
-// level l+1.
-template <typename T, typename P>
-__global__ void k_galerkin(P pv, Dims cd, T *__restrict__ cond, T *__restrict__ exc, T *__restrict__ cross,
-                           T *__restrict__ dinv) {
-    const Dims fd = pv.dims();
-    int I, J, K;
-    const int64_t Pn = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (!node_of(cd, Pn, I, J, K)) return;
-#pragma unroll 1
-    for (int c = 0; c < P::NC; ++c) {
-        T ccnd[6] = {0, 0, 0, 0, 0, 0}, cex = 0, ccrs[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        bool any = false;
-#pragma unroll 1
-        for (int ch = 0; ch < 8; ++ch) {
-            const int i = 2 * I + (ch & 1), j = 2 * J + ((ch >> 1) & 1), k = 2 * K + (ch >> 2);
-            if (!inside(fd, i, j, k)) continue;
-            T cnd[6], ex, crs[8];
-            if (!pv.row(c, i, j, k, cnd, ex, crs)) continue;
-            any = true;
-            cex += ex;
-#pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                int di, dj, dk;
-                self_off(q, di, dj, dk);
-                const int ni = i + di, nj = j + dj, nk = k + dk;
-                const bool internal = ((ni >> 1) == I) && ((nj >> 1) == J) && ((nk >> 1) == K);
-                if (!internal) ccnd[q] += cnd[q];
-            }
-            if (P::NC == 3) {
-#pragma unroll
-                for (int s = 0; s < 8; ++s) {
-                    if (crs[s] == (T)0) continue;
-                    int tc, di, dj, dk;
-                    cross_off(c, s, tc, di, dj, dk);
-                    const int oi = ((i + di) >> 1) - I, oj = ((j + dj) >> 1) - J, ok = ((k + dk) >> 1) - K;
-                    const int base = s & 4;
-#pragma unroll
-                    for (int s2 = 0; s2 < 4; ++s2) {
-                        int tc2, ei, ej, ek;
-                        cross_off(c, base + s2, tc2, ei, ej, ek);
-                        if (ei == oi && ej == oj && ek == ok) ccrs[base + s2] += crs[s];
-                    }
-                }
-            }
-        }
-        const int64_t q = (int64_t)c * cd.n1 + Pn;
-        T diag = cex;
-#pragma unroll
-        for (int s = 0; s < 6; ++s) {
-            cond[((int64_t)c * 6 + s) * cd.n1 + Pn] = any ? ccnd[s] : (T)0;
-            diag += ccnd[s];
-        }
-        exc[q] = any ? cex : (T)0;
-        if (P::NC == 3) {
-#pragma unroll
-            for (int s = 0; s < 8; ++s) cross[((int64_t)c * 8 + s) * cd.n1 + Pn] = any ? ccrs[s] : (T)0;
-        }
-        dinv[q] = any ? (T)1 / diag : (T)0;
+    const int64_t pc = lidx(cd, i >> 1, j >> 1, k >> 1);
+    if (pv.template active<0>(i, j, k)) x[p] += alpha * xc[pc];
+    if (NC == 3) {
+        if (pv.template active<1>(i, j, k)) x[d.n1 + p] += alpha * xc[cd.n1 + pc];
+        if (pv.template active<2>(i, j, k)) x[2 * d.n1 + p] += alpha * xc[2 * cd.n1 + pc];
     }
 }
 
-// Coarsest level: x = Ainv * b over the active unknowns (one workgroup).
+// Coarsest level: x = Ainv b over its active unknowns (one workgroup; the
+// entries that are not unknowns were zeroed before).
 template <typename T>
-__global__ void k_coarse_solve(int n, const int *__restrict__ idx, const double *__restrict__ inv,
-                               const T *__restrict__ b, T *__restrict__ x) {
+__global__ void __launch_bounds__(256) k_mg_coarse(int n, const int *__restrict__ idx, const double *__restrict__ inv,
+                                                   const T *__restrict__ b, T *__restrict__ x) {
     for (int r = threadIdx.x; r < n; r += blockDim.x) {
         double s = 0.0;
         for (int c = 0; c < n; ++c) s += inv[(int64_t)r * n + c] * (double)b[idx[c]];
@@ -347,106 +364,122 @@ __global__ void k_coarse_solve(int n, const int *__restrict__ idx, const double 
 }
 
 template <typename T>
-__global__ void k_zero(T *__restrict__ x, int64_t n) {
+__global__ void __launch_bounds__(256) k_fill(T *__restrict__ x, int64_t n, T v) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < n) x[p] = (T)0;
+    if (p < n) x[p] = v;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_copy_t(T *__restrict__ dst, const T *__restrict__ src, int64_t n) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) dst[p] = src[p];
 }
 
 // ---------------------------------------------------------------------------
-// Host side
+// Host: hierarchy construction in FP64
 // ---------------------------------------------------------------------------
-static inline dim3 g1(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
-static inline Dims dims_of(const Level<float> &L) { return Dims{L.nx, L.ny, L.nz, L.n1}; }
-template <typename T>
-static inline Dims dims_of(const Level<T> &L) { return Dims{L.nx, L.ny, L.nz, L.n1}; }
+namespace {
 
-template <typename T> struct FineCoef;
-template <> struct FineCoef<float> {
-    static const float *im(const Fine &f, int a) { return f.c32.im[a]; }
-    static const float *sg(const Fine &f, int a) { return f.c32.sg[a]; }
-};
-template <> struct FineCoef<double> {
-    static const double *im(const Fine &f, int a) { return f.c64.im[a]; }
-    static const double *sg(const Fine &f, int a) { return f.c64.sg[a]; }
-};
-
-template <typename T>
-static FineA<T> fineA(const Fine &f) {
-    FineA<T> p;
-    p.g = f.g;
-    for (int a = 0; a < 3; ++a) { p.im[a] = FineCoef<T>::im(f, a); p.sg[a] = FineCoef<T>::sg(f, a); }
-    p.inv_dt = f.dt > 0 ? (T)(1.0 / f.dt) : (T)0;
-    return p;
-}
-template <typename T>
-static FineV<T> fineV(const Fine &f) {
-    FineV<T> p;
-    p.g = f.g;
-    for (int a = 0; a < 3; ++a) p.sg[a] = FineCoef<T>::sg(f, a);
-    return p;
-}
-template <typename T, int NC>
-static Expl<T, NC> expl(const Level<T> &L) {
-    return Expl<T, NC>{dims_of(L), L.cond, L.exc, L.cross, L.dinv};
-}
-
-// Launch helpers dispatching on (level, block).
-template <typename T>
-static void smooth(const Fine &f, Hierarchy<T> &h, int l, const T *x, const T *b, T *xo, T w, int first,
-                   hipStream_t s) {
-    const Level<T> &L = h.lev[l];
-    if (l == 0) {
-        if (h.block == 0) hipLaunchKernelGGL((k_smooth<T, FineA<T>>), g1(L.n1), dim3(256), 0, s, fineA<T>(f), x, b, xo, w, first);
-        else hipLaunchKernelGGL((k_smooth<T, FineV<T>>), g1(L.n1), dim3(256), 0, s, fineV<T>(f), x, b, xo, w, first);
-    } else {
-        if (h.block == 0) hipLaunchKernelGGL((k_smooth<T, Expl<T, 3>>), g1(L.n1), dim3(256), 0, s, expl<T, 3>(L), x, b, xo, w, first);
-        else hipLaunchKernelGGL((k_smooth<T, Expl<T, 1>>), g1(L.n1), dim3(256), 0, s, expl<T, 1>(L), x, b, xo, w, first);
-    }
-}
-template <typename T>
-static void resid(const Fine &f, Hierarchy<T> &h, int l, const T *x, const T *b, T *r, hipStream_t s) {
-    const Level<T> &L = h.lev[l];
-    if (l == 0) {
-        if (h.block == 0) hipLaunchKernelGGL((k_resid<T, FineA<T>>), g1(L.n1), dim3(256), 0, s, fineA<T>(f), x, b, r);
-        else hipLaunchKernelGGL((k_resid<T, FineV<T>>), g1(L.n1), dim3(256), 0, s, fineV<T>(f), x, b, r);
-    } else {
-        if (h.block == 0) hipLaunchKernelGGL((k_resid<T, Expl<T, 3>>), g1(L.n1), dim3(256), 0, s, expl<T, 3>(L), x, b, r);
-        else hipLaunchKernelGGL((k_resid<T, Expl<T, 1>>), g1(L.n1), dim3(256), 0, s, expl<T, 1>(L), x, b, r);
-    }
-}
-template <typename T>
-static void prolong(const Fine &f, Hierarchy<T> &h, int l, const T *xc, T *x, T alpha, hipStream_t s) {
-    const Level<T> &L = h.lev[l];
-    const Dims cd = dims_of(h.lev[l + 1]);
-    if (l == 0) {
-        if (h.block == 0) hipLaunchKernelGGL((k_prolong<T, FineA<T>>), g1(L.n1), dim3(256), 0, s, fineA<T>(f), cd, xc, x, alpha);
-        else hipLaunchKernelGGL((k_prolong<T, FineV<T>>), g1(L.n1), dim3(256), 0, s, fineV<T>(f), cd, xc, x, alpha);
-    } else {
-        if (h.block == 0) hipLaunchKernelGGL((k_prolong<T, Expl<T, 3>>), g1(L.n1), dim3(256), 0, s, expl<T, 3>(L), cd, xc, x, alpha);
-        else hipLaunchKernelGGL((k_prolong<T, Expl<T, 1>>), g1(L.n1), dim3(256), 0, s, expl<T, 1>(L), cd, xc, x, alpha);
-    }
-}
-
-template <typename T>
-void mg_galerkin_from(const Fine &f, Hierarchy<T> &h, int l, hipStream_t s) {
-    Level<T> &C = h.lev[l + 1];
-    const Dims cd = dims_of(C);
-    if (l == 0) {
-        if (h.block == 0) hipLaunchKernelGGL((k_galerkin<T, FineA<T>>), g1(C.n1), dim3(256), 0, s, fineA<T>(f), cd, C.cond, C.exc, C.cross, C.dinv);
-        else hipLaunchKernelGGL((k_galerkin<T, FineV<T>>), g1(C.n1), dim3(256), 0, s, fineV<T>(f), cd, C.cond, C.exc, C.cross, C.dinv);
-    } else {
-        const Level<T> &L = h.lev[l];
-        if (h.block == 0) hipLaunchKernelGGL((k_galerkin<T, Expl<T, 3>>), g1(C.n1), dim3(256), 0, s, expl<T, 3>(L), cd, C.cond, C.exc, C.cross, C.dinv);
-        else hipLaunchKernelGGL((k_galerkin<T, Expl<T, 1>>), g1(C.n1), dim3(256), 0, s, expl<T, 1>(L), cd, C.cond, C.exc, C.cross, C.dinv);
-    }
-}
-
-static void hip_check(hipError_t e, const char *what) {
+void hip_check(hipError_t e, const char *what) {
     if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
 }
 
-// Dense FP64 inverse by Gauss-Jordan with partial pivoting (host, n <= ~2000).
-static void dense_inverse(std::vector<double> &A, int n, std::vector<double> &inv) {
+// One explicit level in FP64 on the host.
+struct HostLevel {
+    LevelDims d;
+    int nc;
+    std::vector<double> cond, exc, cross, dinv;   // [nc][6][n1], [nc][n1], [nc][8][n1], [nc][n1]
+};
+
+template <int C, int NC>
+bool host_fine_row(const Grid &g, const HostCoef &hc, const FineConst<double> &fk, int i, int j, int k,
+                   Row<double> &r) {
+    if (!free_f(g, NC == 1 ? FV : C, i, j, k)) return false;
+    const CoefAt<double> ca{{hc.im[0], hc.im[1], hc.im[2]}, {hc.sg[0], hc.sg[1], hc.sg[2]}, gidx(g, i, j, k), g.nx,
+                            g.nx * g.ny};
+    if (NC == 3) row_a_raw<C, double>(ca, fk, r);
+    else row_v_raw<double>(ca, fk, r);
+    eliminate<C, NC, double>(g, i, j, k, r);
+    return true;
+}
+
+bool host_expl_row(const HostLevel &L, int c, int i, int j, int k, Row<double> &r) {
+    const int64_t p = lidx(L.d, i, j, k), n1 = L.d.n1;
+    if (L.dinv[(size_t)(c * n1 + p)] == 0.0) return false;
+    for (int q = 0; q < 6; ++q) r.cond[q] = L.cond[(size_t)((c * 6 + q) * n1 + p)];
+    r.exc = L.exc[(size_t)(c * n1 + p)];
+    for (int s = 0; s < 8; ++s) r.cross[s] = L.nc == 3 ? L.cross[(size_t)((c * 8 + s) * n1 + p)] : 0.0;
+    return true;
+}
+
+// Galerkin product of one level with 2x2x2 aggregation.  `row(c,i,j,k,r)`
+// gives the fine row of component c at (i,j,k) (false if inactive).
+template <typename RowFn>
+HostLevel galerkin(const LevelDims &fd, int nc, RowFn row) {
+    HostLevel C;
+    C.nc = nc;
+    C.d.nx = (fd.nx + 1) / 2;
+    C.d.ny = (fd.ny + 1) / 2;
+    C.d.nz = (fd.nz + 1) / 2;
+    C.d.n1 = (int64_t)C.d.nx * C.d.ny * C.d.nz;
+    const int64_t n1 = C.d.n1;
+    C.cond.assign((size_t)(nc * 6 * n1), 0.0);
+    C.exc.assign((size_t)(nc * n1), 0.0);
+    C.dinv.assign((size_t)(nc * n1), 0.0);
+    if (nc == 3) C.cross.assign((size_t)(nc * 8 * n1), 0.0);
+    for (int K = 0; K < C.d.nz; ++K)
+        for (int J = 0; J < C.d.ny; ++J)
+            for (int I = 0; I < C.d.nx; ++I) {
+                const int64_t P = lidx(C.d, I, J, K);
+                for (int c = 0; c < nc; ++c) {
+                    double cc[6] = {0, 0, 0, 0, 0, 0}, ce = 0.0, cx[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+                    bool any = false;
+                    for (int ch = 0; ch < 8; ++ch) {
+                        const int i = 2 * I + (ch & 1), j = 2 * J + ((ch >> 1) & 1), k = 2 * K + (ch >> 2);
+                        if (!inside(fd, i, j, k)) continue;
+                        Row<double> r;
+                        if (!row(c, i, j, k, r)) continue;
+                        any = true;
+                        ce += r.exc;
+                        for (int q = 0; q < 6; ++q) {
+                            const int ni = i + so_i(q), nj = j + so_j(q), nk = k + so_k(q);
+                            // a link inside the aggregate cancels in P^T A P
+                            if ((ni >> 1) != I || (nj >> 1) != J || (nk >> 1) != K) cc[q] += r.cond[q];
+                        }
+                        if (nc == 3) {
+                            for (int s = 0; s < 8; ++s) {
+                                if (r.cross[s] == 0.0) continue;
+                                const int oi = ((i + xo_i(c, s)) >> 1) - I, oj = ((j + xo_j(c, s)) >> 1) - J,
+                                          ok = ((k + xo_k(c, s)) >> 1) - K;
+                                const int base = s & 4;
+                                int hit = -1;
+                                for (int s2 = 0; s2 < 4; ++s2)
+                                    if (xo_i(c, base + s2) == oi && xo_j(c, base + s2) == oj &&
+                                        xo_k(c, base + s2) == ok)
+                                        hit = base + s2;
+                                if (hit < 0) throw std::logic_error("galerkin: coarse cross offset outside the pattern");
+                                cx[hit] += r.cross[s];
+                            }
+                        }
+                    }
+                    if (!any) continue;
+                    double diag = ce;
+                    for (int q = 0; q < 6; ++q) {
+                        C.cond[(size_t)((c * 6 + q) * n1 + P)] = cc[q];
+                        diag += cc[q];
+                    }
+                    C.exc[(size_t)(c * n1 + P)] = ce;
+                    if (nc == 3)
+                        for (int s = 0; s < 8; ++s) C.cross[(size_t)((c * 8 + s) * n1 + P)] = cx[s];
+                    if (!(diag > 0.0)) throw std::runtime_error("galerkin: coarse row without a positive diagonal");
+                    C.dinv[(size_t)(c * n1 + P)] = 1.0 / diag;
+                }
+            }
+    return C;
+}
+
+// Dense FP64 inverse by Gauss-Jordan with partial pivoting.
+void dense_inverse(std::vector<double> &A, int n, std::vector<double> &inv) {
     inv.assign((size_t)n * n, 0.0);
     for (int i = 0; i < n; ++i) inv[(size_t)i * n + i] = 1.0;
     for (int col = 0; col < n; ++col) {
@@ -457,14 +490,13 @@ static void dense_inverse(std::vector<double> &A, int n, std::vector<double> &in
             if (v > best) { best = v; piv = r; }
         }
         if (best == 0.0) throw std::runtime_error("singular coarsest-level operator");
-        if (piv != col) {
+        if (piv != col)
             for (int c = 0; c < n; ++c) {
                 std::swap(A[(size_t)col * n + c], A[(size_t)piv * n + c]);
                 std::swap(inv[(size_t)col * n + c], inv[(size_t)piv * n + c]);
             }
-        }
-        const double d = 1.0 / A[(size_t)col * n + col];
-        for (int c = 0; c < n; ++c) { A[(size_t)col * n + c] *= d; inv[(size_t)col * n + c] *= d; }
+        const double dd = 1.0 / A[(size_t)col * n + col];
+        for (int c = 0; c < n; ++c) { A[(size_t)col * n + c] *= dd; inv[(size_t)col * n + c] *= dd; }
         for (int r = 0; r < n; ++r) {
             if (r == col) continue;
             const double m = A[(size_t)r * n + col];
@@ -477,149 +509,266 @@ static void dense_inverse(std::vector<double> &A, int n, std::vector<double> &in
     }
 }
 
-// Build levels 1..nlev-1 and the coarsest dense inverse.  Level storage must
-// already be allocated by the caller (sizes from the level dims).
 template <typename T>
-void mg_build(const Fine &f, Hierarchy<T> &h, hipStream_t s) {
-    for (int l = 0; l + 1 < h.nlev; ++l) mg_galerkin_from<T>(f, h, l, s);
-    hip_check(hipStreamSynchronize(s), "galerkin");
-    // coarsest level to host
-    const Level<T> &C = h.lev[h.nlev - 1];
-    const int nc = C.ncomp;
-    const int64_t n1 = C.n1;
-    std::vector<T> cond((size_t)nc * 6 * n1), exc((size_t)nc * n1), dinv((size_t)nc * n1), cross;
-    // All copies are ordered on the handle's stream s (never the legacy null
-    // stream, which a non-blocking stream does not wait for).
-    hip_check(hipMemcpyAsync(cond.data(), C.cond, cond.size() * sizeof(T), hipMemcpyDeviceToHost, s), "d2h");
-    hip_check(hipMemcpyAsync(exc.data(), C.exc, exc.size() * sizeof(T), hipMemcpyDeviceToHost, s), "d2h");
-    hip_check(hipMemcpyAsync(dinv.data(), C.dinv, dinv.size() * sizeof(T), hipMemcpyDeviceToHost, s), "d2h");
-    if (nc == 3) {
-        cross.resize((size_t)nc * 8 * n1);
-        hip_check(hipMemcpyAsync(cross.data(), C.cross, cross.size() * sizeof(T), hipMemcpyDeviceToHost, s), "d2h");
+T *upload(const std::vector<double> &src, std::vector<T> &stage, const MgAlloc &al) {
+    stage.resize(src.size());
+    for (size_t e = 0; e < src.size(); ++e) stage[e] = (T)src[e];
+    T *p = (T *)al.alloc(sizeof(T) * (stage.size() ? stage.size() : 1));
+    if (!stage.empty()) hip_check(hipMemcpyAsync(p, stage.data(), sizeof(T) * stage.size(), hipMemcpyHostToDevice,
+                                                 al.stream), "h2d");
+    hip_check(hipStreamSynchronize(al.stream), "sync");
+    return p;
+}
+
+}  // namespace
+
+FineConst<double> fine_const(const Grid &g, double dt) {
+    FineConst<double> k;
+    const double h[3] = {g.hx, g.hy, g.hz};
+    for (int a = 0; a < 3; ++a) k.ih2[a] = 1.0 / (h[a] * h[a]);
+    k.ixy = 1.0 / (h[0] * h[1]);
+    k.ixz = 1.0 / (h[0] * h[2]);
+    k.iyz = 1.0 / (h[1] * h[2]);
+    k.idt = dt > 0 ? 1.0 / dt : 0.0;
+    k.sigfloor = QMS_SIGFLOOR;
+    return k;
+}
+
+template <typename T>
+void mg_setup(const Grid &g, const HostCoef &hc, const CoeffsT<T> &dc, double dt, int block, const MgAlloc &al,
+              Hierarchy<T> &h) {
+    h = Hierarchy<T>();
+    h.block = block;
+    const int nc = block == 0 ? 3 : 1;
+    const FineConst<double> fk = fine_const(g, dt);
+    const FineConst<double> fkd = fk;
+    h.fk.ih2[0] = (T)fkd.ih2[0]; h.fk.ih2[1] = (T)fkd.ih2[1]; h.fk.ih2[2] = (T)fkd.ih2[2];
+    h.fk.ixy = (T)fkd.ixy; h.fk.ixz = (T)fkd.ixz; h.fk.iyz = (T)fkd.iyz;
+    h.fk.idt = (T)fkd.idt; h.fk.sigfloor = (T)fkd.sigfloor;
+    h.g = g;
+    for (int a = 0; a < 3; ++a) { h.im[a] = dc.im[a]; h.sg[a] = dc.sg[a]; }
+    // level 0 work vectors
+    LevelDims d0{g.nx, g.ny, g.nz, g.n1};
+    h.lev[0].d = d0;
+    h.lev[0].ncomp = nc;
+    h.lev[0].t = (T *)al.alloc(sizeof(T) * (size_t)(nc * g.n1));
+    h.lev[0].r = (T *)al.alloc(sizeof(T) * (size_t)(nc * g.n1));
+    // coarse levels
+    std::vector<HostLevel> hl;
+    auto fine_row = [&](int c, int i, int j, int k, Row<double> &r) {
+        if (nc == 1) return host_fine_row<0, 1>(g, hc, fk, i, j, k, r);
+        if (c == 0) return host_fine_row<0, 3>(g, hc, fk, i, j, k, r);
+        if (c == 1) return host_fine_row<1, 3>(g, hc, fk, i, j, k, r);
+        return host_fine_row<2, 3>(g, hc, fk, i, j, k, r);
+    };
+    LevelDims cur = d0;
+    int nlev = 1;
+    while (nlev < MG_MAX_LEVELS) {
+        const int mx = std::max(cur.nx, std::max(cur.ny, cur.nz));
+        if (mx <= 4 && nlev >= 2) break;
+        HostLevel C = nlev == 1 ? galerkin(cur, nc, fine_row)
+                                : galerkin(cur, nc, [&](int c, int i, int j, int k, Row<double> &r) {
+                                      return host_expl_row(hl.back(), c, i, j, k, r);
+                                  });
+        cur = C.d;
+        hl.push_back(std::move(C));
+        ++nlev;
     }
-    hip_check(hipStreamSynchronize(s), "d2h sync");
-    std::vector<int> map((size_t)nc * n1, -1), idx;
+    h.nlev = nlev;
+    std::vector<T> stage;
+    for (int l = 1; l < nlev; ++l) {
+        const HostLevel &H = hl[(size_t)(l - 1)];
+        MgLevel<T> &L = h.lev[l];
+        L.d = H.d;
+        L.ncomp = nc;
+        L.cond = upload<T>(H.cond, stage, al);
+        L.exc = upload<T>(H.exc, stage, al);
+        L.dinv = upload<T>(H.dinv, stage, al);
+        L.cross = nc == 3 ? upload<T>(H.cross, stage, al) : nullptr;
+        L.x = (T *)al.alloc(sizeof(T) * (size_t)(nc * H.d.n1));
+        L.b = (T *)al.alloc(sizeof(T) * (size_t)(nc * H.d.n1));
+        L.r = (T *)al.alloc(sizeof(T) * (size_t)(nc * H.d.n1));
+        L.t = (T *)al.alloc(sizeof(T) * (size_t)(nc * H.d.n1));
+    }
+    // coarsest level: dense FP64 inverse over its active unknowns
+    const HostLevel &H = hl.back();
+    const int64_t n1 = H.d.n1;
+    std::vector<int> map((size_t)(nc * n1), -1), idx;
     for (int64_t q = 0; q < nc * n1; ++q)
-        if (dinv[q] != (T)0) { map[q] = (int)idx.size(); idx.push_back((int)q); }
+        if (H.dinv[(size_t)q] != 0.0) { map[(size_t)q] = (int)idx.size(); idx.push_back((int)q); }
     const int n = (int)idx.size();
     std::vector<double> A((size_t)n * n, 0.0), inv;
-    static const int SO[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
-    static const int XO[3][8][4] = {
-        {{1, 0, -1, 0}, {1, 1, -1, 0}, {1, 0, 0, 0}, {1, 1, 0, 0}, {2, 0, 0, -1}, {2, 1, 0, -1}, {2, 0, 0, 0}, {2, 1, 0, 0}},
-        {{0, -1, 0, 0}, {0, 0, 0, 0}, {0, -1, 1, 0}, {0, 0, 1, 0}, {2, 0, 0, -1}, {2, 0, 1, -1}, {2, 0, 0, 0}, {2, 0, 1, 0}},
-        {{0, -1, 0, 0}, {0, 0, 0, 0}, {0, -1, 0, 1}, {0, 0, 0, 1}, {1, 0, -1, 0}, {1, 0, 0, 0}, {1, 0, -1, 1}, {1, 0, 0, 1}}};
     for (int r = 0; r < n; ++r) {
-        const int q = idx[r];
+        const int q = idx[(size_t)r];
         const int c = (int)(q / n1);
         const int64_t p = q % n1;
-        const int i = (int)(p % C.nx), j = (int)((p / C.nx) % C.ny), k = (int)(p / ((int64_t)C.nx * C.ny));
-        double diag = (double)exc[q];
-        for (int d = 0; d < 6; ++d) {
-            const double cv = (double)cond[((size_t)c * 6 + d) * n1 + p];
-            if (cv == 0.0) continue;
-            diag += cv;
-            const int ni = i + SO[d][0], nj = j + SO[d][1], nk = k + SO[d][2];
-            const int64_t nq = (int64_t)c * n1 + ni + (int64_t)C.nx * (nj + (int64_t)C.ny * nk);
-            if (map[nq] >= 0) A[(size_t)r * n + map[nq]] -= cv;
+        const int i = (int)(p % H.d.nx), j = (int)((p / H.d.nx) % H.d.ny), k = (int)(p / ((int64_t)H.d.nx * H.d.ny));
+        Row<double> rw;
+        host_expl_row(H, c, i, j, k, rw);
+        double diag = rw.exc;
+        for (int q2 = 0; q2 < 6; ++q2) {
+            if (rw.cond[q2] == 0.0) continue;
+            diag += rw.cond[q2];
+            const int ni = i + so_i(q2), nj = j + so_j(q2), nk = k + so_k(q2);
+            if (!inside(H.d, ni, nj, nk)) continue;
+            const int m = map[(size_t)(c * n1 + lidx(H.d, ni, nj, nk))];
+            if (m >= 0) A[(size_t)r * n + m] -= rw.cond[q2];
         }
         A[(size_t)r * n + r] += diag;
-        if (nc == 3) {
-            for (int sI = 0; sI < 8; ++sI) {
-                const double av = (double)cross[((size_t)c * 8 + sI) * n1 + p];
-                if (av == 0.0) continue;
-                const int tc = XO[c][sI][0];
-                const int ni = i + XO[c][sI][1], nj = j + XO[c][sI][2], nk = k + XO[c][sI][3];
-                if (ni < 0 || nj < 0 || nk < 0 || ni >= C.nx || nj >= C.ny || nk >= C.nz) continue;
-                const int64_t nq = (int64_t)tc * n1 + ni + (int64_t)C.nx * (nj + (int64_t)C.ny * nk);
-                if (map[nq] >= 0) A[(size_t)r * n + map[nq]] += av;
+        if (nc == 3)
+            for (int s = 0; s < 8; ++s) {
+                if (rw.cross[s] == 0.0) continue;
+                const int ni = i + xo_i(c, s), nj = j + xo_j(c, s), nk = k + xo_k(c, s);
+                if (!inside(H.d, ni, nj, nk)) continue;
+                const int m = map[(size_t)(xo_c(c, s) * n1 + lidx(H.d, ni, nj, nk))];
+                if (m >= 0) A[(size_t)r * n + m] += rw.cross[s];
             }
-        }
     }
     if (n > 0) dense_inverse(A, n, inv);
     h.ncoarse = n;
-    hip_check(hipMalloc(&h.coarse_idx, sizeof(int) * (size_t)(n > 0 ? n : 1)), "malloc");
-    hip_check(hipMalloc(&h.coarse_inv, sizeof(double) * (size_t)(n > 0 ? (size_t)n * n : 1)), "malloc");
+    h.coarse_idx = (int *)al.alloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    h.coarse_inv = (double *)al.alloc(sizeof(double) * (size_t)(n > 0 ? (size_t)n * n : 1));
     if (n > 0) {
-        // Uploaded on s, the stream k_coarse_solve runs on, and waited for
-        // before the host vectors go out of scope.  A legacy-stream hipMemcpy
-        // from pageable memory may return before the data reaches the device
-        // and is not ordered with work on a non-blocking stream.
-        hip_check(hipMemcpyAsync(h.coarse_idx, idx.data(), sizeof(int) * n, hipMemcpyHostToDevice, s), "h2d");
-        hip_check(hipMemcpyAsync(h.coarse_inv, inv.data(), sizeof(double) * (size_t)n * n, hipMemcpyHostToDevice, s),
-                  "h2d");
-        hip_check(hipStreamSynchronize(s), "h2d sync");
+        hip_check(hipMemcpyAsync(h.coarse_idx, idx.data(), sizeof(int) * n, hipMemcpyHostToDevice, al.stream), "h2d");
+        hip_check(hipMemcpyAsync(h.coarse_inv, inv.data(), sizeof(double) * (size_t)n * n, hipMemcpyHostToDevice,
+                                 al.stream), "h2d");
+        hip_check(hipStreamSynchronize(al.stream), "sync");
     }
 }
 
-// One V-cycle: x0 <- MG(b0).  x0/b0 hold [ncomp][n1] of level 0.
+// ---------------------------------------------------------------------------
+// Host: V-cycle launches
+// ---------------------------------------------------------------------------
+static inline dim3 g1(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+template <typename T, int NC>
+static FineProv<T, NC> fine_prov(const Hierarchy<T> &h) {
+    FineProv<T, NC> p;
+    p.g = h.g;
+    for (int a = 0; a < 3; ++a) { p.im[a] = h.im[a]; p.sg[a] = h.sg[a]; }
+    p.fk = h.fk;
+    return p;
+}
+template <typename T, int NC>
+static ExplProv<T, NC> expl_prov(const MgLevel<T> &L) {
+    return ExplProv<T, NC>{L.d, L.cond, L.exc, L.cross, L.dinv};
+}
+
+// Dispatch a kernel template on (level 0 or explicit, block).
+template <typename T, typename F>
+static void on_level(const Hierarchy<T> &h, int l, F f) {
+    if (l == 0) {
+        if (h.block == 0) f(fine_prov<T, 3>(h), std::integral_constant<int, 3>());
+        else f(fine_prov<T, 1>(h), std::integral_constant<int, 1>());
+    } else {
+        if (h.block == 0) f(expl_prov<T, 3>(h.lev[l]), std::integral_constant<int, 3>());
+        else f(expl_prov<T, 1>(h.lev[l]), std::integral_constant<int, 1>());
+    }
+}
+
 template <typename T>
-static void vcycle_rec(const Fine &f, Hierarchy<T> &h, const MGParams &mp, int l, T *x, const T *b, hipStream_t s) {
-    Level<T> &L = h.lev[l];
-    const int64_t nv = (int64_t)L.ncomp * L.n1;
+static void smooth(const Hierarchy<T> &h, int l, const T *x, const T *b, T *xo, T w, bool first, hipStream_t s) {
+    const int64_t n1 = h.lev[l].d.n1;
+    on_level(h, l, [&](auto pv, auto nc) {
+        using P = decltype(pv);
+        constexpr int NC = decltype(nc)::value;
+        if (first) hipLaunchKernelGGL((k_mg_smooth<T, P, NC, true>), g1(n1), dim3(256), 0, s, pv, x, b, xo, w);
+        else hipLaunchKernelGGL((k_mg_smooth<T, P, NC, false>), g1(n1), dim3(256), 0, s, pv, x, b, xo, w);
+    });
+}
+template <typename T>
+static void resid(const Hierarchy<T> &h, int l, const T *x, const T *b, T *r, hipStream_t s) {
+    const int64_t n1 = h.lev[l].d.n1;
+    on_level(h, l, [&](auto pv, auto nc) {
+        using P = decltype(pv);
+        constexpr int NC = decltype(nc)::value;
+        hipLaunchKernelGGL((k_mg_resid<T, P, NC>), g1(n1), dim3(256), 0, s, pv, x, b, r);
+    });
+}
+template <typename T>
+static void prolong(const Hierarchy<T> &h, int l, const T *xc, T *x, T alpha, hipStream_t s) {
+    const int64_t n1 = h.lev[l].d.n1;
+    const LevelDims cd = h.lev[l + 1].d;
+    on_level(h, l, [&](auto pv, auto nc) {
+        using P = decltype(pv);
+        constexpr int NC = decltype(nc)::value;
+        hipLaunchKernelGGL((k_mg_prolong<T, P, NC>), g1(n1), dim3(256), 0, s, pv, cd, xc, x, alpha);
+    });
+}
+
+template <typename T>
+static void vcycle_rec(Hierarchy<T> &h, const MGParams &mp, int l, T *x, const T *b, hipStream_t s) {
+    MgLevel<T> &L = h.lev[l];
+    const int nc = L.ncomp;
+    const int64_t nv = (int64_t)nc * L.d.n1;
     if (l == h.nlev - 1) {
-        hipLaunchKernelGGL(k_zero<T>, g1(nv), dim3(256), 0, s, x, nv);
+        hipLaunchKernelGGL(k_fill<T>, g1(nv), dim3(256), 0, s, x, nv, (T)0);
         if (h.ncoarse > 0)
-            hipLaunchKernelGGL(k_coarse_solve<T>, dim3(1), dim3(256), 0, s, h.ncoarse, h.coarse_idx, h.coarse_inv, b, x);
+            hipLaunchKernelGGL(k_mg_coarse<T>, dim3(1), dim3(256), 0, s, h.ncoarse, (const int *)h.coarse_idx,
+                               (const double *)h.coarse_inv, b, x);
         return;
     }
     const T w = (T)mp.omega;
-    // pre-smoothing, ping-pong between x and L.x (scratch)
-    T *cur = x, *oth = L.t;
-    // arrange so that the last pre-smoothing sweep lands in x
-    if ((mp.nu % 2) == 0) { cur = L.t; oth = x; }
-    smooth<T>(f, h, l, nullptr, b, cur, w, 1, s);
+    // pre-smoothing from zero, ping-pong between x and L.t, ending in x
+    T *cur = (mp.nu % 2) ? x : L.t, *oth = (mp.nu % 2) ? L.t : x;
+    smooth<T>(h, l, nullptr, b, cur, w, true, s);
     for (int it = 1; it < mp.nu; ++it) {
-        smooth<T>(f, h, l, cur, b, oth, w, 0, s);
-        T *t = cur; cur = oth; oth = t;
+        smooth<T>(h, l, cur, b, oth, w, false, s);
+        std::swap(cur, oth);
     }
-    // cur == x here
-    resid<T>(f, h, l, x, b, L.r, s);
-    Level<T> &C = h.lev[l + 1];
-    hipLaunchKernelGGL(k_restrict<T>, g1(C.n1), dim3(256), 0, s, dims_of(L), dims_of(C), L.ncomp,
-                       (const T *)L.r, C.b);
-    vcycle_rec<T>(f, h, mp, l + 1, C.x, C.b, s);
-    prolong<T>(f, h, l, C.x, x, (T)mp.alpha, s);
-    cur = x; oth = L.t;
+    resid<T>(h, l, x, b, L.r, s);
+    MgLevel<T> &C = h.lev[l + 1];
+    hipLaunchKernelGGL(k_mg_restrict<T>, g1(C.d.n1), dim3(256), 0, s, L.d, C.d, nc, (const T *)L.r, C.b);
+    vcycle_rec<T>(h, mp, l + 1, C.x, C.b, s);
+    prolong<T>(h, l, C.x, x, (T)mp.alpha, s);
+    cur = x;
+    oth = L.t;
     for (int it = 0; it < mp.nu; ++it) {
-        smooth<T>(f, h, l, cur, b, oth, w, 0, s);
-        T *t = cur; cur = oth; oth = t;
+        smooth<T>(h, l, cur, b, oth, w, false, s);
+        std::swap(cur, oth);
     }
-    if (cur != x) hipMemcpyAsync(x, cur, sizeof(T) * nv, hipMemcpyDeviceToDevice, s);
+    if (cur != x) hipLaunchKernelGGL(k_copy_t<T>, g1(nv), dim3(256), 0, s, x, (const T *)cur, nv);
 }
 
 template <typename T>
-void mg_vcycle(const Fine &f, Hierarchy<T> &h, const MGParams &mp, T *x0, const T *b0, hipStream_t s) {
-    vcycle_rec<T>(f, h, mp, 0, x0, b0, s);
+void mg_vcycle(Hierarchy<T> &h, const MGParams &mp, T *x0, const T *b0, hipStream_t s) {
+    vcycle_rec<T>(h, mp, 0, x0, b0, s);
 }
 
 template <typename T>
-void mg_fine_sweep(const Fine &f, Hierarchy<T> &h, const MGParams &mp, const T *x, const T *b, T *xo, hipStream_t s) {
-    smooth<T>(f, h, 0, x, b, xo, (T)mp.omega, 0, s);
+void mg_fine_sweep(Hierarchy<T> &h, const MGParams &mp, const T *x, const T *b, T *xo, hipStream_t s) {
+    smooth<T>(h, 0, x, b, xo, (T)mp.omega, false, s);
 }
 
-// explicit instantiations
-template void mg_fine_sweep<float>(const Fine &, Hierarchy<float> &, const MGParams &, const float *, const float *, float *, hipStream_t);
-template void mg_fine_sweep<double>(const Fine &, Hierarchy<double> &, const MGParams &, const double *, const double *, double *, hipStream_t);
-template void mg_build<float>(const Fine &, Hierarchy<float> &, hipStream_t);
-template void mg_build<double>(const Fine &, Hierarchy<double> &, hipStream_t);
-template void mg_vcycle<float>(const Fine &, Hierarchy<float> &, const MGParams &, float *, const float *, hipStream_t);
-template void mg_vcycle<double>(const Fine &, Hierarchy<double> &, const MGParams &, double *, const double *, hipStream_t);
+template <typename T>
+void launch_fill(T *x, int64_t n, T v, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill<T>, g1(n), dim3(256), 0, s, x, n, v);
+}
+
+template void mg_setup<float>(const Grid &, const HostCoef &, const CoeffsT<float> &, double, int, const MgAlloc &,
+                              Hierarchy<float> &);
+template void mg_setup<double>(const Grid &, const HostCoef &, const CoeffsT<double> &, double, int, const MgAlloc &,
+                               Hierarchy<double> &);
+template void mg_vcycle<float>(Hierarchy<float> &, const MGParams &, float *, const float *, hipStream_t);
+template void mg_vcycle<double>(Hierarchy<double> &, const MGParams &, double *, const double *, hipStream_t);
+template void mg_fine_sweep<float>(Hierarchy<float> &, const MGParams &, const float *, const float *, float *,
+                                   hipStream_t);
+template void mg_fine_sweep<double>(Hierarchy<double> &, const MGParams &, const double *, const double *, double *,
+                                    hipStream_t);
+template void launch_fill<float>(float *, int64_t, float, hipStream_t);
+template void launch_fill<double>(double *, int64_t, double, hipStream_t);
 
 // ---------------------------------------------------------------------------
-// Vector conversion kernels used at the Krylov/MG boundary.
+// Vector conversions at the Krylov / multigrid boundary
 // ---------------------------------------------------------------------------
 template <typename TD>
-__global__ void k_d2t(const double *__restrict__ src, const double *__restrict__ scale, TD *__restrict__ dst, int64_t n) {
+__global__ void __launch_bounds__(256) k_d2t(const double *__restrict__ src, const double *__restrict__ scale,
+                                             TD *__restrict__ dst, int64_t n) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < n) dst[p] = (TD)(scale ? src[p] * scale[p] : src[p]);
-}
-template <typename TS>
-__global__ void k_t2d(const TS *__restrict__ src, double *__restrict__ dst, int64_t n) {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < n) dst[p] = (double)src[p];
+    if (p < n) dst[p] = (TD)(src[p] * scale[p]);
 }
 template <typename TT>
-__global__ void k_sub(TT *__restrict__ a, const TT *__restrict__ b, int64_t n) {
+__global__ void __launch_bounds__(256) k_sub(TT *__restrict__ a, const TT *__restrict__ b, int64_t n) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p < n) a[p] -= b[p];
 }
@@ -628,12 +777,6 @@ void launch_d2t(const double *src, const double *scale, float *dst, int64_t n, h
 }
 void launch_d2t(const double *src, const double *scale, double *dst, int64_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_d2t<double>, g1(n), dim3(256), 0, s, src, scale, dst, n);
-}
-void launch_t2d(const float *src, double *dst, int64_t n, hipStream_t s) {
-    hipLaunchKernelGGL(k_t2d<float>, g1(n), dim3(256), 0, s, src, dst, n);
-}
-void launch_t2d(const double *src, double *dst, int64_t n, hipStream_t s) {
-    hipLaunchKernelGGL(k_t2d<double>, g1(n), dim3(256), 0, s, src, dst, n);
 }
 void launch_sub(float *a, const float *b, int64_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_sub<float>, g1(n), dim3(256), 0, s, a, b, n);

@@ -994,6 +994,9 @@ void launch_resid(const Grid &g, const CoeffsD &c, const OpParams &op, const dou
 void launch_apply(const Grid &g, const CoeffsD &c, const OpParams &op, const double *x, double *y, hipStream_t s) {
     launch_operator<double, double, OP_APPLY>(g, c, op, x, nullptr, y, s);
 }
+void launch_apply_f32in(const Grid &g, const CoeffsD &c, const OpParams &op, const float *x, double *y, hipStream_t s) {
+    launch_operator<float, double, OP_APPLY>(g, c, op, x, nullptr, y, s);
+}
 void launch_va_f32(const Grid &g, const CoeffsD &c, const OpParams &op, const float *x, float *y, hipStream_t s) {
     launch_operator<float, float, OP_VA>(g, c, op, x, nullptr, y, s);
 }

@@ -64,6 +64,19 @@ double time_on(hipStream_t s, int reps, const std::function<void()> &f) {
 
 }  // namespace
 
+// Preconditioner state in the multigrid precision T: the hierarchies and the
+// FGMRES Z basis (z_j = P^-1 v_j) in T, the rest of FGMRES in FP64.
+template <typename T>
+struct PrecondState {
+    Hierarchy<T> hA[2];        // A block: steady (0) / transient (1) operator
+    Hierarchy<T> hV;           // V block (independent of dt)
+    bool builtA[2] = {false, false};
+    bool builtV = false;
+    T *rT = nullptr, *tT = nullptr;   // D v in T; the V rows' coupling term
+    std::vector<T *> Z;
+    T **dZ = nullptr;
+};
+
 struct qms_handle {
     qms_config cfg;
     Grid g;
@@ -78,6 +91,7 @@ struct qms_handle {
     Coeffs coef;
     CoeffsD c64;
     CoeffsF c32;
+    std::vector<double> hcoef;  // host copy of the six FP64 coefficient arrays (multigrid setup)
 
     int64_t n = 0;              // 4*n1
     uint16_t *cls = nullptr;    // coefficient class per node (null: not built)
@@ -86,84 +100,52 @@ struct qms_handle {
     bool use_cls = true;
     double *x = nullptr, *b = nullptr, *r = nullptr, *w = nullptr, *tmp = nullptr;
     double *Dst = nullptr, *Dtr = nullptr;   // row scaling, steady / transient
-    std::vector<double *> V, Z;
-    double **dV = nullptr, **dZ = nullptr;
+    std::vector<double *> V;
+    double **dV = nullptr;
     double *partial = nullptr, *dsmall = nullptr;
     double *hsmall = nullptr;   // pinned
+    // FGMRES bookkeeping on the device
+    double *Hd = nullptr, *csd = nullptr, *snd = nullptr, *gd = nullptr, *yd = nullptr, *std_ = nullptr;
     int nblk = 1;
     int zchunk = 4;             // planes per operator workgroup (set by default_shape)
     int colw = 32;              // operator column width (nodes)
+    int pred[2] = {0, 0};       // iterations of the last steady / transient solve (launch-ahead count)
 
-    // preconditioner work vectors (4*n1 in MG precision)
-    void *rT = nullptr, *zT = nullptr, *tT = nullptr;
-    Hierarchy<float> hA32[2], hV32;
-    Hierarchy<double> hA64[2], hV64;
-    bool built[2] = {false, false};
-    bool builtV = false;
+    PrecondState<float> p32;
+    PrecondState<double> p64;
     bool have_state = false;
 
     qms_stats st;
 
-    template <typename T>
-    T *dalloc(int64_t count) {
+    void *raw_alloc(size_t nb) {
         void *p = nullptr;
-        const size_t nb = sizeof(T) * (size_t)(count > 0 ? count : 1);
         check(hipMalloc(&p, nb), "hipMalloc");
         allocs.push_back(p);
         bytes += (int64_t)nb;
         check(hipMemsetAsync(p, 0, nb, s), "hipMemsetAsync");
-        return (T *)p;
+        return p;
+    }
+    template <typename T>
+    T *dalloc(int64_t count) {
+        return (T *)raw_alloc(sizeof(T) * (size_t)(count > 0 ? count : 1));
     }
     ~qms_handle() {
         if (s) hipStreamSynchronize(s);
         for (void *p : allocs) hipFree(p);
         if (hsmall) hipHostFree(hsmall);
-        for (auto *h : {&hA32[0], &hA32[1], &hV32}) {
-            if (h->nlev) { hipFree(h->coarse_idx); hipFree(h->coarse_inv); }
-        }
-        for (auto *h : {&hA64[0], &hA64[1], &hV64}) {
-            if (h->nlev) { hipFree(h->coarse_idx); hipFree(h->coarse_inv); }
-        }
         if (s) hipStreamDestroy(s);
     }
-
-    Fine fine(double dt) const {
-        Fine f;
-        f.g = g;
-        f.c32 = c32;
-        f.c64 = c64;
-        f.dt = dt;
-        return f;
+    MgAlloc mg_alloc() {
+        return MgAlloc{[this](size_t nb) { return raw_alloc(nb); }, s};
     }
-
-    template <typename T>
-    void alloc_hierarchy(Hierarchy<T> &h, int block) {
-        memset(&h, 0, sizeof(h));
-        h.block = block;
-        const int nc = block == 0 ? 3 : 1;
-        int nx = g.nx, ny = g.ny, nz = g.nz;
-        int l = 0;
-        while (true) {
-            Level<T> &L = h.lev[l];
-            L.nx = nx; L.ny = ny; L.nz = nz;
-            L.n1 = (int64_t)nx * ny * nz;
-            L.ncomp = nc;
-            L.t = dalloc<T>(nc * L.n1);
-            L.r = dalloc<T>(nc * L.n1);
-            if (l > 0) {
-                L.x = dalloc<T>(nc * L.n1);
-                L.b = dalloc<T>(nc * L.n1);
-                L.cond = dalloc<T>((int64_t)nc * 6 * L.n1);
-                L.exc = dalloc<T>(nc * L.n1);
-                L.dinv = dalloc<T>(nc * L.n1);
-                if (nc == 3) L.cross = dalloc<T>((int64_t)nc * 8 * L.n1);
-            }
-            ++l;
-            const int mx = std::max(nx, std::max(ny, nz));
-            if ((mx <= 4 && l >= 2) || l >= 15) break;
-            nx = (nx + 1) / 2; ny = (ny + 1) / 2; nz = (nz + 1) / 2;
+    HostCoef host_coef() const {
+        const int64_t n1 = g.n1;
+        HostCoef c;
+        for (int a = 0; a < 3; ++a) {
+            c.im[a] = hcoef.data() + (size_t)a * n1;
+            c.sg[a] = hcoef.data() + (size_t)(3 + a) * n1;
         }
-        h.nlev = l;
+        return c;
     }
 };
 
@@ -181,9 +163,11 @@ struct TupleHash {
     }
 };
 
+// Also keeps the host copy of the six arrays that the multigrid setup reads.
 void build_classes(qms_handle *h) {
     const int64_t n1 = h->g.n1;
-    std::vector<double> a((size_t)6 * n1);
+    std::vector<double> &a = h->hcoef;
+    a.assign((size_t)6 * n1, 0.0);
     for (int q = 0; q < 6; ++q)
         check(hipMemcpyAsync(a.data() + (size_t)q * n1, q < 3 ? h->coef.im[q] : h->coef.sg[q - 3],
                              sizeof(double) * n1, hipMemcpyDeviceToHost, h->s), "d2h");
@@ -249,51 +233,73 @@ MGParams mgp(const qms_handle *h) {
     return m;
 }
 
-template <typename T> Hierarchy<T> &hierA(qms_handle *h, int tr);
-template <> Hierarchy<float> &hierA<float>(qms_handle *h, int tr) { return h->hA32[tr]; }
-template <> Hierarchy<double> &hierA<double>(qms_handle *h, int tr) { return h->hA64[tr]; }
-template <typename T> Hierarchy<T> &hierV(qms_handle *h);
-template <> Hierarchy<float> &hierV<float>(qms_handle *h) { return h->hV32; }
-template <> Hierarchy<double> &hierV<double>(qms_handle *h) { return h->hV64; }
+template <typename T> PrecondState<T> &pstate(qms_handle *h);
+template <> PrecondState<float> &pstate<float>(qms_handle *h) { return h->p32; }
+template <> PrecondState<double> &pstate<double>(qms_handle *h) { return h->p64; }
+template <typename T> CoeffsT<T> dev_coef(const qms_handle *h);
+template <> CoeffsT<float> dev_coef<float>(const qms_handle *h) {
+    return CoeffsT<float>{{h->c32.im[0], h->c32.im[1], h->c32.im[2]}, {h->c32.sg[0], h->c32.sg[1], h->c32.sg[2]}};
+}
+template <> CoeffsT<double> dev_coef<double>(const qms_handle *h) {
+    return CoeffsT<double>{{h->c64.im[0], h->c64.im[1], h->c64.im[2]}, {h->c64.sg[0], h->c64.sg[1], h->c64.sg[2]}};
+}
 
 void launch_va(qms_handle *h, const OpParams &op, const float *x, float *y) { launch_va_f32(h->g, op_coeffs(h), op, x, y, h->s); }
 void launch_va(qms_handle *h, const OpParams &op, const double *x, double *y) { launch_va_f64(h->g, op_coeffs(h), op, x, y, h->s); }
-
-template <typename T>
-void ensure_mg(qms_handle *h, int tr) {
-    const Fine f = h->fine(tr ? h->cfg.dt : 0.0);
-    if (!h->builtV) {
-        h->alloc_hierarchy(hierV<T>(h), 1);
-        mg_build<T>(f, hierV<T>(h), h->s);
-        h->builtV = true;
-    }
-    if (!h->built[tr]) {
-        h->alloc_hierarchy(hierA<T>(h, tr), 0);
-        mg_build<T>(f, hierA<T>(h, tr), h->s);
-        h->built[tr] = true;
-    }
-    h->st.n_levels_a = hierA<T>(h, tr).nlev;
-    h->st.n_levels_v = hierV<T>(h).nlev;
+void launch_apply_t(qms_handle *h, const OpParams &op, const float *x, double *y) {
+    launch_apply_f32in(h->g, op_coeffs(h), op, x, y, h->s);
+}
+void launch_apply_t(qms_handle *h, const OpParams &op, const double *x, double *y) {
+    launch_apply(h->g, op_coeffs(h), op, x, y, h->s);
 }
 
-// z = P^-1 v : block lower-triangular [K_A 0; M_VA L_V]^-1 with one V-cycle
+// Everything the solve in precision T needs, allocated and built before any
+// kernel of the solve is queued: the multigrid hierarchies (host-built Galerkin
+// levels, uploaded), the Z basis and the preconditioner's work vectors.
+template <typename T>
+void ensure_solver(qms_handle *h, int tr) {
+    PrecondState<T> &ps = pstate<T>(h);
+    const HostCoef hc = h->host_coef();
+    const CoeffsT<T> dc = dev_coef<T>(h);
+    const MgAlloc al = h->mg_alloc();
+    if (ps.Z.empty()) {
+        const int m = h->cfg.restart;
+        for (int j = 0; j < m; ++j) ps.Z.push_back(h->dalloc<T>(h->n));
+        ps.dZ = h->dalloc<T *>(m);
+        check(hipMemcpyAsync(ps.dZ, ps.Z.data(), sizeof(T *) * m, hipMemcpyHostToDevice, h->s), "h2d");
+        ps.rT = h->dalloc<T>(h->n);
+        ps.tT = h->dalloc<T>(h->n);
+    }
+    if (!ps.builtV) {
+        mg_setup<T>(h->g, hc, dc, 0.0, 1, al, ps.hV);
+        ps.builtV = true;
+    }
+    if (!ps.builtA[tr]) {
+        mg_setup<T>(h->g, hc, dc, tr ? h->cfg.dt : 0.0, 0, al, ps.hA[tr]);
+        ps.builtA[tr] = true;
+    }
+    check(hipStreamSynchronize(h->s), "sync");
+    h->st.n_levels_a = ps.hA[tr].nlev;
+    h->st.n_levels_v = ps.hV.nlev;
+}
+
+// z = P^-1 v: block lower-triangular [K_A 0; M_VA L_V]^-1 with one V-cycle
 // per diagonal block, applied to D v (unscaled residual units).
 template <typename T>
-void precond(qms_handle *h, const OpParams &op, int tr, const double *D, const double *v, double *z) {
+void precond(qms_handle *h, const OpParams &op, int tr, const double *D, const double *v, T *z) {
+    PrecondState<T> &ps = pstate<T>(h);
     const int64_t n1 = h->g.n1, n = h->n;
-    T *rT = (T *)h->rT, *zT = (T *)h->zT, *tT = (T *)h->tT;
-    const Fine f = h->fine(tr ? h->cfg.dt : 0.0);
     const MGParams mp = mgp(h);
-    launch_d2t(v, D, rT, n, h->s);
-    mg_vcycle<T>(f, hierA<T>(h, tr), mp, zT, rT, h->s);
+    launch_d2t(v, D, ps.rT, n, h->s);
+    mg_vcycle<T>(ps.hA[tr], mp, z, ps.rT, h->s);
     if (op.tr || op.adv) {
-        launch_va(h, op, zT, tT);
-        launch_sub(rT + 3 * n1, tT + 3 * n1, n1, h->s);
+        launch_va(h, op, z, ps.tT);
+        launch_sub(ps.rT + 3 * n1, ps.tT + 3 * n1, n1, h->s);
     }
-    mg_vcycle<T>(f, hierV<T>(h), mp, zT + 3 * n1, rT + 3 * n1, h->s);
-    launch_t2d(zT, z, n, h->s);
+    mg_vcycle<T>(ps.hV, mp, z + 3 * n1, ps.rT + 3 * n1, h->s);
 }
 
+// ||w||_2, read back on the host (one synchronisation).
 double norm_dev(qms_handle *h, const double *w) {
     launch_norm2(w, h->n, h->partial, h->nblk, h->s);
     launch_reduce_partials(h->partial, 1, h->nblk, h->dsmall, h->s);
@@ -312,21 +318,45 @@ void gram_schmidt(qms_handle *h, int nv) {
     launch_reduce_partials(h->partial, 1, h->nblk, h->dsmall + nv, h->s);
 }
 
-// Restarted FGMRES with FP64 residual recomputation at each restart.
+// One FGMRES iteration j, queued with no host round trip: z_j = P^-1 v_j,
+// w = D^-1 M z_j, Gram-Schmidt, Givens update on the device, v_{j+1}.
+template <typename T>
+void fgmres_iteration(qms_handle *h, const OpParams &op, int tr, const double *D, int j) {
+    PrecondState<T> &ps = pstate<T>(h);
+    const int m = h->cfg.restart;
+    precond<T>(h, op, tr, D, h->V[j], ps.Z[j]);
+    launch_apply_t(h, op, ps.Z[j], h->w);
+    gram_schmidt(h, j + 1);
+    launch_givens(j, m, h->dsmall, h->Hd, h->csd, h->snd, h->gd, h->std_, h->s);
+    launch_scale_dev(h->V[j + 1], h->w, h->std_ + KS_INVH, h->n, h->s);
+}
+
+// Reads the device cycle scalars (one synchronisation).
+void read_cycle(qms_handle *h, double out[KS_COUNT]) {
+    check(hipMemcpyAsync(h->hsmall, h->std_, sizeof(double) * KS_COUNT, hipMemcpyDeviceToHost, h->s), "d2h");
+    check(hipStreamSynchronize(h->s), "sync");
+    memcpy(out, h->hsmall, sizeof(double) * KS_COUNT);
+}
+
+// Restarted FGMRES on D^-1 M x = D^-1 b with an FP64 residual at every
+// restart.  Iterations are queued ahead in chunks (the first chunk as long as
+// the last solve took), and the host reads the device's convergence record
+// only between chunks.
 template <typename T>
 int fgmres(qms_handle *h, const OpParams &op, int tr) {
+    PrecondState<T> &ps = pstate<T>(h);
     const int64_t n = h->n;
     const int m = h->cfg.restart;
     const double *D = tr ? h->Dtr : h->Dst;
     hipStream_t s = h->s;
-    // ||D^-1 b||: residual of x = 0
-    launch_scale(h->w, h->x, 0.0, n, s);
+    // ||D^-1 b||: the residual of x = 0
+    launch_fill<double>(h->w, n, 0.0, s);
     launch_resid(h->g, op_coeffs(h), op, h->w, h->b, h->r, s);
     const double bnorm = norm_dev(h, h->r);
-    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), gv(m + 1), y(m);
-    int total = 0, restarts = 0;
-    double rel = 0.0, rel0 = -1.0;
     const double tol = h->cfg.tol;
+    int used = 0, queued = 0, restarts = 0;
+    double rel = 0.0, rel0 = -1.0;
+    double cyc[KS_COUNT];
     while (true) {
         launch_resid(h->g, op_coeffs(h), op, h->x, h->b, h->r, s);
         const double beta = norm_dev(h, h->r);
@@ -334,88 +364,53 @@ int fgmres(qms_handle *h, const OpParams &op, int tr) {
         if (rel0 < 0) rel0 = rel;
         if (!(rel == rel)) throw HipError("residual is NaN", QMS_ERR_NOCONV);
         if (rel <= tol || beta == 0.0) break;
-        if (total >= h->cfg.max_iters) {
-            h->st.last_iters = total; h->st.last_restarts = restarts; h->st.last_relres = rel;
+        if (used >= h->cfg.max_iters) {
+            h->st.last_iters = used; h->st.last_restarts = restarts; h->st.last_relres = rel;
             h->st.last_relres0 = rel0;
             char buf[160];
             snprintf(buf, sizeof buf, "FGMRES stopped after %d iterations at relative residual %.3e (tol %.1e)",
-                     total, rel, tol);
+                     used, rel, tol);
             throw HipError(buf, QMS_ERR_NOCONV);
         }
         ++restarts;
         launch_scale(h->V[0], h->r, 1.0 / beta, n, s);
-        std::fill(gv.begin(), gv.end(), 0.0);
-        gv[0] = beta;
-        int k = 0;
-        for (int j = 0; j < m && total < h->cfg.max_iters; ++j) {
-            precond<T>(h, op, tr, D, h->V[j], h->Z[j]);
-            launch_apply(h->g, op_coeffs(h), op, h->Z[j], h->w, s);
-            gram_schmidt(h, j + 1);
-            check(hipMemcpyAsync(h->hsmall, h->dsmall, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, s), "d2h");
-            check(hipStreamSynchronize(s), "sync");
-            for (int i = 0; i <= j; ++i) H[(size_t)i * m + j] = h->hsmall[i];
-            const double hn = sqrt(h->hsmall[j + 1]);
-            H[(size_t)(j + 1) * m + j] = hn;
-            ++total;
-            // Givens
-            for (int i = 0; i < j; ++i) {
-                const double a = H[(size_t)i * m + j], bb = H[(size_t)(i + 1) * m + j];
-                H[(size_t)i * m + j] = cs[i] * a + sn[i] * bb;
-                H[(size_t)(i + 1) * m + j] = -sn[i] * a + cs[i] * bb;
-            }
-            const double a = H[(size_t)j * m + j], bb = H[(size_t)(j + 1) * m + j];
-            const double dd = hypot(a, bb);
-            cs[j] = dd > 0 ? a / dd : 1.0;
-            sn[j] = dd > 0 ? bb / dd : 0.0;
-            H[(size_t)j * m + j] = dd;
-            H[(size_t)(j + 1) * m + j] = 0.0;
-            gv[j + 1] = -sn[j] * gv[j];
-            gv[j] = cs[j] * gv[j];
-            k = j + 1;
-            const double est = fabs(gv[j + 1]) / (bnorm > 0 ? bnorm : 1.0);
-            if (hn == 0.0 || est <= tol * 0.5) break;
-            launch_scale(h->V[j + 1], h->w, 1.0 / hn, n, s);
+        launch_cycle_start(m, beta, bnorm, tol, h->gd, h->std_, s);
+        int j = 0, k = 0;
+        int chunk = h->pred[tr] > 0 ? std::min(h->pred[tr], m) : 4;
+        while (j < m) {
+            const int nq = std::min(chunk, m - j);
+            for (int q = 0; q < nq; ++q) fgmres_iteration<T>(h, op, tr, D, j + q);
+            j += nq;
+            queued += nq;
+            read_cycle(h, cyc);
+            if (!(cyc[KS_EST] == cyc[KS_EST])) throw HipError("FGMRES residual estimate is NaN", QMS_ERR_NOCONV);
+            if (cyc[KS_K] > 0) { k = (int)cyc[KS_K]; break; }
+            chunk = 2;
         }
-        // back substitution
-        for (int i = k - 1; i >= 0; --i) {
-            double t = gv[i];
-            for (int c = i + 1; c < k; ++c) t -= H[(size_t)i * m + c] * y[c];
-            y[i] = t / H[(size_t)i * m + i];
-        }
-        memcpy(h->hsmall, y.data(), sizeof(double) * k);
-        check(hipMemcpyAsync(h->dsmall, h->hsmall, sizeof(double) * k, hipMemcpyHostToDevice, s), "h2d");
-        launch_axpy_list(h->x, (const double *const *)h->dZ, k, h->dsmall, n, s);
-        check(hipStreamSynchronize(s), "sync");
+        if (k == 0) k = m;
+        used += k;
+        launch_backsub(k, m, h->Hd, h->gd, h->yd, s);
+        launch_axpy_list(h->x, (const T *const *)ps.dZ, k, h->yd, n, s);
+        h->pred[tr] = k;
     }
-    h->st.last_iters = total;
+    h->st.last_iters = used;
     h->st.last_restarts = restarts;
     h->st.last_relres = rel;
     h->st.last_relres0 = rel0;
-    h->st.total_iters += total;
+    h->st.total_iters += used;
     h->st.solves += 1;
+    h->st.queued_iters += queued;
     return QMS_OK;
 }
 
 int solve(qms_handle *h, const OpParams &op, int tr) {
-    if (h->cfg.precision == QMS_MIXED) {
-        ensure_mg<float>(h, tr);
-        return fgmres<float>(h, op, tr);
-    }
-    ensure_mg<double>(h, tr);
+    if (h->cfg.precision == QMS_MIXED) return fgmres<float>(h, op, tr);
     return fgmres<double>(h, op, tr);
 }
 
-// The multigrid-preconditioned solve has faulted the GPU in its first
-// hierarchy build, with no cause found in this code (a CPU build of the same
-// sources runs the same tests clean under AddressSanitizer).  Until the cause
-// is shown, the solve entry points refuse to run unless QMS_ENABLE_SOLVE=1 is
-// set explicitly.
-bool solve_enabled(qms_handle *h) {
-    const char *e = getenv("QMS_ENABLE_SOLVE");
-    if (e && e[0] == '1') return true;
-    h->err = "solve path disabled: it faulted the GPU during the first multigrid hierarchy build "
-             "(cause not identified); set QMS_ENABLE_SOLVE=1 to run it";
-    return false;
+void prepare_solve(qms_handle *h, int tr) {
+    if (h->cfg.precision == QMS_MIXED) ensure_solver<float>(h, tr);
+    else ensure_solver<double>(h, tr);
 }
 
 int fail(qms_handle *h, const std::exception &e, int code) {
@@ -511,21 +506,19 @@ int qms_create(const qms_config *cfg, const uint16_t *cell_slot, const double *s
     launch_diag(g, h->c64, make_op(h.get(), v0, cfg->dt, cfg->dt > 0 ? 1 : 0), h->Dtr, h->s);
     const int m = cfg->restart;
     for (int j = 0; j <= m; ++j) h->V.push_back(h->dalloc<double>(h->n));
-    for (int j = 0; j < m; ++j) h->Z.push_back(h->dalloc<double>(h->n));
     h->dV = h->dalloc<double *>(m + 1);
-    h->dZ = h->dalloc<double *>(m);
     check(hipMemcpyAsync(h->dV, h->V.data(), sizeof(double *) * (m + 1), hipMemcpyHostToDevice, h->s), "h2d");
-    check(hipMemcpyAsync(h->dZ, h->Z.data(), sizeof(double *) * m, hipMemcpyHostToDevice, h->s), "h2d");
     h->nblk = reduce_blocks(h->n);
     h->partial = h->dalloc<double>((int64_t)(m + 2) * h->nblk);
     h->dsmall = h->dalloc<double>(m + 8);
-    check(hipHostMalloc((void **)&h->hsmall, sizeof(double) * (m + 8), hipHostMallocDefault), "hipHostMalloc");
-    const size_t tsz = cfg->precision == QMS_MIXED ? sizeof(float) : sizeof(double);
-    if (tsz == sizeof(float)) {
-        h->rT = h->dalloc<float>(h->n); h->zT = h->dalloc<float>(h->n); h->tT = h->dalloc<float>(h->n);
-    } else {
-        h->rT = h->dalloc<double>(h->n); h->zT = h->dalloc<double>(h->n); h->tT = h->dalloc<double>(h->n);
-    }
+    h->Hd = h->dalloc<double>((int64_t)(m + 1) * m);
+    h->csd = h->dalloc<double>(m);
+    h->snd = h->dalloc<double>(m);
+    h->gd = h->dalloc<double>(m + 1);
+    h->yd = h->dalloc<double>(m);
+    h->std_ = h->dalloc<double>(KS_COUNT);
+    check(hipHostMalloc((void **)&h->hsmall, sizeof(double) * (m + 8 > KS_COUNT ? m + 8 : KS_COUNT),
+                        hipHostMallocDefault), "hipHostMalloc");
     check(hipStreamSynchronize(h->s), "sync");
     h->st.device_bytes = h->bytes;
     *out = h.release();
@@ -535,13 +528,11 @@ int qms_create(const qms_config *cfg, const uint16_t *cell_slot, const double *s
 
 int qms_steady(qms_handle *h, const double v[3]) {
     if (!h || !v) return QMS_ERR_ARG;
-    if (!solve_enabled(h)) return QMS_ERR_STATE;
     QMS_TRY(h)
+    prepare_solve(h, 0);
     const OpParams op = make_op(h, v, 0.0, 0);
-    const double zero[3] = {0, 0, 0};
-    (void)zero;
     launch_rhs(h->g, op_coeffs(h), h->bc, op, h->x, h->b, h->s);
-    launch_scale(h->x, h->x, 0.0, h->n, h->s);
+    launch_fill<double>(h->x, h->n, 0.0, h->s);
     launch_setbc(h->g, h->bc, op, h->x, h->s);
     solve(h, op, 0);
     h->have_state = true;
@@ -553,8 +544,8 @@ int qms_step(qms_handle *h, const double v[3]) {
     if (!h || !v) return QMS_ERR_ARG;
     if (!h->have_state) { h->err = "qms_step: no state (call qms_steady or qms_set_fields first)"; return QMS_ERR_STATE; }
     if (!(h->cfg.dt > 0)) { h->err = "qms_step: dt must be positive"; return QMS_ERR_ARG; }
-    if (!solve_enabled(h)) return QMS_ERR_STATE;
     QMS_TRY(h)
+    prepare_solve(h, 1);
     const OpParams op = make_op(h, v, h->cfg.dt, 1);
     launch_rhs(h->g, op_coeffs(h), h->bc, op, h->x, h->b, h->s);
     launch_setbc(h->g, h->bc, op, h->x, h->s);
@@ -693,10 +684,7 @@ int qms_time_kernel(qms_handle *h, int which, const double v[3], int reps, doubl
     const int64_t nc = (int64_t)h->cfg.cells[0] * h->cfg.cells[1] * h->cfg.cells[2];
     const bool mixed = h->cfg.precision == QMS_MIXED;
     const double s = mixed ? 4.0 : 8.0;
-    if (which == 1 || which == 2) {
-        if (!solve_enabled(h)) return QMS_ERR_STATE;
-        if (mixed) ensure_mg<float>(h, tr); else ensure_mg<double>(h, tr);
-    }
+    if (which == 1 || which == 2) prepare_solve(h, tr);
     // The arrays a launch reads (x, b) and writes (w): the handle's own, or
     // one of the copies the cold timing rotates through.
     struct Arrays {
@@ -718,15 +706,14 @@ int qms_time_kernel(qms_handle *h, int which, const double v[3], int reps, doubl
         } else if (which == 7) {
             gram_schmidt(h, h->cfg.restart + 1);
         } else {
-            const Fine f = h->fine(tr ? h->cfg.dt : 0.0);
             const MGParams mp = mgp(h);
-            if (mixed) {
-                if (which == 1) mg_fine_sweep<float>(f, h->hA32[tr], mp, (const float *)h->zT, (const float *)h->rT, (float *)h->tT, h->s);
-                else mg_fine_sweep<float>(f, h->hV32, mp, (const float *)h->zT + 3 * n1, (const float *)h->rT + 3 * n1, (float *)h->tT + 3 * n1, h->s);
-            } else {
-                if (which == 1) mg_fine_sweep<double>(f, h->hA64[tr], mp, (const double *)h->zT, (const double *)h->rT, (double *)h->tT, h->s);
-                else mg_fine_sweep<double>(f, h->hV64, mp, (const double *)h->zT + 3 * n1, (const double *)h->rT + 3 * n1, (double *)h->tT + 3 * n1, h->s);
-            }
+            auto sweep = [&](auto &ps) {
+                auto *z = ps.Z[0], *rt = ps.rT, *tt = ps.tT;
+                if (which == 1) mg_fine_sweep(ps.hA[tr], mp, z, rt, tt, h->s);
+                else mg_fine_sweep(ps.hV, mp, z + 3 * n1, rt + 3 * n1, tt + 3 * n1, h->s);
+            };
+            if (mixed) sweep(h->p32);
+            else sweep(h->p64);
         }
     };
     if (cold && (which == 0 || (which >= 3 && which <= 6))) {

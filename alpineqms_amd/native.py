@@ -76,6 +76,7 @@ class Stats(ctypes.Structure):
         ("n_levels_a", ctypes.c_int),
         ("n_levels_v", ctypes.c_int),
         ("coef_classes", ctypes.c_int),
+        ("queued_iters", ctypes.c_int64),
     ]
 
 

@@ -66,6 +66,8 @@ typedef struct {
     int64_t device_bytes;    /* device memory held by the handle */
     int n_levels_a, n_levels_v;
     int coef_classes;        /* distinct coefficient tuples (0: operator streams the arrays) */
+    int64_t queued_iters;    /* iterations queued since creation (>= total_iters: the solver
+                                queues iterations ahead and reads convergence between chunks) */
 } qms_stats;
 
 /* Fills defaults: FP64, tol 1e-12, restart 30, 3000 iterations, nu 4,

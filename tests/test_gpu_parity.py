@@ -233,18 +233,9 @@ def _run_both(d, precision, steps):
     return p, coef, ref, out
 
 
-SOLVE_SKIP = ("multigrid solve path faulted the GPU twice (first hierarchy build); cause not shown, "
-              "path disabled in libqms unless QMS_ENABLE_SOLVE=1")
-
-
-@pytest.mark.skip(reason=SOLVE_SKIP)
-@pytest.mark.parametrize("precision", [native.QMS_FP64, native.QMS_MIXED], ids=["fp64", "mixed"])
-@pytest.mark.parametrize("name,mk,steps", [
-    ("N13_ramp", lambda: configs.small_box_design(13, 2, nbox=3, v=4.0, ramp=True, gradient=True, steps=3), 3),
-    ("N16_cfg1", lambda: configs.config1(16, steps=3), 3),
-], ids=["N13_ramp", "N16_cfg1"])
-def test_solve_matches_direct(precision, name, mk, steps):
-    d = mk()
+def check_solve(d, precision, steps, tol=SOL_TOL):
+    """Steady state plus `steps` transient steps through the C ABI against the
+    oracle's direct solves: A, H, B and sigma*grad(V) within `tol`."""
     p, coef, ref, out = _run_both(d, precision, steps)
     for n, ((f, H, B, its), fr, Hr) in enumerate(zip(out, ref.fields, ref.H)):
         Br = O.bfield(p, fr)
@@ -252,6 +243,15 @@ def test_solve_matches_direct(precision, name, mk, steps):
         eH = rel(H, Hr)
         eB = rel(B, Br)
         eJ = rel(sigma_grad_v(p, coef, f), sigma_grad_v(p, coef, fr)) if np.any(fr[3]) else 0.0
-        print(f"{name} {precision} state {n}: iters {its} eA {eA:.2e} eH {eH:.2e} eB {eB:.2e} e(sigma gradV) {eJ:.2e}",
-              flush=True)
-        assert eA < SOL_TOL and eH < SOL_TOL and eB < SOL_TOL and eJ < SOL_TOL, (n, eA, eH, eB, eJ)
+        print(f"{d.name} {precision} state {n}: iters {its} eA {eA:.2e} eH {eH:.2e} eB {eB:.2e} "
+              f"e(sigma gradV) {eJ:.2e}", flush=True)
+        assert eA < tol and eH < tol and eB < tol and eJ < tol, (n, eA, eH, eB, eJ)
+
+
+@pytest.mark.parametrize("precision", [native.QMS_FP64, native.QMS_MIXED], ids=["fp64", "mixed"])
+@pytest.mark.parametrize("name,mk,steps", [
+    ("N13_ramp", lambda: configs.small_box_design(13, 2, nbox=3, v=4.0, ramp=True, gradient=True, steps=3), 3),
+    ("N16_cfg1", lambda: configs.config1(16, steps=3), 3),
+], ids=["N13_ramp", "N16_cfg1"])
+def test_solve_matches_direct(precision, name, mk, steps):
+    check_solve(mk(), precision, steps)
