@@ -28,6 +28,7 @@ struct CoeffsF {
     const float *sg[3];
 };
 
+struct OpK;
 struct OpParams {
     double v[3];   // frame velocity (solve_vx, solve_vy, solve_vz)
     double dt;     // del_t (transient rows only)
@@ -35,6 +36,10 @@ struct OpParams {
     int adv;       // 1 if any velocity component is non-zero
     int zchunk;    // planes per operator workgroup (z-march depth)
     int colw;      // operator column width in nodes (64 or 32)
+    // Device copy of make_opk(g, *this): when set, the operator, RHS and
+    // Dirichlet kernels read their row constants from it instead of their
+    // launch arguments (captured graphs then follow the per-step velocity).
+    const OpK *kdev = nullptr;
 };
 
 // Per-launch constants of the operator rows, computed once on the host so
@@ -89,7 +94,7 @@ void launch_operator_batch(const BatchItem *items_dev, const BatchSeg *segs_dev,
 // multiple of 8).
 int batch_resident_blocks(int tx, int cls);
 
-enum { OP_RESID = 0, OP_APPLY = 1, OP_VA = 2 };
+enum { OP_RESID = 0, OP_APPLY = 1, OP_VRHS = 2 };
 
 void launch_coeffs(const Grid &g, const uint16_t *slot, const double *mat, const Coeffs &c, hipStream_t s);
 void launch_diag(const Grid &g, const CoeffsD &c, const OpParams &op, double *D, hipStream_t s);
@@ -99,8 +104,13 @@ void launch_setbc(const Grid &g, const BC &bc, const OpParams &op, double *x, hi
 void launch_resid(const Grid &g, const CoeffsD &c, const OpParams &op, const double *x, const double *b, double *r,
                   hipStream_t s);
 void launch_apply(const Grid &g, const CoeffsD &c, const OpParams &op, const double *x, double *y, hipStream_t s);
-void launch_va_f32(const Grid &g, const CoeffsD &c, const OpParams &op, const float *x, float *y, hipStream_t s);
-void launch_va_f64(const Grid &g, const CoeffsD &c, const OpParams &op, const double *x, double *y, hipStream_t s);
+// The V-block right-hand side of the block-triangular preconditioner, in
+// FP64: out[p] = D_V v_V - (M_VA z_A) on free V rows, 0 elsewhere (n1 entries;
+// z in the A block's precision, v the FP64 Krylov vector).
+void launch_vrhs(const Grid &g, const CoeffsD &c, const OpParams &op, const float *z, const double *v, double *out,
+                 hipStream_t s);
+void launch_vrhs(const Grid &g, const CoeffsD &c, const OpParams &op, const double *z, const double *v, double *out,
+                 hipStream_t s);
 // y = D^-1 M x for an FP32-stored x (FP64 arithmetic and output)
 void launch_apply_f32in(const Grid &g, const CoeffsD &c, const OpParams &op, const float *x, double *y, hipStream_t s);
 void launch_hfield(const Grid &g, const CoeffsD &c, const double *x, double *H, int unit_mu, hipStream_t s);
@@ -197,8 +207,7 @@ void launch_fill(T *x, int64_t n, T v, hipStream_t s);
 // conversions / vector kernels at the Krylov / multigrid boundary
 void launch_d2t(const double *src, const double *scale, float *dst, int64_t n, hipStream_t s);
 void launch_d2t(const double *src, const double *scale, double *dst, int64_t n, hipStream_t s);
-void launch_sub(float *a, const float *b, int64_t n, hipStream_t s);
-void launch_sub(double *a, const double *b, int64_t n, hipStream_t s);
+void launch_cvt(const double *src, float *dst, int64_t n, hipStream_t s);
 
 // Krylov helpers (FP64)
 void launch_multidot(const double *const *V, int nv, const double *w, int64_t n, double *partial, int nblk,
@@ -217,6 +226,7 @@ void launch_givens(int j, int m, const double *hcol, double *H, double *cs, doub
                    hipStream_t s);
 void launch_backsub(int k, int m, const double *H, const double *g, double *y, hipStream_t s);
 void launch_sqrt_to(const double *src, double *dst, hipStream_t s);
+void launch_extrapolate(double *x, double *h1, double *h2, int nh, int64_t n, hipStream_t s);
 void launch_norm2(const double *w, int64_t n, double *partial, int nblk, hipStream_t s);
 int reduce_blocks(int64_t n);
 void launch_copy(double *dst, const double *src, int64_t n, hipStream_t s);

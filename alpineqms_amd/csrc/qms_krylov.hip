@@ -245,6 +245,20 @@ __global__ void k_backsub(int k, int m, const double *__restrict__ H, const doub
     }
 }
 
+// Initial guess of a time step from the solutions of the last steps, and the
+// history shifted by one: with x0 = x^n, x1 = x^(n-1), x2 = x^(n-2),
+//   nh = 0: x0;  nh = 1: 2 x0 - x1;  nh >= 2: 3 x0 - 3 x1 + x2.
+__global__ void __launch_bounds__(256) k_extrapolate(double *__restrict__ x, double *__restrict__ h1,
+                                                     double *__restrict__ h2, int nh, int64_t n) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const double x0 = x[e], x1 = h1[e], x2 = h2[e];
+    const double gss = nh >= 2 ? 3.0 * (x0 - x1) + x2 : nh == 1 ? 2.0 * x0 - x1 : x0;
+    h2[e] = x1;
+    h1[e] = x0;
+    x[e] = gss;
+}
+
 // dst = sqrt(max(src, 0)) (a reduced sum of squares to a norm, on the device)
 __global__ void k_sqrt_to(const double *__restrict__ src, double *__restrict__ dst) {
     if (threadIdx.x == 0 && blockIdx.x == 0) dst[0] = sqrt(fmax(src[0], 0.0));
@@ -315,6 +329,9 @@ void launch_givens(int j, int m, const double *hcol, double *H, double *cs, doub
 }
 void launch_backsub(int k, int m, const double *H, const double *g, double *y, hipStream_t s) {
     hipLaunchKernelGGL(k_backsub, dim3(1), dim3(64), 0, s, k, m, H, g, y);
+}
+void launch_extrapolate(double *x, double *h1, double *h2, int nh, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_extrapolate, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, h1, h2, nh, n);
 }
 void launch_sqrt_to(const double *src, double *dst, hipStream_t s) {
     hipLaunchKernelGGL(k_sqrt_to, dim3(1), dim3(64), 0, s, src, dst);

@@ -767,10 +767,9 @@ __global__ void __launch_bounds__(256) k_d2t(const double *__restrict__ src, con
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p < n) dst[p] = (TD)(src[p] * scale[p]);
 }
-template <typename TT>
-__global__ void __launch_bounds__(256) k_sub(TT *__restrict__ a, const TT *__restrict__ b, int64_t n) {
+__global__ void __launch_bounds__(256) k_cvt(const double *__restrict__ src, float *__restrict__ dst, int64_t n) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < n) a[p] -= b[p];
+    if (p < n) dst[p] = (float)src[p];
 }
 void launch_d2t(const double *src, const double *scale, float *dst, int64_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_d2t<float>, g1(n), dim3(256), 0, s, src, scale, dst, n);
@@ -778,11 +777,8 @@ void launch_d2t(const double *src, const double *scale, float *dst, int64_t n, h
 void launch_d2t(const double *src, const double *scale, double *dst, int64_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_d2t<double>, g1(n), dim3(256), 0, s, src, scale, dst, n);
 }
-void launch_sub(float *a, const float *b, int64_t n, hipStream_t s) {
-    hipLaunchKernelGGL(k_sub<float>, g1(n), dim3(256), 0, s, a, b, n);
-}
-void launch_sub(double *a, const double *b, int64_t n, hipStream_t s) {
-    hipLaunchKernelGGL(k_sub<double>, g1(n), dim3(256), 0, s, a, b, n);
+void launch_cvt(const double *src, float *dst, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_cvt, g1(n), dim3(256), 0, s, src, dst, n);
 }
 
 }  // namespace qms

@@ -633,10 +633,17 @@ __device__ __forceinline__ void op_column(const Grid &gin, const ColSpan cs, con
         stage(kk + 1, sp);
         __syncthreads();
         const LdsView<TX> xv{lds, {sm * S::PL, s0 * S::PL, sp * S::PL}, ctr};
-        if (MODE == OP_VA) {
-            if (live && free_v(g, i, j, kk)) {
-                double d;
-                out[3 * n1 + p] = (TOUT)row_v(xv, cv, k, B_VA, d);
+        if (MODE == OP_VRHS) {
+            // V-block right-hand side of the preconditioner, in FP64:
+            // D_V v_V - M_VA z_A (n1 entries, 0 on inactive V rows)
+            if (live) {
+                double rv = 0.0;
+                if (free_v(g, i, j, kk)) {
+                    double d;
+                    const double va = row_v(xv, cv, k, B_VA, d);
+                    rv = d * b[3 * n1 + p] - va;
+                }
+                out[p] = (TOUT)rv;
             }
         } else {
             double rr4[4], dd4[4];
@@ -668,10 +675,23 @@ __device__ __forceinline__ void op_column(const Grid &gin, const ColSpan cs, con
     }
 }
 
-template <typename TIN, typename TOUT, int MODE, int TX, bool CLS>
-__global__ void __launch_bounds__(256) k_operator(Grid g, ZTiling t, CoeffsD c, OpK k, const TIN *__restrict__ x,
-                                                  const double *__restrict__ b, TOUT *__restrict__ out) {
+// The row constants as the kernel sees them: the launch argument, or (KP) a
+// device-resident copy, so that a captured launch stays valid when the frame
+// velocity changes between steps.  The copy is read through a read-only,
+// non-aliased kernel-argument pointer: uniform loads the compiler issues as
+// scalar loads.
+template <bool KP>
+__device__ __forceinline__ OpK op_consts(const OpK &k, const OpK *__restrict__ kp) {
+    if constexpr (KP) return *kp;
+    else return k;
+}
+
+template <typename TIN, typename TOUT, int MODE, int TX, bool CLS, bool KP>
+__global__ void __launch_bounds__(256) k_operator(Grid g, ZTiling t, CoeffsD c, OpK k0, const OpK *__restrict__ kp,
+                                                  const TIN *__restrict__ x, const double *__restrict__ b,
+                                                  TOUT *__restrict__ out) {
     __shared__ double lds[4 * 3 * ZShape<TX>::PL];
+    const OpK k = op_consts<KP>(k0, kp);
     // The unrolled staging is used where the kernel still fits in 128 VGPRs
     // (four waves per SIMD) with it: the class-table apply on 32-wide
     // columns.  Elsewhere it costs a wave per SIMD and runs slower.
@@ -779,8 +799,10 @@ __global__ void k_diag(Grid g, CoeffsD c, OpK k, double *__restrict__ D) {
 // Right-hand side b(A^n), QMS_3D.f90:1459-1475, :1624, :1772, :1906-1928
 // (steady: :459-470 etc. with zero interior RHS).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_rhs(Grid g, Tiling t, CoeffsD c, BC bc, OpK k,
+template <bool KP>
+__global__ void __launch_bounds__(256) k_rhs(Grid g, Tiling t, CoeffsD c, BC bc, OpK k0, const OpK *__restrict__ kp,
                                              const double *__restrict__ aprev, double *__restrict__ b) {
+    const OpK k = op_consts<KP>(k0, kp);
     int i, j, kk;
     if (!tile_node(g, t, i, j, kk)) return;
     const int64_t p = gidx(g, i, j, kk);
@@ -812,7 +834,10 @@ __global__ void __launch_bounds__(256) k_rhs(Grid g, Tiling t, CoeffsD c, BC bc,
 
 // x <- Dirichlet values on fixed entries, 0 where no unknown exists; free
 // entries keep their value (warm start).
-__global__ void k_setbc(Grid g, BC bc, OpK k, double *__restrict__ x) {
+template <bool KP>
+__global__ void __launch_bounds__(256) k_setbc(Grid g, BC bc, OpK k0, const OpK *__restrict__ kp,
+                                               double *__restrict__ x) {
+    const OpK k = op_consts<KP>(k0, kp);
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= g.n1) return;
     const int i = (int)(p % g.nx), j = (int)((p / g.nx) % g.ny), kk = (int)(p / ((int64_t)g.nx * g.ny));
@@ -922,13 +947,23 @@ static void launch_operator(const Grid &g, const CoeffsD &c, const OpParams &op,
     unsigned ncol;
     const ZTiling t = ztiling(g, op.zchunk, op.colw, ncol);
     const OpK k = make_opk(g, op);
+    const OpK *kp = op.kdev;
     const dim3 grid(ncol), block(256);
-    if (c.cls) {
-        if (t.tx == 32) hipLaunchKernelGGL((k_operator<TIN, TOUT, MODE, 32, true>), grid, block, 0, s, g, t, c, k, x, b, out);
-        else hipLaunchKernelGGL((k_operator<TIN, TOUT, MODE, 64, true>), grid, block, 0, s, g, t, c, k, x, b, out);
+    auto go = [&](auto tx, auto cls, auto kpc) {
+        constexpr int TX = decltype(tx)::value;
+        constexpr bool CLS = decltype(cls)::value, KP = decltype(kpc)::value;
+        hipLaunchKernelGGL((k_operator<TIN, TOUT, MODE, TX, CLS, KP>), grid, block, 0, s, g, t, c, k, kp, x, b, out);
+    };
+    using T32 = std::integral_constant<int, 32>;
+    using T64 = std::integral_constant<int, 64>;
+    using Y = std::true_type;
+    using N = std::false_type;
+    if (kp) {
+        if (c.cls) { if (t.tx == 32) go(T32(), Y(), Y()); else go(T64(), Y(), Y()); }
+        else { if (t.tx == 32) go(T32(), N(), Y()); else go(T64(), N(), Y()); }
     } else {
-        if (t.tx == 32) hipLaunchKernelGGL((k_operator<TIN, TOUT, MODE, 32, false>), grid, block, 0, s, g, t, c, k, x, b, out);
-        else hipLaunchKernelGGL((k_operator<TIN, TOUT, MODE, 64, false>), grid, block, 0, s, g, t, c, k, x, b, out);
+        if (c.cls) { if (t.tx == 32) go(T32(), Y(), N()); else go(T64(), Y(), N()); }
+        else { if (t.tx == 32) go(T32(), N(), N()); else go(T64(), N(), N()); }
     }
 }
 
@@ -982,10 +1017,12 @@ void launch_rhs(const Grid &g, const CoeffsD &c, const BC &bc, const OpParams &o
                 hipStream_t s) {
     dim3 grid;
     const Tiling t = rhs_tiling(g, grid);
-    hipLaunchKernelGGL(k_rhs, grid, dim3(256), 0, s, g, t, c, bc, make_opk(g, op), aprev, b);
+    if (op.kdev) hipLaunchKernelGGL(k_rhs<true>, grid, dim3(256), 0, s, g, t, c, bc, make_opk(g, op), op.kdev, aprev, b);
+    else hipLaunchKernelGGL(k_rhs<false>, grid, dim3(256), 0, s, g, t, c, bc, make_opk(g, op), op.kdev, aprev, b);
 }
 void launch_setbc(const Grid &g, const BC &bc, const OpParams &op, double *x, hipStream_t s) {
-    hipLaunchKernelGGL(k_setbc, grid1(g.n1), dim3(256), 0, s, g, bc, make_opk(g, op), x);
+    if (op.kdev) hipLaunchKernelGGL(k_setbc<true>, grid1(g.n1), dim3(256), 0, s, g, bc, make_opk(g, op), op.kdev, x);
+    else hipLaunchKernelGGL(k_setbc<false>, grid1(g.n1), dim3(256), 0, s, g, bc, make_opk(g, op), op.kdev, x);
 }
 void launch_resid(const Grid &g, const CoeffsD &c, const OpParams &op, const double *x, const double *b, double *r,
                   hipStream_t s) {
@@ -997,11 +1034,13 @@ void launch_apply(const Grid &g, const CoeffsD &c, const OpParams &op, const dou
 void launch_apply_f32in(const Grid &g, const CoeffsD &c, const OpParams &op, const float *x, double *y, hipStream_t s) {
     launch_operator<float, double, OP_APPLY>(g, c, op, x, nullptr, y, s);
 }
-void launch_va_f32(const Grid &g, const CoeffsD &c, const OpParams &op, const float *x, float *y, hipStream_t s) {
-    launch_operator<float, float, OP_VA>(g, c, op, x, nullptr, y, s);
+void launch_vrhs(const Grid &g, const CoeffsD &c, const OpParams &op, const float *z, const double *v, double *out,
+                 hipStream_t s) {
+    launch_operator<float, double, OP_VRHS>(g, c, op, z, v, out, s);
 }
-void launch_va_f64(const Grid &g, const CoeffsD &c, const OpParams &op, const double *x, double *y, hipStream_t s) {
-    launch_operator<double, double, OP_VA>(g, c, op, x, nullptr, y, s);
+void launch_vrhs(const Grid &g, const CoeffsD &c, const OpParams &op, const double *z, const double *v, double *out,
+                 hipStream_t s) {
+    launch_operator<double, double, OP_VRHS>(g, c, op, z, v, out, s);
 }
 void launch_hfield(const Grid &g, const CoeffsD &c, const double *x, double *H, int unit_mu, hipStream_t s) {
     const int64_t nc = (int64_t)(g.nx - 1) * (g.ny - 1) * (g.nz - 1);

@@ -20,6 +20,7 @@
 #include <array>
 #include <algorithm>
 #include <unordered_map>
+#include <type_traits>
 #include "qms_kernels.h"
 #include "qms_plan.h"
 #include "../../include/qms.h"
@@ -64,17 +65,22 @@ double time_on(hipStream_t s, int reps, const std::function<void()> &f) {
 
 }  // namespace
 
-// Preconditioner state in the multigrid precision T: the hierarchies and the
-// FGMRES Z basis (z_j = P^-1 v_j) in T, the rest of FGMRES in FP64.
-template <typename T>
+// Preconditioner state.  The A block's multigrid and the FGMRES Z basis
+// (z_j = P^-1 v_j) are in TA (float in mixed precision, double otherwise);
+// the V block's multigrid is always FP64: the conductors' floating
+// potentials (a large constant on each conductor, fixed only through the
+// 1e-3 S/m background) do not survive FP32 storage of the V correction, and
+// FP32 there costs the Krylov solve about half as many iterations again.
+template <typename TA>
 struct PrecondState {
-    Hierarchy<T> hA[2];        // A block: steady (0) / transient (1) operator
-    Hierarchy<T> hV;           // V block (independent of dt)
+    Hierarchy<TA> hA[2];       // A block: steady (0) / transient (1) operator
+    Hierarchy<double> hV;      // V block (independent of dt)
     bool builtA[2] = {false, false};
     bool builtV = false;
-    T *rT = nullptr, *tT = nullptr;   // D v in T; the V rows' coupling term
-    std::vector<T *> Z;
-    T **dZ = nullptr;
+    TA *rT = nullptr;          // D v on the A rows, in TA
+    double *rV = nullptr, *zV = nullptr;   // V-block right-hand side and correction
+    std::vector<TA *> Z;
+    TA **dZ = nullptr;
 };
 
 struct qms_handle {
@@ -100,6 +106,8 @@ struct qms_handle {
     bool use_cls = true;
     double *x = nullptr, *b = nullptr, *r = nullptr, *w = nullptr, *tmp = nullptr;
     double *Dst = nullptr, *Dtr = nullptr;   // row scaling, steady / transient
+    double *xh1 = nullptr, *xh2 = nullptr;   // solutions of the last two steps (initial guess)
+    int nhist = 0;                           // how many of them are valid
     std::vector<double *> V;
     double **dV = nullptr;
     double *partial = nullptr, *dsmall = nullptr;
@@ -114,6 +122,12 @@ struct qms_handle {
     PrecondState<float> p32;
     PrecondState<double> p64;
     bool have_state = false;
+    // Per-solve row constants on the device (read by captured launches), and
+    // the pinned staging copy they are uploaded from.
+    OpK *kdev = nullptr, *kstage = nullptr;
+    // One captured graph per FGMRES iteration index and operator (steady /
+    // transient); the precision is fixed per handle.
+    std::vector<hipGraphExec_t> graphs[2];
 
     qms_stats st;
 
@@ -131,8 +145,12 @@ struct qms_handle {
     }
     ~qms_handle() {
         if (s) hipStreamSynchronize(s);
+        for (auto &gv : graphs)
+            for (hipGraphExec_t e : gv)
+                if (e) hipGraphExecDestroy(e);
         for (void *p : allocs) hipFree(p);
         if (hsmall) hipHostFree(hsmall);
+        if (kstage) hipHostFree(kstage);
         if (s) hipStreamDestroy(s);
     }
     MgAlloc mg_alloc() {
@@ -244,8 +262,6 @@ template <> CoeffsT<double> dev_coef<double>(const qms_handle *h) {
     return CoeffsT<double>{{h->c64.im[0], h->c64.im[1], h->c64.im[2]}, {h->c64.sg[0], h->c64.sg[1], h->c64.sg[2]}};
 }
 
-void launch_va(qms_handle *h, const OpParams &op, const float *x, float *y) { launch_va_f32(h->g, op_coeffs(h), op, x, y, h->s); }
-void launch_va(qms_handle *h, const OpParams &op, const double *x, double *y) { launch_va_f64(h->g, op_coeffs(h), op, x, y, h->s); }
 void launch_apply_t(qms_handle *h, const OpParams &op, const float *x, double *y) {
     launch_apply_f32in(h->g, op_coeffs(h), op, x, y, h->s);
 }
@@ -253,29 +269,30 @@ void launch_apply_t(qms_handle *h, const OpParams &op, const double *x, double *
     launch_apply(h->g, op_coeffs(h), op, x, y, h->s);
 }
 
-// Everything the solve in precision T needs, allocated and built before any
-// kernel of the solve is queued: the multigrid hierarchies (host-built Galerkin
-// levels, uploaded), the Z basis and the preconditioner's work vectors.
-template <typename T>
+// Everything the solve in precision TA needs, allocated and built before any
+// kernel of the solve is queued: the multigrid hierarchies (host-built
+// Galerkin levels, uploaded), the Z basis and the preconditioner's work
+// vectors.
+template <typename TA>
 void ensure_solver(qms_handle *h, int tr) {
-    PrecondState<T> &ps = pstate<T>(h);
+    PrecondState<TA> &ps = pstate<TA>(h);
     const HostCoef hc = h->host_coef();
-    const CoeffsT<T> dc = dev_coef<T>(h);
     const MgAlloc al = h->mg_alloc();
     if (ps.Z.empty()) {
         const int m = h->cfg.restart;
-        for (int j = 0; j < m; ++j) ps.Z.push_back(h->dalloc<T>(h->n));
-        ps.dZ = h->dalloc<T *>(m);
-        check(hipMemcpyAsync(ps.dZ, ps.Z.data(), sizeof(T *) * m, hipMemcpyHostToDevice, h->s), "h2d");
-        ps.rT = h->dalloc<T>(h->n);
-        ps.tT = h->dalloc<T>(h->n);
+        for (int j = 0; j < m; ++j) ps.Z.push_back(h->dalloc<TA>(h->n));
+        ps.dZ = h->dalloc<TA *>(m);
+        check(hipMemcpyAsync(ps.dZ, ps.Z.data(), sizeof(TA *) * m, hipMemcpyHostToDevice, h->s), "h2d");
+        ps.rT = h->dalloc<TA>(3 * h->g.n1);
+        ps.rV = h->dalloc<double>(h->g.n1);
+        ps.zV = h->dalloc<double>(h->g.n1);
     }
     if (!ps.builtV) {
-        mg_setup<T>(h->g, hc, dc, 0.0, 1, al, ps.hV);
+        mg_setup<double>(h->g, hc, dev_coef<double>(h), 0.0, 1, al, ps.hV);
         ps.builtV = true;
     }
     if (!ps.builtA[tr]) {
-        mg_setup<T>(h->g, hc, dc, tr ? h->cfg.dt : 0.0, 0, al, ps.hA[tr]);
+        mg_setup<TA>(h->g, hc, dev_coef<TA>(h), tr ? h->cfg.dt : 0.0, 0, al, ps.hA[tr]);
         ps.builtA[tr] = true;
     }
     check(hipStreamSynchronize(h->s), "sync");
@@ -284,19 +301,22 @@ void ensure_solver(qms_handle *h, int tr) {
 }
 
 // z = P^-1 v: block lower-triangular [K_A 0; M_VA L_V]^-1 with one V-cycle
-// per diagonal block, applied to D v (unscaled residual units).
-template <typename T>
-void precond(qms_handle *h, const OpParams &op, int tr, const double *D, const double *v, T *z) {
-    PrecondState<T> &ps = pstate<T>(h);
-    const int64_t n1 = h->g.n1, n = h->n;
+// per diagonal block, applied to D v (unscaled residual units).  The V block's
+// right-hand side D_V v_V - M_VA z_A is formed in FP64 by the operator kernel.
+template <typename TA>
+void precond(qms_handle *h, const OpParams &op, int tr, const double *D, const double *v, TA *z) {
+    PrecondState<TA> &ps = pstate<TA>(h);
+    const int64_t n1 = h->g.n1;
     const MGParams mp = mgp(h);
-    launch_d2t(v, D, ps.rT, n, h->s);
-    mg_vcycle<T>(ps.hA[tr], mp, z, ps.rT, h->s);
-    if (op.tr || op.adv) {
-        launch_va(h, op, z, ps.tT);
-        launch_sub(ps.rT + 3 * n1, ps.tT + 3 * n1, n1, h->s);
+    launch_d2t(v, D, ps.rT, 3 * n1, h->s);
+    mg_vcycle<TA>(ps.hA[tr], mp, z, ps.rT, h->s);
+    launch_vrhs(h->g, op_coeffs(h), op, z, v, ps.rV, h->s);
+    if constexpr (std::is_same<TA, double>::value) {
+        mg_vcycle<double>(ps.hV, mp, z + 3 * n1, ps.rV, h->s);
+    } else {
+        mg_vcycle<double>(ps.hV, mp, ps.zV, ps.rV, h->s);
+        launch_cvt(ps.zV, z + 3 * n1, n1, h->s);
     }
-    mg_vcycle<T>(ps.hV, mp, z + 3 * n1, ps.rT + 3 * n1, h->s);
 }
 
 // ||w||_2, read back on the host (one synchronisation).
@@ -329,6 +349,44 @@ void fgmres_iteration(qms_handle *h, const OpParams &op, int tr, const double *D
     gram_schmidt(h, j + 1);
     launch_givens(j, m, h->dsmall, h->Hd, h->csd, h->snd, h->gd, h->std_, h->s);
     launch_scale_dev(h->V[j + 1], h->w, h->std_ + KS_INVH, h->n, h->s);
+}
+
+// Iteration j as a captured graph (captured on first use; every argument it
+// holds -- buffers, the device row constants, the iteration index -- is the
+// same for every later solve of this operator, so replays are the same work).
+template <typename T>
+void run_iteration(qms_handle *h, const OpParams &op, int tr, const double *D, int j) {
+    if (!h->cfg.use_graphs) { fgmres_iteration<T>(h, op, tr, D, j); return; }
+    std::vector<hipGraphExec_t> &gv = h->graphs[tr];
+    if (gv.empty()) gv.assign(h->cfg.restart, nullptr);
+    if (!gv[j]) {
+        hipGraph_t g = nullptr;
+        check(hipStreamBeginCapture(h->s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        try {
+            fgmres_iteration<T>(h, op, tr, D, j);
+        } catch (...) {
+            hipStreamEndCapture(h->s, &g);
+            if (g) hipGraphDestroy(g);
+            throw;
+        }
+        check(hipStreamEndCapture(h->s, &g), "hipStreamEndCapture");
+        hipGraphExec_t e = nullptr;
+        const hipError_t ie = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        check(ie, "hipGraphInstantiate");
+        gv[j] = e;
+    }
+    check(hipGraphLaunch(gv[j], h->s), "hipGraphLaunch");
+}
+
+// Uploads the row constants of this solve to the device copy the solve's
+// kernels read (stream-ordered; the staging buffer is rewritten only after the
+// previous solve synchronised).
+OpParams device_op(qms_handle *h, OpParams op) {
+    *h->kstage = make_opk(h->g, op);
+    check(hipMemcpyAsync(h->kdev, h->kstage, sizeof(OpK), hipMemcpyHostToDevice, h->s), "h2d");
+    op.kdev = h->kdev;
+    return op;
 }
 
 // Reads the device cycle scalars (one synchronisation).
@@ -376,10 +434,12 @@ int fgmres(qms_handle *h, const OpParams &op, int tr) {
         launch_scale(h->V[0], h->r, 1.0 / beta, n, s);
         launch_cycle_start(m, beta, bnorm, tol, h->gd, h->std_, s);
         int j = 0, k = 0;
-        int chunk = h->pred[tr] > 0 ? std::min(h->pred[tr], m) : 4;
+        // first chunk: three quarters of the last solve's count (the count
+        // varies by a few iterations between steps), then two at a time
+        int chunk = h->pred[tr] > 0 ? std::max(2, std::min(3 * h->pred[tr] / 4, m)) : 4;
         while (j < m) {
             const int nq = std::min(chunk, m - j);
-            for (int q = 0; q < nq; ++q) fgmres_iteration<T>(h, op, tr, D, j + q);
+            for (int q = 0; q < nq; ++q) run_iteration<T>(h, op, tr, D, j + q);
             j += nq;
             queued += nq;
             read_cycle(h, cyc);
@@ -403,7 +463,8 @@ int fgmres(qms_handle *h, const OpParams &op, int tr) {
     return QMS_OK;
 }
 
-int solve(qms_handle *h, const OpParams &op, int tr) {
+int solve(qms_handle *h, const OpParams &op0, int tr) {
+    const OpParams op = device_op(h, op0);
     if (h->cfg.precision == QMS_MIXED) return fgmres<float>(h, op, tr);
     return fgmres<double>(h, op, tr);
 }
@@ -440,6 +501,7 @@ int qms_config_default(qms_config *cfg) {
     cfg->mg_nu = 4;
     cfg->mg_omega = 0.7;
     cfg->mg_alpha = 1.8;
+    cfg->use_graphs = 1;
     return QMS_OK;
 }
 
@@ -501,6 +563,8 @@ int qms_create(const qms_config *cfg, const uint16_t *cell_slot, const double *s
     h->tmp = h->dalloc<double>(h->n);
     h->Dst = h->dalloc<double>(h->n);
     h->Dtr = h->dalloc<double>(h->n);
+    h->xh1 = h->dalloc<double>(h->n);
+    h->xh2 = h->dalloc<double>(h->n);
     const double v0[3] = {0, 0, 0};
     launch_diag(g, h->c64, make_op(h.get(), v0, 0.0, 0), h->Dst, h->s);
     launch_diag(g, h->c64, make_op(h.get(), v0, cfg->dt, cfg->dt > 0 ? 1 : 0), h->Dtr, h->s);
@@ -517,6 +581,8 @@ int qms_create(const qms_config *cfg, const uint16_t *cell_slot, const double *s
     h->gd = h->dalloc<double>(m + 1);
     h->yd = h->dalloc<double>(m);
     h->std_ = h->dalloc<double>(KS_COUNT);
+    h->kdev = h->dalloc<OpK>(1);
+    check(hipHostMalloc((void **)&h->kstage, sizeof(OpK), hipHostMallocDefault), "hipHostMalloc");
     check(hipHostMalloc((void **)&h->hsmall, sizeof(double) * (m + 8 > KS_COUNT ? m + 8 : KS_COUNT),
                         hipHostMallocDefault), "hipHostMalloc");
     check(hipStreamSynchronize(h->s), "sync");
@@ -534,6 +600,7 @@ int qms_steady(qms_handle *h, const double v[3]) {
     launch_rhs(h->g, op_coeffs(h), h->bc, op, h->x, h->b, h->s);
     launch_fill<double>(h->x, h->n, 0.0, h->s);
     launch_setbc(h->g, h->bc, op, h->x, h->s);
+    h->nhist = 0;
     solve(h, op, 0);
     h->have_state = true;
     return QMS_OK;
@@ -548,6 +615,11 @@ int qms_step(qms_handle *h, const double v[3]) {
     prepare_solve(h, 1);
     const OpParams op = make_op(h, v, h->cfg.dt, 1);
     launch_rhs(h->g, op_coeffs(h), h->bc, op, h->x, h->b, h->s);
+    // Initial guess: the solution extrapolated in time from the last steps
+    // (exact for a solution that moves linearly or quadratically in time,
+    // as under the reference's constant-acceleration trajectory).
+    launch_extrapolate(h->x, h->xh1, h->xh2, h->nhist, h->n, h->s);
+    h->nhist = std::min(h->nhist + 1, 2);
     launch_setbc(h->g, h->bc, op, h->x, h->s);
     solve(h, op, 1);
     return QMS_OK;
@@ -578,6 +650,7 @@ int qms_set_fields(qms_handle *h, const double *in) {
     check(hipMemcpyAsync(h->x, in, sizeof(double) * h->n, hipMemcpyHostToDevice, h->s), "h2d");
     check(hipStreamSynchronize(h->s), "sync");
     h->have_state = true;
+    h->nhist = 0;
     return QMS_OK;
     QMS_CATCH(h)
 }
@@ -708,9 +781,8 @@ int qms_time_kernel(qms_handle *h, int which, const double v[3], int reps, doubl
         } else {
             const MGParams mp = mgp(h);
             auto sweep = [&](auto &ps) {
-                auto *z = ps.Z[0], *rt = ps.rT, *tt = ps.tT;
-                if (which == 1) mg_fine_sweep(ps.hA[tr], mp, z, rt, tt, h->s);
-                else mg_fine_sweep(ps.hV, mp, z + 3 * n1, rt + 3 * n1, tt + 3 * n1, h->s);
+                if (which == 1) mg_fine_sweep(ps.hA[tr], mp, ps.Z[0], ps.rT, ps.hA[tr].lev[0].t, h->s);
+                else mg_fine_sweep(ps.hV, mp, ps.zV, ps.rV, ps.hV.lev[0].t, h->s);
             };
             if (mixed) sweep(h->p32);
             else sweep(h->p64);

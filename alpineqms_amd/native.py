@@ -61,6 +61,7 @@ class Config(ctypes.Structure):
         ("mg_nu", ctypes.c_int),
         ("mg_omega", ctypes.c_double),
         ("mg_alpha", ctypes.c_double),
+        ("use_graphs", ctypes.c_int),
     ]
 
 
@@ -154,7 +155,7 @@ class Solver:
     def __init__(self, cells: Sequence[int], h: Sequence[float], dt: float, bc: Sequence[float],
                  cell_slot: np.ndarray, slot_props: np.ndarray, precision: int = QMS_FP64,
                  tol: float = 1e-12, restart: int = 30, max_iters: int = 3000, mg_nu: int = 4,
-                 mg_omega: float = 0.7, mg_alpha: float = 1.8):
+                 mg_omega: float = 0.7, mg_alpha: float = 1.8, use_graphs: bool = True):
         L = load()
         cfg = default_config()
         for a in range(3):
@@ -173,6 +174,7 @@ class Solver:
         cfg.mg_nu = int(mg_nu)
         cfg.mg_omega = float(mg_omega)
         cfg.mg_alpha = float(mg_alpha)
+        cfg.use_graphs = 1 if use_graphs else 0
         self.cells = tuple(int(c) for c in cells)
         self.nodes = tuple(c + 1 for c in self.cells)
         self.dt = float(dt)

@@ -54,6 +54,8 @@ typedef struct {
     int mg_nu;           /* Jacobi sweeps before and after each coarse correction */
     double mg_omega;     /* Jacobi damping */
     double mg_alpha;     /* coarse-correction scaling */
+    int use_graphs;      /* 1: each FGMRES iteration replays a captured hipGraph (default);
+                            0: its kernels are launched one by one (same results bit for bit) */
 } qms_config;
 
 typedef struct {
@@ -71,7 +73,7 @@ typedef struct {
 } qms_stats;
 
 /* Fills defaults: FP64, tol 1e-12, restart 30, 3000 iterations, nu 4,
- * omega 0.7, alpha 1.8. */
+ * omega 0.7, alpha 1.8, graphs on. */
 int qms_config_default(qms_config *cfg);
 
 /* Setup and material coefficients: replaces QMS_3D.f90:172-269 (sizes,

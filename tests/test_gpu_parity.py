@@ -255,3 +255,24 @@ def check_solve(d, precision, steps, tol=SOL_TOL):
 ], ids=["N13_ramp", "N16_cfg1"])
 def test_solve_matches_direct(precision, name, mk, steps):
     check_solve(mk(), precision, steps)
+
+
+@pytest.mark.parametrize("precision", [native.QMS_FP64, native.QMS_MIXED], ids=["fp64", "mixed"])
+def test_graph_replays_match_launches(precision):
+    """The same steps with every FGMRES iteration replayed from a captured
+    hipGraph and with its kernels launched one by one: bit-identical states
+    after every step (the graphs are replayed many times, with the steps'
+    own eager kernels and host synchronisations between the replays)."""
+    d = configs.small_box_design(16, 5, nbox=3, v=4.0, ramp=True, gradient=True, steps=4)
+    runs = []
+    for graphs in (True, False):
+        states = []
+        with solver_for(d, precision=precision, use_graphs=graphs) as s:
+            s.steady(program.steady_velocity(d.inputs.traj, d.inputs.dt))
+            states.append(s.get_fields())
+            for n in range(1, 5):
+                s.step(program.step_velocity(d.inputs.traj, d.inputs.dt, n))
+                states.append(s.get_fields())
+        runs.append(states)
+    for n, (a, b) in enumerate(zip(*runs)):
+        assert np.array_equal(a, b), n

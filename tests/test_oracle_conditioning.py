@@ -36,3 +36,5 @@ def test_floating_potential_mode_dominates_v_sensitivity():
     eV = rel(f2[3], fx[3])
     assert eA < 1e-10 and eB < 1e-10 and eJ < 1e-8
     assert eV > 1e3 * max(eA, eB)
+    # a 1e-13 right-hand side change already moves raw V past the 1e-8 bar
+    assert eV > 1e-8, eV

@@ -296,6 +296,48 @@ def _quadratic_field(p, B0, G):
     return f
 
 
+def test_quadratic_background_with_frame_velocity_pins_the_v_dirichlet_values():
+    """At a frame velocity v != 0 and a background gradient G != 0, the
+    quadratic field A = 1/2 B0 x r + c3 (G r) x r with the electric potential
+    V = -(v x B0).r - 1/2 (v x (G r)).r (r the unstaggered node position) is
+    an exact discrete solution of the steady rows of a uniform medium: the
+    potential cancels the motional field sigma v x B in every A row and its
+    divergence in every V row.  So the oracle's fixed V rows (the Dirichlet
+    values, QMS_3D.f90:982-996) must equal that closed form, and the interior
+    rows must leave only round-off.  A dropped term, a wrong sign or a
+    swapped velocity component in the V values breaks one of the two."""
+    rng = np.random.default_rng(12)
+    B0 = np.array([0.03, 0.07, -0.05])
+    g = rng.normal(size=(3, 3))
+    G = (g + g.T) / 2
+    G -= np.eye(3) * np.trace(G) / 3
+    v = np.array([1.5, -0.7, 2.2])
+    p = _uniform_problem(10, B0, G, v=tuple(v))
+    coef = O.coeffs(p)
+    f = _quadratic_field(p, B0, G)
+    nx, ny, nz = p.nodes
+    hx, hy, hz = p.h
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    r = np.stack([i * hx, j * hy, k * hz])
+    Gr = np.einsum("ab,b...->a...", G, r)
+    vxB = np.cross(v, B0)
+    vxGr = np.stack([v[1] * Gr[2] - v[2] * Gr[1], v[2] * Gr[0] - v[0] * Gr[2], v[0] * Gr[1] - v[1] * Gr[0]])
+    f[3] = -np.einsum("a,a...->...", vxB, r) - 0.5 * np.einsum("a...,a...->...", vxGr, r)
+    M, b = O.assemble(p, coef, v)
+    x = O.pack(p, f)
+    fixed = np.diff(M.indptr) == 1
+    nV = nx * ny * nz
+    vrows = np.zeros(x.size, dtype=bool)
+    vrows[-nV:] = True
+    assert np.any(fixed & vrows) and np.any(np.abs(x[fixed & vrows]) > 1e-6)
+    scale = np.abs(x[fixed]).max()
+    assert np.abs(b[fixed] - x[fixed]).max() < 1e-14 * scale
+    assert np.abs(O.pack(p, O.bc_fields(p, v))[fixed] - x[fixed]).max() < 1e-14 * scale
+    d = np.abs(M.diagonal())
+    res = (b - M @ x) / d
+    assert np.abs(res).max() < 1e-12 * np.abs(x).max()
+
+
 def test_quadratic_background_is_exact_and_h_closed_form():
     rng = np.random.default_rng(11)
     B0 = np.array([0.02, -0.05, 0.1])
