@@ -67,6 +67,14 @@ struct ZTiling {
 };
 ZTiling ztiling(const Grid &g, int kz, int tx, unsigned &ncol);
 
+// Hardware workgroup index -> tile index such that each of the 8 XCDs (which
+// receive workgroups round-robin) works on one contiguous range of tiles, and
+// neighbouring tiles share that XCD's L2.
+__device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned nb) {
+    const unsigned q = nb >> 3, r = nb & 7u, x = b & 7u;
+    return x * q + min(x, r) + (b >> 3);
+}
+
 // One design of a batched operator launch (device resident).
 struct BatchItem {
     Grid g;
@@ -145,6 +153,10 @@ template <typename T>
 struct CoeffsT {
     const T *im[3];
     const T *sg[3];
+    // the same values as a 2-byte class per node and a table of FP64 tuples
+    // (1/Mu_x, 1/Mu_y, 1/Mu_z, Sigma_x, Sigma_y, Sigma_z); null if not built
+    const uint16_t *cls;
+    const double *ctab;
 };
 // Host copies of the FP64 coefficient arrays (node-sized).
 struct HostCoef {
@@ -171,6 +183,10 @@ struct Hierarchy {
     const T *im[3] = {nullptr, nullptr, nullptr};
     const T *sg[3] = {nullptr, nullptr, nullptr};
     FineConst<T> fk{};
+    const uint16_t *cls = nullptr;   // coefficient classes (level-0 A-block z-march)
+    const double *ctab = nullptr;
+    ZTiling zt{};                    // level-0 z-march columns (32 x 8 nodes, zt.kz planes)
+    unsigned zcols = 0;
     MgLevel<T> lev[MG_MAX_LEVELS];
     int ncoarse = 0;           // active unknowns of the coarsest level
     int *coarse_idx = nullptr; // [ncoarse] -> comp*n1 + p on the coarsest level

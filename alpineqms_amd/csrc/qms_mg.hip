@@ -317,6 +317,160 @@ __global__ void __launch_bounds__(256) k_mg_resid(P pv, const T *__restrict__ x,
     }
 }
 
+// ---------------------------------------------------------------------------
+// Level 0 of the A block on a z-march: a workgroup owns a 32 x 8 column of
+// nodes and walks it through zt.kz planes; the three components of planes
+// k-1, k, k+1 (with a one-node x/y halo) sit in an LDS ring, so every x value
+// is read from memory about once per column instead of once per stencil use
+// (the per-node kernel issues about 70 loads per node).  Coefficients come
+// from the class table when there is one (2 bytes per node instead of six
+// arrays) and the plane-(k-1) tuple is carried in registers.  The arithmetic
+// is that of k_mg_smooth / k_mg_resid with FineProv, term for term.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct A0Z {
+    Grid g;
+    ZTiling zt;
+    const T *im[3];
+    const T *sg[3];
+    const uint16_t *cls;
+    const double *ctab;
+    FineConst<T> fk;
+};
+
+// Coefficient tuples of a node and of its -x, -y and -z neighbours.
+template <typename T>
+struct ZmTuple {
+    T im0[3], imx[3], imy[3], imz[3], sg0[3];
+    __device__ __forceinline__ T IM(int a, int di, int dj, int dk) const {
+        return dk ? imz[a] : di ? imx[a] : dj ? imy[a] : im0[a];
+    }
+    __device__ __forceinline__ T SG(int a, int, int, int) const { return sg0[a]; }
+};
+
+template <typename T, bool CLS>
+__device__ __forceinline__ void load_tuple(const A0Z<T> &z, int64_t q, T im[3], T sg[3]) {
+    if constexpr (CLS) {
+        const double *t = z.ctab + 6 * (int)z.cls[q];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) { im[a] = (T)t[a]; sg[a] = (T)t[3 + a]; }
+    } else {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) { im[a] = z.im[a][q]; sg[a] = z.sg[a][q]; }
+    }
+}
+
+constexpr int A0_TX = 32, A0_TY = 8, A0_SY = A0_TX + 2, A0_PL = A0_SY * (A0_TY + 2);
+
+template <typename T>
+struct A0View {
+    const T *lds;      // [comp][slot][A0_PL]
+    int b[3];          // slot offsets of planes k-1, k, k+1
+    int ctr;
+    __device__ __forceinline__ T get(int c, int di, int dj, int dk) const {
+        return lds[c * 3 * A0_PL + b[dk + 1] + ctr + di + dj * A0_SY];
+    }
+};
+
+template <int C, typename T>
+__device__ __forceinline__ T apply_lds(const A0View<T> &xv, const Row<T> &r, T &diag, T &x0) {
+    x0 = xv.get(C, 0, 0, 0);
+    T y = r.exc * x0;
+    diag = r.exc;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        diag += r.cond[q];
+        if (r.cond[q] != (T)0) y += r.cond[q] * (x0 - xv.get(C, so_i(q), so_j(q), so_k(q)));
+    }
+#pragma unroll
+    for (int s = 0; s < 8; ++s)
+        if (r.cross[s] != (T)0) y += r.cross[s] * xv.get(xo_c(C, s), xo_i(C, s), xo_j(C, s), xo_k(C, s));
+    return y;
+}
+
+template <int C, int MODE, typename T>
+__device__ __forceinline__ void a0_row(const Grid &g, const FineConst<T> &fk, const ZmTuple<T> &cf,
+                                       const A0View<T> &xv, int i, int j, int k, int64_t p, const T *bq,
+                                       T *__restrict__ out, T omega) {
+    const int64_t q = (int64_t)C * g.n1 + p;
+    if (!free_f(g, C, i, j, k)) { out[q] = (T)0; return; }
+    Row<T> r;
+    row_a_raw<C, T>(cf, fk, r);
+    eliminate<C, 3, T>(g, i, j, k, r);
+    T diag, x0;
+    const T y = apply_lds<C, T>(xv, r, diag, x0);
+    out[q] = MODE == 0 ? x0 + omega * (bq[C] - y) / diag : bq[C] - y;
+}
+
+// MODE 0: damped Jacobi sweep out = x + w (b - A x)/diag; MODE 1: residual
+// out = b - A x.  Inactive rows get 0.
+template <typename T, int MODE, bool CLS>
+__global__ void __launch_bounds__(256) k_a0_zm(A0Z<T> z, const T *__restrict__ x, const T *__restrict__ b,
+                                               T *__restrict__ out, T omega) {
+    __shared__ T lds[3 * 3 * A0_PL];
+    Grid g = z.g;
+    g.nx = __builtin_amdgcn_readfirstlane(z.g.nx);
+    g.ny = __builtin_amdgcn_readfirstlane(z.g.ny);
+    g.nz = __builtin_amdgcn_readfirstlane(z.g.nz);
+    g.n1 = (int64_t)g.nx * g.ny * g.nz;
+    const unsigned tid = xcd_tile(blockIdx.x, gridDim.x);
+    const unsigned tx = tid % (unsigned)z.zt.ntx, r1 = tid / (unsigned)z.zt.ntx;
+    const unsigned ty = r1 % (unsigned)z.zt.nty, tz = r1 / (unsigned)z.zt.nty;
+    const int kb = (int)tz * z.zt.kz, ke = min(kb + z.zt.kz, g.nz);
+    const int i0 = (int)tx * A0_TX, j0 = (int)ty * A0_TY;
+    const int li = (int)(threadIdx.x % A0_TX), lj = (int)(threadIdx.x / A0_TX);
+    const int i = i0 + li, j = j0 + lj;
+    const int ic = min(i, g.nx - 1), jc = min(j, g.ny - 1);
+    const bool live = i < g.nx && j < g.ny;
+    const int ctr = (lj + 1) * A0_SY + (li + 1);
+    const int64_t n1 = g.n1, plane = (int64_t)g.nx * g.ny;
+    auto stage = [&](int kp, int slot) {
+        kp = max(0, min(kp, g.nz - 1));
+        for (int e = (int)threadIdx.x; e < A0_PL; e += 256) {
+            const int ei = e % A0_SY, ej = e / A0_SY;
+            const int gi = max(0, min(i0 - 1 + ei, g.nx - 1)), gj = max(0, min(j0 - 1 + ej, g.ny - 1));
+            const int64_t q = gi + (int64_t)g.nx * gj + plane * kp;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) lds[(c * 3 + slot) * A0_PL + e] = x[c * n1 + q];
+        }
+    };
+    stage(kb - 1, 0);
+    stage(kb, 1);
+    const int64_t pc = ic + (int64_t)g.nx * jc;
+    const int dxm = ic > 0 ? -1 : 0;
+    const int64_t dym = jc > 0 ? -(int64_t)g.nx : 0;
+    ZmTuple<T> cf;
+    {
+        T sgd[3];
+        load_tuple<T, CLS>(z, pc + plane * max(kb - 1, 0), cf.imz, sgd);
+    }
+    for (int kk = kb; kk < ke; ++kk) {
+        const int r = kk - kb;
+        const int sm = r % 3, s0 = (r + 1) % 3, sp = (r + 2) % 3;
+        const int64_t p = pc + plane * kk;
+        T sgd[3];
+        load_tuple<T, CLS>(z, p, cf.im0, cf.sg0);
+        load_tuple<T, CLS>(z, p + dxm, cf.imx, sgd);
+        load_tuple<T, CLS>(z, p + dym, cf.imy, sgd);
+        T bq[3] = {(T)0, (T)0, (T)0};
+        if (live) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) bq[c] = b[c * n1 + p];
+        }
+        stage(kk + 1, sp);
+        __syncthreads();
+        if (live) {
+            const A0View<T> xv{lds, {sm * A0_PL, s0 * A0_PL, sp * A0_PL}, ctr};
+            a0_row<0, MODE, T>(g, z.fk, cf, xv, i, j, kk, p, bq, out, omega);
+            a0_row<1, MODE, T>(g, z.fk, cf, xv, i, j, kk, p, bq, out, omega);
+            a0_row<2, MODE, T>(g, z.fk, cf, xv, i, j, kk, p, bq, out, omega);
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) cf.imz[a] = cf.im0[a];
+        __syncthreads();
+    }
+}
+
 // bc = P^T r: the sum over the (up to) 2x2x2 children of each coarse node.
 template <typename T>
 __global__ void __launch_bounds__(256) k_mg_restrict(LevelDims fd, LevelDims cd, int ncomp, const T *__restrict__ r,
@@ -547,6 +701,17 @@ void mg_setup(const Grid &g, const HostCoef &hc, const CoeffsT<T> &dc, double dt
     h.fk.idt = (T)fkd.idt; h.fk.sigfloor = (T)fkd.sigfloor;
     h.g = g;
     for (int a = 0; a < 3; ++a) { h.im[a] = dc.im[a]; h.sg[a] = dc.sg[a]; }
+    h.cls = dc.cls;
+    h.ctab = dc.ctab;
+    {
+        // level-0 z-march: 32 x 8 columns, the deepest z-chunk of 1..16 planes
+        // that leaves at least 768 workgroups (three per CU)
+        const int64_t cols2d = (int64_t)((g.nx + A0_TX - 1) / A0_TX) * ((g.ny + A0_TY - 1) / A0_TY);
+        int kz = 1;
+        for (int k : {2, 4, 8, 16})
+            if (cols2d * ((g.nz + k - 1) / k) >= 768) kz = k;
+        h.zt = ztiling(g, kz, A0_TX, h.zcols);
+    }
     // level 0 work vectors
     LevelDims d0{g.nx, g.ny, g.nz, g.n1};
     h.lev[0].d = d0;
@@ -667,8 +832,28 @@ static void on_level(const Hierarchy<T> &h, int l, F f) {
 }
 
 template <typename T>
+static A0Z<T> a0z(const Hierarchy<T> &h) {
+    A0Z<T> z;
+    z.g = h.g;
+    z.zt = h.zt;
+    for (int a = 0; a < 3; ++a) { z.im[a] = h.im[a]; z.sg[a] = h.sg[a]; }
+    z.cls = h.cls;
+    z.ctab = h.ctab;
+    z.fk = h.fk;
+    return z;
+}
+
+// Level 0 of the A block, z-march form (MODE 0: sweep, 1: residual).
+template <typename T, int MODE>
+static void a0_launch(const Hierarchy<T> &h, const T *x, const T *b, T *out, T w, hipStream_t s) {
+    if (h.cls) hipLaunchKernelGGL((k_a0_zm<T, MODE, true>), dim3(h.zcols), dim3(256), 0, s, a0z(h), x, b, out, w);
+    else hipLaunchKernelGGL((k_a0_zm<T, MODE, false>), dim3(h.zcols), dim3(256), 0, s, a0z(h), x, b, out, w);
+}
+
+template <typename T>
 static void smooth(const Hierarchy<T> &h, int l, const T *x, const T *b, T *xo, T w, bool first, hipStream_t s) {
     const int64_t n1 = h.lev[l].d.n1;
+    if (l == 0 && h.block == 0 && !first) { a0_launch<T, 0>(h, x, b, xo, w, s); return; }
     on_level(h, l, [&](auto pv, auto nc) {
         using P = decltype(pv);
         constexpr int NC = decltype(nc)::value;
@@ -679,6 +864,7 @@ static void smooth(const Hierarchy<T> &h, int l, const T *x, const T *b, T *xo, 
 template <typename T>
 static void resid(const Hierarchy<T> &h, int l, const T *x, const T *b, T *r, hipStream_t s) {
     const int64_t n1 = h.lev[l].d.n1;
+    if (l == 0 && h.block == 0) { a0_launch<T, 1>(h, x, b, r, (T)0, s); return; }
     on_level(h, l, [&](auto pv, auto nc) {
         using P = decltype(pv);
         constexpr int NC = decltype(nc)::value;

@@ -408,10 +408,6 @@ struct Tiling {
     int ntx, nty;       // tiles per row / per plane column
 };
 
-__device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned nb) {
-    const unsigned q = nb >> 3, r = nb & 7u, x = b & 7u;
-    return x * q + min(x, r) + (b >> 3);
-}
 
 __device__ __forceinline__ bool tile_node(const Grid &g, const Tiling &t, int &i, int &j, int &k) {
     const unsigned tid = xcd_tile(blockIdx.x, gridDim.x);

@@ -256,10 +256,12 @@ template <> PrecondState<float> &pstate<float>(qms_handle *h) { return h->p32; }
 template <> PrecondState<double> &pstate<double>(qms_handle *h) { return h->p64; }
 template <typename T> CoeffsT<T> dev_coef(const qms_handle *h);
 template <> CoeffsT<float> dev_coef<float>(const qms_handle *h) {
-    return CoeffsT<float>{{h->c32.im[0], h->c32.im[1], h->c32.im[2]}, {h->c32.sg[0], h->c32.sg[1], h->c32.sg[2]}};
+    return CoeffsT<float>{{h->c32.im[0], h->c32.im[1], h->c32.im[2]}, {h->c32.sg[0], h->c32.sg[1], h->c32.sg[2]},
+                          h->use_cls ? h->cls : nullptr, h->use_cls ? h->ctab : nullptr};
 }
 template <> CoeffsT<double> dev_coef<double>(const qms_handle *h) {
-    return CoeffsT<double>{{h->c64.im[0], h->c64.im[1], h->c64.im[2]}, {h->c64.sg[0], h->c64.sg[1], h->c64.sg[2]}};
+    return CoeffsT<double>{{h->c64.im[0], h->c64.im[1], h->c64.im[2]}, {h->c64.sg[0], h->c64.sg[1], h->c64.sg[2]},
+                           h->use_cls ? h->cls : nullptr, h->use_cls ? h->ctab : nullptr};
 }
 
 void launch_apply_t(qms_handle *h, const OpParams &op, const float *x, double *y) {
@@ -870,6 +872,17 @@ int qms_gram_schmidt(qms_handle *h, int nv, const double *basis, const double *w
     memcpy(hcoef, h->hsmall, sizeof(double) * (nv + 1));
     return QMS_OK;
     QMS_CATCH(h)
+}
+
+int qms_get_coeffs(qms_handle *h, double *out) {
+    if (!h || !out) return QMS_ERR_ARG;
+    const int64_t n1 = h->g.n1;
+    for (int q = 0; q < 6; ++q)
+        for (int64_t p = 0; p < n1; ++p) {
+            const double c = h->hcoef[(size_t)q * n1 + p];
+            out[q * n1 + p] = q < 3 ? 1.0 / c : c;
+        }
+    return QMS_OK;
 }
 
 int qms_get_work(qms_handle *h, double *out) {

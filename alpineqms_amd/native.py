@@ -26,7 +26,7 @@ QMS_MIXED = 1
 # Every symbol include/qms.h declares.
 EXPORTS = (
     "qms_config_default", "qms_create", "qms_steady", "qms_step", "qms_run",
-    "qms_get_fields", "qms_set_fields", "qms_get_h", "qms_apply", "qms_residual", "qms_rhs",
+    "qms_get_fields", "qms_set_fields", "qms_get_h", "qms_get_coeffs", "qms_apply", "qms_residual", "qms_rhs",
     "qms_get_stats", "qms_set_zchunk", "qms_set_colwidth", "qms_use_coef_classes", "qms_time_kernel", "qms_gram_schmidt", "qms_get_work", "qms_destroy", "qms_last_error",
     "qms_batch_create", "qms_batch_set_blocks", "qms_batch_apply", "qms_batch_tune", "qms_batch_time",
     "qms_batch_plan", "qms_batch_last_error", "qms_batch_destroy",
@@ -104,6 +104,7 @@ def load(path: Optional[str] = None):
     L.qms_get_fields.argtypes = [H, d]
     L.qms_set_fields.argtypes = [H, d]
     L.qms_get_h.argtypes = [H, ctypes.c_int, d]
+    L.qms_get_coeffs.argtypes = [H, d]
     L.qms_apply.argtypes = [H, d, ctypes.c_double, d, d]
     L.qms_rhs.argtypes = [H, d, ctypes.c_double, d, d]
     L.qms_get_stats.argtypes = [H, ctypes.POINTER(Stats)]
@@ -248,6 +249,13 @@ class Solver:
         cx, cy, cz = self.cells
         out = np.empty((3, cz, cy, cx), dtype=np.float64)
         self._check(self._L.qms_get_h(self._h, 1 if unit_mu else 0, _dp(out)))
+        return out
+
+    def get_coeffs(self) -> np.ndarray:
+        """(6, Nz, Ny, Nx): Mu_x, Mu_y, Mu_z, Sigma_x, Sigma_y, Sigma_z."""
+        nx, ny, nz = self.nodes
+        out = np.empty((6, nz, ny, nx), dtype=np.float64)
+        self._check(self._L.qms_get_coeffs(self._h, _dp(out)))
         return out
 
     def apply(self, v, dt: float, x: np.ndarray) -> np.ndarray:

@@ -1,44 +1,42 @@
 #!/usr/bin/env python3
 """Benchmark of the QMS_3D time-stepping core on one MI355X.
 
-Workload: the 128^3-node design of the benchmark set (config 3: 0.25 mm
-cells, 24 random boxes plus a 40x40x8-cell conducting slab, frame velocity
-ramped to 5 m/s over 200 steps of 2 us), the configuration the headline
-metric "A-V time steps/s at 128^3" is quoted on.
+Workload (the configuration the headline "A-V time steps/s at 128^3" is
+quoted on): config 3, 128^3 nodes (8,339,456 DoF), 0.25 mm cells, 24 random
+boxes plus a 40x40x8-cell conducting slab, frame velocity ramped 0 -> 5 m/s
+over 200 steps of 2 us (the reference's constant-acceleration trajectory,
+QMS_3D.f90:2331-2350; with a constant velocity every step reproduces the
+steady state, so the ramp is the variant whose steps carry work).
 
-Printed: one JSON line.  The time-step rate needs the multigrid solve path,
-which is disabled in the library for now (it faulted the GPU during its first
-hierarchy build, cause not identified), so `value` is null and says so.
-What is measured on the GPU: the fused A-V operator kernel at 128^3 (the
-matrix-free replacement of the reference's per-step CSR assembly and the
-operator inside every Krylov iteration) with the rows of the last ramp step
-(transient, v = 5 m/s), against the HBM roofline, with HIP events on the
-stream it is launched on; its column shape and z-chunk depth (planes per
-workgroup) are picked from a sweep.  Beside it: the FP64 residual, the step right-hand side, the
-cell-centred H, a plain copy kernel of the same state (calibration) and
-the FGMRES Gram-Schmidt step against 31 basis vectors (multi-dot,
-reduction, update, norm).
-Every kernel figure is timed with cold caches, as inside a solve, where the
-Krylov basis streams through the caches between two launches of a kernel:
-back-to-back launches rotate through copies of the kernel's arrays, enough
-that 640 MB of other arrays (more than the L2s and the 256 MB Infinity Cache
-hold) pass between two uses of one copy.  Back-to-back launches (reported beside,
-as `warm_repeated`) can keep a working set below 256 MB, such as the apply's
-142 MB at 128^3, in the Infinity Cache and run faster than HBM allows.
-The `batch` object measures the config-4 workload (64 uneven designs): the
-operator of every design in one persistent batched launch against one
-launch per design.  The CPU baseline is the oracle (CSR assembly +
-equilibrated SuperLU factor/solve, the restated reference algorithm) timed
-on one core over ramp steps 1-3 of a 24^3-node design drawn from the same
-box distribution.  `operator_sizes` repeats the apply at 32^3, 64^3, 96^3 and
-160^3 nodes, each at its fastest column shape.
+Per step (QMS_3D.f90:1424-2234 restated on the GPU): right-hand side,
+Dirichlet values, extrapolated initial guess, FGMRES to a scaled relative
+residual of 1e-12 (the tolerance at which sigma*grad(V) agrees with the FP64
+path to about 1e-8; at 1e-10 it agrees to only 1e-6) with the block-triangular
+multigrid preconditioner, every iteration a replayed hipGraph.  Fields stay on the device between steps.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--zchunk KZ --colwidth CW] [--no-cpu-baseline]
-       [--no-batch] [--no-sizes] [--only-batch]
+Timed: `--warmup` untimed steps after the steady state, then `--steps` steps,
+each a blocking call; `ms_per_step` is the mean of the timed steps.  The
+mixed-precision path (FP32 A-block multigrid and Z basis, FP64 V block,
+FP64 Krylov basis, residuals and fields) is `value`; the plain FP64 path
+runs the same steps beside it and every timed step's A, B and sigma*grad(V)
+are compared with it.
+
+`roofline` is the dominant kernel of the mixed step, the level-0 A-block
+smoother sweep, timed with HIP events on the handle's stream; `kernels` holds
+the other kernels of the step at 128^3 (cold-cache operator figures as
+defined in DESIGN.md).  `cpu_baseline` is the oracle (the C restatement of
+the reference's CSR assembly with SciPy's SuperLU in place of PARDISO) on one
+core: config 1's design at 24^3 nodes (its 32^3 factorisation takes over two
+minutes on one core), steady factorisation then steps that reuse it, as the
+reference's phase 33 does for a constant velocity; the GPU runs the same
+design and steps and its errors against the port are reported there.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--nodes N] [--no-cpu-baseline] [--no-batch]
 """
 import argparse
 import json
 import os
+import shutil
 import sys
 import time
 
@@ -46,42 +44,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TOL = 1e-12
 
 
-def cpu_baseline():
-    import numpy as np
-    from oracle import oracle as O
-    from alpineqms_amd import configs
-    from tests.helpers import problem
-    d = configs.config2(N=24, steps=10, ramp_v=5.0)
-    p = problem(d)
-    coef = O.coeffs(p)
-    M, b = O.assemble(p, coef, O.steady_velocity(p))
-    f = O.unpack(p, O.DirectSolver(M).solve(b))
-    nsteps = 3
-    t0 = time.perf_counter()
-    for n in range(1, nsteps + 1):
-        v = O.step_velocity(p, n)
-        M, b = O.assemble(p, coef, v, p.dt, f)      # per-step CSR rebuild (QMS_3D.f90:1452-2191)
-        x = O.DirectSolver(M).solve(b)              # factor + solve (phase 23: the velocity changed)
-        f = O.unpack(p, x)
-    t = (time.perf_counter() - t0) / nsteps
-    return {"value": 1.0 / t, "unit": "steps/s", "cores": 1, "kind": "port",
-            "sample": f"ramp steps 1-3 of a 24^3-node design ({d.dof} DoF, 8 boxes, v ramp to 5 m/s): "
-                      f"CSR assembly + row-equilibrated SuperLU factor and solve, {t:.2f} s per step",
-            "dof_steps_per_s": d.dof / t}
-
-
-SHAPES = ((64, 4), (64, 6), (64, 8), (32, 4), (32, 6), (32, 8), (32, 12), (32, 16))   # (column width, z-chunk)
-
-
-def traffic_from_profile(kernel: str, shape: str):
-    """HBM bytes per launch of `kernel` at column shape `shape` ("32x8": column
-    width x z-chunk) from the committed rocprofv3 PMC summary (FETCH_SIZE and
-    WRITE_SIZE per dispatch, each scaled by what the counter reads for the
-    plain copy kernel of known size), or None when no summary of that shape
-    is committed."""
-    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{shape}.txt")
+def traffic_from_profile(name: str):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 PMC
+    summary profiles/traffic_<name>.txt (FETCH_SIZE and WRITE_SIZE per
+    dispatch, each scaled by what the counter reads for a plain copy kernel of
+    known size), or (None, None) when no such summary is committed."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{name}.txt")
     if not os.path.exists(path):
         return None, None
     kv = {}
@@ -96,213 +67,234 @@ def traffic_from_profile(kernel: str, shape: str):
     return t * 1024.0, os.path.relpath(path, ROOT)
 
 
+def make_solver(d, precision, **kw):
+    from alpineqms_amd import native
+    from alpineqms_amd.qms_io import material_slots
+    inp = d.inputs
+    slots, props = material_slots(inp, d.geom)
+    return native.Solver(inp.cells, inp.h, inp.dt, inp.bc_vector(), slots, props, precision=precision,
+                         tol=kw.pop("tol", TOL), **kw)
+
+
+def state_errors(d, coef, f, fr, B, Br):
+    from alpineqms_amd.fields import a_part, rel, sigma_grad_v
+    h = d.inputs.h
+    return {"A": rel(a_part(f), a_part(fr)), "B": rel(B, Br),
+            "sigma_grad_V": rel(sigma_grad_v(h, coef, f), sigma_grad_v(h, coef, fr))}
+
+
+def reference_binary_probe():
+    """The reference binary needs Intel ifx (or another Fortran compiler) and
+    oneMKL's PARDISO (3D/compile.sh:4-12); report what this host has."""
+    import ctypes.util
+    found = {c: shutil.which(c) for c in ("ifx", "ifort", "gfortran", "flang")}
+    mkl = ctypes.util.find_library("mkl_rt")
+    ok = bool(mkl) and any(found[c] for c in ("ifx", "ifort", "gfortran"))
+    return {"available": ok, "compilers": {k: v for k, v in found.items() if v}, "libmkl_rt": mkl,
+            "note": None if ok else "no Fortran compiler with oneMKL PARDISO on this host: the reference binary "
+                                    "cannot be built, so the baseline is the oracle port"}
+
+
+def cpu_baseline(nsteps: int = 10):
+    """The oracle port on one core, config 1's design at 24^3 nodes, and the
+    GPU paths on the same design and steps checked against it."""
+    import numpy as np
+    from oracle import oracle as O
+    from alpineqms_amd import configs, native, program
+    from alpineqms_amd.fields import a_part, rel
+    from tests.helpers import problem
+    d = configs.config1(24, steps=nsteps)
+    p = problem(d)
+    coef = O.coeffs(p)
+    t0 = time.perf_counter()
+    M, b = O.assemble(p, coef, O.steady_velocity(p))
+    S = O.DirectSolver(M)
+    f = O.unpack(p, S.solve(b))
+    t_factor = time.perf_counter() - t0
+    ref = [f]
+    # constant frame velocity: the factorisation is reused, as phase 33 does
+    v = O.step_velocity(p, 1)
+    M, b = O.assemble(p, coef, v, p.dt, f)
+    S = O.DirectSolver(M)
+    t1 = time.perf_counter()
+    for n in range(1, nsteps + 1):
+        M, b = O.assemble(p, coef, O.step_velocity(p, n), p.dt, f)    # per-step CSR rebuild (:1452-2191)
+        S.M = M.tocsr()
+        f = O.unpack(p, S.solve(b))                                   # triangular solves + 2 refinement steps
+        ref.append(f)
+    t_step = (time.perf_counter() - t1) / nsteps
+    out = {"value": 1.0 / t_step, "unit": "steps/s", "cores": 1, "kind": "port",
+           "sample": f"config 1 design at 24^3 nodes ({d.dof} DoF; 31^3 cells scaled to 23^3, 12^3 block), "
+                     f"steady factorisation {t_factor:.1f} s, then {nsteps} steps of CSR assembly + "
+                     f"triangular solves with the step-1 factorisation ({t_step:.3f} s per step)",
+           "dof_steps_per_s": d.dof / t_step, "factorisation_s": t_factor,
+           "reference_binary": reference_binary_probe()}
+    errs = {}
+    for name, prec in (("fp64", native.QMS_FP64), ("mixed", native.QMS_MIXED)):
+        with make_solver(d, prec, tol=1e-12) as s:
+            s.steady(program.steady_velocity(d.inputs.traj, d.inputs.dt))
+            e = []
+            for n in range(1, nsteps + 1):
+                s.step(program.step_velocity(d.inputs.traj, d.inputs.dt, n))
+                g = s.get_fields()
+                e.append(max(rel(a_part(g), a_part(ref[n])),
+                             rel(O.bfield(p, g), O.bfield(p, ref[n]))))
+            errs[name] = max(e)
+    out["gpu_max_rel_error_vs_port_A_B"] = errs
+    return out
+
+
+def smoother_roofline(s, mixed: bool, reps: int):
+    from alpineqms_amd import native
+    s.time_kernel(native.K_SMOOTH_A, 3)
+    ms, nb = s.time_kernel(native.K_SMOOTH_A, reps)
+    gbs = nb / (ms * 1e-3) / 1e9
+    return ms, nb, gbs
+
+
 def batch_config4(reps: int, ndesigns: int = 64):
-    """Config 4: 64 independent designs (24..96 nodes per axis, 4-12 boxes
-    each), operator rows of a transient step at a random frame velocity per
-    design, applied in one persistent batched launch (z-chunks and grid size
-    from the autotuner) and, for comparison, as one launch per design."""
+    """Config 4's 64 designs: the operator of every design in one persistent
+    batched launch against one launch per design."""
     import numpy as np
     from alpineqms_amd import configs, native
-    from alpineqms_amd.qms_io import material_slots
     ds = configs.config4_designs(ndesigns)
     rng = np.random.default_rng(4)
     ss = []
     for d in ds:
-        slots, props = material_slots(d.inputs, d.geom)
-        s = native.Solver(d.inputs.cells, d.inputs.h, d.inputs.dt, d.inputs.bc_vector(), slots, props, restart=1)
+        s = make_solver(d, native.QMS_FP64, restart=1)
         nx, ny, nz = s.nodes
         s.set_fields(rng.normal(size=(4, nz, ny, nx)))
         ss.append(s)
     v = rng.normal(size=(len(ss), 3))
     dof = sum(d.dof for d in ds)
-    print(f"bench: config 4 batch of {len(ds)} designs, {dof} DoF", file=sys.stderr, flush=True)
     with native.Batch(ss) as b:
-        seq0, _ = b.time(v, reps=reps, sequential=True)          # default shape, one launch per design
-        # best shape of each design on its own, then one launch per design
-        for s_, vd in zip(ss, v):
-            best = None
-            for cw_, kz_ in SHAPES + ((64, 1), (64, 2), (32, 1), (32, 2)):
-                s_.set_colwidth(cw_)
-                s_.set_zchunk(kz_)
-                s_.time_kernel(native.K_APPLY, 2, v=vd)
-                t = s_.time_kernel(native.K_APPLY, 5, v=vd)[0]
-                if best is None or t < best[0]:
-                    best = (t, cw_, kz_)
-            s_.set_colwidth(best[1])
-            s_.set_zchunk(best[2])
-        ms_s, _ = b.time(v, reps=reps, sequential=True)
+        seq0, _ = b.time(v, reps=reps, sequential=True)
         kz, cw, blocks = b.tune(v, reps=10)
         ms_b, nbytes = b.time(v, reps=reps)
     for s in ss:
         s.close()
     gbs = nbytes / (ms_b * 1e-3) / 1e9
-    return {"workload": f"config 4: {len(ds)} designs, 24-96 nodes per axis, {dof} DoF, transient rows",
-            "dof": dof, "algorithmic_bytes": nbytes, "zchunks": kz, "colwidths": cw, "blocks": blocks,
-            "batched_ms": ms_b, "sequential_tuned_ms": ms_s, "sequential_default_ms": seq0,
-            "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS, "dof_applies_per_s": dof / (ms_b * 1e-3),
-            "speedup_vs_sequential": min(ms_s, seq0) / ms_b}
-
-
-def size_scan(reps: int, sizes=(32, 64, 96, 160)):
-    """The same apply (config-3 box layout and v = 5 m/s rows, spacing scaled
-    to the size) at other node counts per axis: config 1/2 sizes and the
-    largest served size, each at the fastest column shape of the sweep (a
-    small design alone cannot fill the GPU: 32^3 nodes are 128 columns of
-    32 x 8 nodes even one plane deep).  Algorithmic bytes and timing (cold
-    caches, warm beside) as for the headline."""
-    import numpy as np
-    from alpineqms_amd import configs, native, program
-    from alpineqms_amd.qms_io import material_slots
-    out = {}
-    for n in sizes:
-        d = configs.config3(N=n, ramp=True)
-        inp = d.inputs
-        slots, props = material_slots(inp, d.geom)
-        s = native.Solver(inp.cells, inp.h, inp.dt, inp.bc_vector(), slots, props, restart=1)
-        nx, ny, nz = d.nodes
-        s.set_fields(np.random.default_rng(n).normal(size=(4, nz, ny, nx)))
-        v = program.step_velocity(inp.traj, inp.dt, inp.steps)
-        best = None
-        for cw, kz in SHAPES + ((64, 1), (64, 2), (32, 1), (32, 2)):
-            s.set_colwidth(cw)
-            s.set_zchunk(kz)
-            s.time_kernel(native.K_APPLY, 2, v=v)
-            ms = s.time_kernel(native.K_APPLY | native.K_COLD, 5, v=v)[0]
-            if best is None or ms < best[0]:
-                best = (ms, cw, kz)
-        s.set_colwidth(best[1])
-        s.set_zchunk(best[2])
-        s.time_kernel(native.K_APPLY, 3, v=v)
-        ms_warm, nbytes = s.time_kernel(native.K_APPLY, reps, v=v)
-        ms, _ = s.time_kernel(native.K_APPLY | native.K_COLD, reps, v=v)
-        s.close()
-        gbs = nbytes / (ms * 1e-3) / 1e9
-        out[str(n)] = {"dof": d.dof, "column": best[1], "zchunk": best[2], "ms_per_launch": ms,
-                       "algorithmic_bytes": nbytes, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS,
-                       "dof_applies_per_s": d.dof / (ms * 1e-3), "warm_ms_per_launch": ms_warm,
-                       "warm_frac": nbytes / (ms_warm * 1e-3) / 1e9 / HBM_PEAK_GBS}
-        print(f"bench: apply at {n}^3 nodes (column {best[1]}, z-chunk {best[2]}): {ms * 1e3:.1f} us cold "
-              f"({gbs / HBM_PEAK_GBS:.2f} of HBM peak), {ms_warm * 1e3:.1f} us warm", file=sys.stderr, flush=True)
-    return out
+    return {"workload": f"config 4: {len(ds)} designs, 24-96 nodes per axis, {dof} DoF, transient operator rows",
+            "dof": dof, "batched_ms": ms_b, "sequential_default_ms": seq0, "achieved_GBs": gbs,
+            "frac": gbs / HBM_PEAK_GBS, "speedup_vs_sequential": seq0 / ms_b, "blocks": blocks}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--nodes", type=int, default=128)
-    ap.add_argument("--zchunk", type=int, default=0,
-                    help="planes per operator workgroup (0: sweep column shapes and z-chunks)")
-    ap.add_argument("--colwidth", type=int, default=32, help="operator column width with --zchunk (32 or 64)")
+    ap.add_argument("--tol", type=float, default=TOL, help="scaled relative residual of every solve")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch each FGMRES iteration's kernels one by one instead of replaying captured graphs "
+                         "(the same kernels and results; for profilers that do not follow graph launches)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-batch", action="store_true", help="skip the config-4 batched operator measurement")
-    ap.add_argument("--only-batch", action="store_true", help="print only the config-4 batched measurement")
-    ap.add_argument("--no-sizes", action="store_true", help="skip the apply at other node counts")
+    ap.add_argument("--no-batch", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py runs on one GPU only: every design here fits one MI355X, and the multi-GPU "
-              "path (whole designs per GPU, no inter-GPU traffic) is not implemented", file=sys.stderr)
+        print("bench.py runs on one GPU only: every design of the benchmark set fits one MI355X, and the "
+              "multi-GPU path (whole designs per GPU) is not built", file=sys.stderr)
         return 2
-    import numpy as np
     from alpineqms_amd import configs, native, program
-    from alpineqms_amd.qms_io import material_slots
-
-    if a.only_batch:
-        print(json.dumps({"batch": batch_config4(a.steps)}), flush=True)
-        return 0
     d = configs.config3(N=a.nodes, ramp=True)
     inp = d.inputs
-    slots, props = material_slots(inp, d.geom)
-    print(f"bench: {d.name} nodes={d.nodes} dof={d.dof}", file=sys.stderr, flush=True)
-    s = native.Solver(inp.cells, inp.h, inp.dt, inp.bc_vector(), slots, props, precision=native.QMS_FP64)
     nx, ny, nz = d.nodes
-    rng = np.random.default_rng(0)
-    s.set_fields(rng.normal(size=(4, nz, ny, nx)))
-    # rows of the last ramp step: transient, frame velocity 5 m/s (advection terms on)
-    v = program.step_velocity(inp.traj, inp.dt, inp.steps)
-    w = max(a.warmup, 1)
-
-    def timed(which):
-        """(cold ms, bytes, warm ms) of one kernel: cold caches before every
-        launch, then back-to-back launches."""
-        s.time_kernel(which, w, v=v)
-        warm, nb = s.time_kernel(which, a.steps, v=v)
-        cold, _ = s.time_kernel(which | native.K_COLD, a.steps, v=v)
-        return cold, nb, warm
-
-    def rate(ms, nb):
+    print(f"bench: {d.name} nodes={d.nodes} dof={d.dof}", file=sys.stderr, flush=True)
+    runs = {}
+    solvers = {}
+    for name, prec in (("fp64", native.QMS_FP64), ("mixed", native.QMS_MIXED)):
+        s = make_solver(d, prec, tol=a.tol, use_graphs=not a.no_graphs)
+        t0 = time.perf_counter()
+        s.steady(program.steady_velocity(inp.traj, inp.dt))
+        st = s.stats()
+        print(f"bench: {name} steady state {time.perf_counter() - t0:.2f} s, {st.last_iters} iterations",
+              file=sys.stderr, flush=True)
+        solvers[name] = s
+        runs[name] = {"steady_iters": st.last_iters, "step_ms": [], "iters": []}
+    coef = solvers["fp64"].get_coeffs()
+    errors = {"A": 0.0, "B": 0.0, "sigma_grad_V": 0.0}
+    for n in range(1, a.warmup + a.steps + 1):
+        v = program.step_velocity(inp.traj, inp.dt, n)
+        for name, s in solvers.items():
+            q0 = s.stats().total_iters
+            t0 = time.perf_counter()
+            s.step(v)
+            t = time.perf_counter() - t0
+            if n > a.warmup:
+                runs[name]["step_ms"].append(t * 1e3)
+                runs[name]["iters"].append(int(s.stats().total_iters - q0))
+        if n > a.warmup:
+            f64, fmx = solvers["fp64"].get_fields(), solvers["mixed"].get_fields()
+            B64, Bmx = solvers["fp64"].get_h(unit_mu=True), solvers["mixed"].get_h(unit_mu=True)
+            e = state_errors(d, coef, fmx, f64, Bmx, B64)
+            for k in errors:
+                errors[k] = max(errors[k], e[k])
+            print(f"bench: step {n}: fp64 {runs['fp64']['step_ms'][-1]:.2f} ms ({runs['fp64']['iters'][-1]} it), "
+                  f"mixed {runs['mixed']['step_ms'][-1]:.2f} ms ({runs['mixed']['iters'][-1]} it), "
+                  f"mixed vs fp64: A {e['A']:.1e} B {e['B']:.1e} sgV {e['sigma_grad_V']:.1e}",
+                  file=sys.stderr, flush=True)
+    for name in runs:
+        r = runs[name]
+        r["ms_per_step"] = sum(r["step_ms"]) / len(r["step_ms"])
+        r["steps_per_s"] = 1e3 / r["ms_per_step"]
+        r["dof_steps_per_s"] = d.dof * r["steps_per_s"]
+        r["mean_iters"] = sum(r["iters"]) / len(r["iters"])
+        del r["step_ms"]
+    # kernels of the step (mixed handle for the smoother, FP64 for the operator figures)
+    smx = solvers["mixed"]
+    ms_s, nb_s, gbs_s = smoother_roofline(smx, True, 50)
+    s64 = solvers["fp64"]
+    ms_s64, nb_s64, gbs_s64 = smoother_roofline(s64, False, 50)
+    vlast = program.step_velocity(inp.traj, inp.dt, a.warmup + a.steps)
+    kernels = {"smooth_A_fp64": {"ms_per_launch": ms_s64, "algorithmic_bytes": nb_s64, "achieved_GBs": gbs_s64,
+                                 "frac": gbs_s64 / HBM_PEAK_GBS}}
+    for kname, which in (("operator_apply", native.K_APPLY), ("operator_resid", native.K_RESID),
+                         ("rhs", native.K_RHS), ("hfield", native.K_HFIELD), ("copy", native.K_COPY),
+                         ("smooth_V_fp64", native.K_SMOOTH_V), ("gram_schmidt_31", native.K_GRAM_SCHMIDT)):
+        sel = which | (native.K_COLD if which in (native.K_APPLY, native.K_RESID, native.K_RHS, native.K_HFIELD,
+                                                   native.K_COPY) else 0)
+        s64.time_kernel(sel, 2, v=vlast)
+        ms, nb = s64.time_kernel(sel, 20, v=vlast)
         gbs = nb / (ms * 1e-3) / 1e9
-        return {"ms_per_launch": ms, "algorithmic_bytes": nb, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
-
-    sweep = {}
-    shapes = ((a.colwidth, a.zchunk),) if a.zchunk else SHAPES
-    for cw, kz in shapes:
-        s.set_colwidth(cw)
-        s.set_zchunk(kz)
-        sweep[(cw, kz)] = timed(native.K_APPLY)
-        print(f"bench: operator column {cw}, z-chunk {kz}: {sweep[(cw, kz)][0] * 1e3:.1f} us cold, "
-              f"{sweep[(cw, kz)][2] * 1e3:.1f} us warm", file=sys.stderr, flush=True)
-    cw, kz = min(sweep, key=lambda k: sweep[k][0])
-    s.set_colwidth(cw)
-    s.set_zchunk(kz)
-    ms, nbytes, ms_warm = timed(native.K_APPLY)
-    achieved = nbytes / (ms * 1e-3) / 1e9
-    ncls = s.stats().coef_classes
-    # the same apply with the six coefficient arrays streamed from HBM
-    s.use_coef_classes(False)
-    ms_st, bytes_st, ms_st_warm = timed(native.K_APPLY)
-    s.use_coef_classes(True)
-    others = {}
-    for name, which in (("k_operator_resid", native.K_RESID), ("k_rhs", native.K_RHS),
-                        ("k_hfield", native.K_HFIELD), ("k_copy", native.K_COPY),
-                        ("gram_schmidt_31", native.K_GRAM_SCHMIDT)):
-        mk, bk, mw = timed(which)
-        others[name] = dict(rate(mk, bk), warm_repeated=rate(mw, bk))
-    status = None
-    try:
-        s.steady(np.zeros(3))
-    except native.QmsError as e:
-        status = str(e)
-    s.close()
-    traffic, tsrc = traffic_from_profile("k_operator_apply_config3_128" + ("_classes" if ncls else ""), f"{cw}x{kz}")
+        kernels[kname] = {"ms_per_launch": ms, "algorithmic_bytes": nb, "achieved_GBs": gbs,
+                          "frac": gbs / HBM_PEAK_GBS, "cold_caches": bool(sel & native.K_COLD)}
+    for s in solvers.values():
+        s.close()
+    mx, fp = runs["mixed"], runs["fp64"]
+    traffic_s, traffic_src = traffic_from_profile(f"k_mg_smooth_A_fp32_config3_{a.nodes}")
     line = {
         "metric": "A-V time steps/s at 128^3, mixed vs FP64 @1e-8 rel err; served p50/p99 latency",
-        "value": None,
+        "value": mx["steps_per_s"],
         "unit": "steps/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": None,
+        "ms_per_step": mx["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64",
         "data": "synthetic",
-        "config": {"workload": f"config 3: {nx}x{ny}x{nz} nodes, 0.25 mm, 24 boxes + slab, v ramp 0->5 m/s "
-                               f"(rows of the last ramp step: transient, v = {v[0]:g} m/s)",
+        "config": {"workload": f"config 3: {nx}x{ny}x{nz} nodes, 0.25 mm, 24 boxes + 40x40x8 slab, frame velocity "
+                               f"ramp 0 -> 5 m/s over 200 steps of 2 us, steps {a.warmup + 1}-"
+                               f"{a.warmup + a.steps}, FGMRES(30) to {a.tol:g} scaled residual",
                    "dof": d.dof},
-        "status": "time-step rate not measured: " + (status or "solve path unavailable"),
-        "roofline": {"kernel": f"k_operator (fused A-V operator apply, FP64, LDS z-march, column {cw}, z-chunk {kz}"
-                               f"{', coefficient classes' if ncls else ''})",
-                     "bound": "hbm",
-                     "algorithmic_bytes_definition": "8*(2n + 6*N^3): x and y (n = 4N^3 doubles each) and the six "
-                                                     "FP64 coefficient arrays of the reference operator",
-                     "kernel_stream_bytes": (16.0 * 4 * nx * ny * nz + 2.0 * nx * ny * nz + 48.0 * ncls) if ncls
-                     else nbytes,
-                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": tsrc,
-                     "algorithmic_bytes": nbytes, "ms_per_launch": ms, "launches": a.steps,
-                     "caches": "cold: back-to-back launches rotate through copies of x and y, 640 MB of "
-                               "other arrays between two uses of one copy (L2s + 256 MB Infinity Cache evicted)",
-                     "warm_repeated": dict(rate(ms_warm, nbytes), launches=a.steps)},
-        "shape_sweep_ms": {f"{c}x{z}": m[0] for (c, z), m in sweep.items()},
-        "shape_sweep_warm_ms": {f"{c}x{z}": m[2] for (c, z), m in sweep.items()},
-        "coefficient_classes": ncls,
-        "operator_streamed_coefficients": dict(rate(ms_st, bytes_st), warm_repeated=rate(ms_st_warm, bytes_st)),
-        "kernels": others,
-        "operator_sizes": None if a.no_sizes else size_scan(a.steps),
-        "batch": None if a.no_batch else batch_config4(a.steps),
+        "paths": {"mixed": mx, "fp64": fp, "mixed_over_fp64": mx["steps_per_s"] / fp["steps_per_s"]},
+        "dof_steps_per_s": mx["dof_steps_per_s"],
+        "max_rel_error_mixed_vs_fp64": errors,
+        "roofline": {"kernel": "k_mg_smooth (level-0 A-block damped Jacobi sweep, FP32, 3 components per node, "
+                               "rows from the Yee coefficients on the fly)",
+                     "bound": "hbm", "achieved": gbs_s, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": gbs_s / HBM_PEAK_GBS, "traffic": traffic_s, "traffic_source": traffic_src,
+                     "algorithmic_bytes": nb_s,
+                     "algorithmic_bytes_definition": "4*(9*N^3 + 6*N^3): x, b and x_out (3 components each) and "
+                                                     "the six FP32 coefficient arrays",
+                     "ms_per_launch": ms_s, "launches": 50,
+                     "caches": "back to back on the handle's arrays: the sweeps of a V-cycle reuse the same b and "
+                               "coefficients, so this is how they run inside a step"},
+        "kernels": kernels,
+        "batch": None if a.no_batch else batch_config4(20),
         "cpu_baseline": None if a.no_cpu_baseline else cpu_baseline(),
     }
     print(json.dumps(line), flush=True)

@@ -107,6 +107,11 @@ int qms_set_fields(qms_handle *h, const double *in);
  * 3 arrays of cells_x*cells_y*cells_z (Hx, Hy, Hz). */
 int qms_get_h(qms_handle *h, int unit_mu, double *out);
 
+/* The Yee material coefficients the design's rows use, as 6 node-sized arrays
+ * (Mu_x, Mu_y, Mu_z, Sigma_x, Sigma_y, Sigma_z), QMS_3D.f90:211-269; Mu is
+ * returned as 1/(1/Mu) of the stored reciprocal. */
+int qms_get_coeffs(qms_handle *h, double *out);
+
 /* y = M(v, dt) x on 4 node-sized arrays, where M is the reference's CSR
  * matrix (dt <= 0: steady S(v), :436-1296; dt > 0: transient T(v, dt),
  * :1452-2191).  Fixed rows are identity rows; entries with no unknown are 0. */

@@ -14,23 +14,11 @@ def problem(design) -> "O.Problem":
                      materials=mats, geom=np.asarray(design.geom, dtype=np.float64))
 
 
-def rel(a, b) -> float:
-    nb = np.linalg.norm(b)
-    return float(np.linalg.norm(a - b) / nb) if nb > 0 else float(np.linalg.norm(a - b))
-
-
-def a_part(f):
-    return f[:3].ravel()
+from alpineqms_amd.fields import a_part, rel  # noqa: E402,F401  (shared with bench.py)
+from alpineqms_amd.fields import sigma_grad_v as _sgv  # noqa: E402
 
 
 def sigma_grad_v(p, coef, fields):
-    """sigma * grad V on the Ax, Ay, Az edges: the V coupling of the A rows
-    (QMS_3D.f90:595-607).  Well conditioned even where V itself carries a
-    floating-conductor constant."""
-    V = fields[3]
-    hx, hy, hz = p.h
-    sx, sy, sz = coef[3], coef[4], coef[5]
-    gx = sx[:, :, :-1] * (V[:, :, 1:] - V[:, :, :-1]) / hx
-    gy = sy[:, :-1, :] * (V[:, 1:, :] - V[:, :-1, :]) / hy
-    gz = sz[:-1, :, :] * (V[1:, :, :] - V[:-1, :, :]) / hz
-    return np.concatenate([gx[2:-2, 2:-2, 1:-1].ravel(), gy[2:-2, 1:-1, 2:-2].ravel(), gz[1:-1, 2:-2, 2:-2].ravel()])
+    """sigma * grad V on the free A edges (QMS_3D.f90:595-607) for an oracle
+    problem and its coefficient array."""
+    return _sgv(p.h, coef, fields)
