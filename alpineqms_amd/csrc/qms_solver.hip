@@ -18,6 +18,8 @@
 #include <memory>
 #include <functional>
 #include <array>
+#include <atomic>
+#include <thread>
 #include <algorithm>
 #include <unordered_map>
 #include <type_traits>
@@ -39,7 +41,7 @@ void check(hipError_t e, const char *what) {
     if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e), QMS_ERR_HIP);
 }
 
-std::string g_last_error;
+thread_local std::string g_last_error;
 
 // Mean milliseconds per call of f over `reps` calls on stream s, after one
 // untimed call, with HIP events (released on every exit path).
@@ -131,6 +133,12 @@ struct qms_handle {
 
     qms_stats st;
 
+    // qms_run_async: the library thread running the steps and its status
+    std::thread worker;
+    std::atomic<int> running{0};
+    int async_rc = QMS_OK;
+    std::vector<double> async_v;
+
     void *raw_alloc(size_t nb) {
         void *p = nullptr;
         check(hipMalloc(&p, nb), "hipMalloc");
@@ -144,6 +152,7 @@ struct qms_handle {
         return (T *)raw_alloc(sizeof(T) * (size_t)(count > 0 ? count : 1));
     }
     ~qms_handle() {
+        if (worker.joinable()) worker.join();
         if (s) hipStreamSynchronize(s);
         for (auto &gv : graphs)
             for (hipGraphExec_t e : gv)
@@ -476,6 +485,16 @@ void prepare_solve(qms_handle *h, int tr) {
     else ensure_solver<double>(h, tr);
 }
 
+// True (and the handle's message set) when another thread than the handle's
+// asynchronous run calls while that run is going.
+bool busy(qms_handle *h) {
+    if (h->running.load() && std::this_thread::get_id() != h->worker.get_id()) {
+        h->err = "handle busy: an asynchronous run is going (qms_wait first)";
+        return true;
+    }
+    return false;
+}
+
 int fail(qms_handle *h, const std::exception &e, int code) {
     if (h) h->err = e.what();
     g_last_error = e.what();
@@ -596,6 +615,7 @@ int qms_create(const qms_config *cfg, const uint16_t *cell_slot, const double *s
 
 int qms_steady(qms_handle *h, const double v[3]) {
     if (!h || !v) return QMS_ERR_ARG;
+    if (busy(h)) return QMS_ERR_STATE;
     QMS_TRY(h)
     prepare_solve(h, 0);
     const OpParams op = make_op(h, v, 0.0, 0);
@@ -611,6 +631,7 @@ int qms_steady(qms_handle *h, const double v[3]) {
 
 int qms_step(qms_handle *h, const double v[3]) {
     if (!h || !v) return QMS_ERR_ARG;
+    if (busy(h)) return QMS_ERR_STATE;
     if (!h->have_state) { h->err = "qms_step: no state (call qms_steady or qms_set_fields first)"; return QMS_ERR_STATE; }
     if (!(h->cfg.dt > 0)) { h->err = "qms_step: dt must be positive"; return QMS_ERR_ARG; }
     QMS_TRY(h)
@@ -630,6 +651,7 @@ int qms_step(qms_handle *h, const double v[3]) {
 
 int qms_run(qms_handle *h, int nsteps, const double *v) {
     if (!h || (nsteps > 0 && !v) || nsteps < 0) return QMS_ERR_ARG;
+    if (busy(h)) return QMS_ERR_STATE;
     for (int sidx = 0; sidx < nsteps; ++sidx) {
         const int rc = qms_step(h, v + 3 * sidx);
         if (rc != QMS_OK) return rc;
@@ -637,8 +659,89 @@ int qms_run(qms_handle *h, int nsteps, const double *v) {
     return QMS_OK;
 }
 
+int qms_run_async(qms_handle *h, int nsteps, const double *v) {
+    if (!h || (nsteps > 0 && !v) || nsteps < 0) return QMS_ERR_ARG;
+    if (h->running.load()) { h->err = "qms_run_async: a run is already going on this handle"; return QMS_ERR_STATE; }
+    if (h->worker.joinable()) h->worker.join();
+    h->async_v.assign(v, v + 3 * (size_t)nsteps);
+    h->async_rc = QMS_OK;
+    h->running.store(1);
+    try {
+        h->worker = std::thread([h, nsteps]() {
+            h->async_rc = qms_run(h, nsteps, h->async_v.data());
+            h->running.store(0);
+        });
+    } catch (const std::exception &e) {
+        h->running.store(0);
+        return fail(h, e, QMS_ERR_ARG);
+    }
+    return QMS_OK;
+}
+
+int qms_poll(qms_handle *h) {
+    if (!h) return QMS_ERR_ARG;
+    if (h->running.load()) return QMS_BUSY;
+    return h->async_rc;
+}
+
+int qms_wait(qms_handle *h) {
+    if (!h) return QMS_ERR_ARG;
+    if (h->worker.joinable()) h->worker.join();
+    return h->async_rc;
+}
+
+int64_t qms_estimate_bytes(const qms_config *cfg) {
+    if (!cfg) return 0;
+    for (int a = 0; a < 3; ++a)
+        if (cfg->cells[a] < 5 || cfg->cells[a] > 4096) return 0;
+    if (cfg->restart < 1 || cfg->restart > 64) return 0;
+    const int64_t nx = cfg->cells[0] + 1, ny = cfg->cells[1] + 1, nz = cfg->cells[2] + 1;
+    const int64_t n1 = nx * ny * nz, n = 4 * n1, nc = (int64_t)cfg->cells[0] * cfg->cells[1] * cfg->cells[2];
+    const int64_t m = cfg->restart;
+    const int64_t ta = cfg->precision == QMS_MIXED ? 4 : 8;
+    int64_t b = 0, allocs = 0;
+    auto add = [&](int64_t bytes) { b += bytes; ++allocs; };
+    // qms_create: cell slots, material table (at most 65536 slots), coefficients in FP64 and FP32,
+    // class indices and table, state and work vectors, row scalings, step history, Krylov basis
+    add(2 * nc); add(48 * 65536);
+    for (int a = 0; a < 6; ++a) { add(8 * n1); add(4 * n1); }
+    add(2 * n1); add(48 * 65535);
+    for (int v = 0; v < 9; ++v) add(8 * n);
+    for (int j = 0; j <= m; ++j) add(8 * n);
+    add(8 * (m + 1));
+    add(8 * (m + 2) * std::max<int64_t>(1, std::min<int64_t>(1024, (n + 2047) / 2048)));
+    add(8 * (m + 8)); add(8 * (m + 1) * m); add(8 * m); add(8 * m); add(8 * (m + 1)); add(8 * m);
+    add(8 * KS_COUNT); add((int64_t)sizeof(OpK));
+    // first solve: Z basis and preconditioner vectors
+    for (int j = 0; j < m; ++j) add(ta * n);
+    add(8 * m); add(ta * 3 * n1); add(8 * n1); add(8 * n1);
+    // hierarchies: A block steady and transient (precision ta), V block (FP64)
+    auto hier = [&](int nc_, int64_t s) {
+        int64_t x = nx, y = ny, z = nz;
+        add(s * nc_ * x * y * z); add(s * nc_ * x * y * z);     // level 0: t, r
+        int nlev = 1;
+        int64_t cn = 0;
+        while (nlev < MG_MAX_LEVELS) {
+            if (std::max(x, std::max(y, z)) <= 4 && nlev >= 2) break;
+            x = (x + 1) / 2; y = (y + 1) / 2; z = (z + 1) / 2;
+            const int64_t l1 = x * y * z;
+            add(s * nc_ * 6 * l1); add(s * nc_ * l1); add(s * nc_ * l1);
+            if (nc_ == 3) add(s * nc_ * 8 * l1);
+            for (int v = 0; v < 4; ++v) add(s * nc_ * l1);
+            cn = nc_ * l1;
+            ++nlev;
+        }
+        add(4 * cn); add(8 * cn * cn);                              // coarsest dense inverse (upper bound)
+    };
+    hier(3, ta);
+    hier(3, ta);
+    hier(1, 8);
+    return b + allocs * 256;       // allocation granularity
+}
+
 int qms_get_fields(qms_handle *h, double *out) {
     if (!h || !out) return QMS_ERR_ARG;
+    if (busy(h)) return QMS_ERR_STATE;
     QMS_TRY(h)
     check(hipMemcpyAsync(out, h->x, sizeof(double) * h->n, hipMemcpyDeviceToHost, h->s), "d2h");
     check(hipStreamSynchronize(h->s), "sync");
@@ -648,6 +751,7 @@ int qms_get_fields(qms_handle *h, double *out) {
 
 int qms_set_fields(qms_handle *h, const double *in) {
     if (!h || !in) return QMS_ERR_ARG;
+    if (busy(h)) return QMS_ERR_STATE;
     QMS_TRY(h)
     check(hipMemcpyAsync(h->x, in, sizeof(double) * h->n, hipMemcpyHostToDevice, h->s), "h2d");
     check(hipStreamSynchronize(h->s), "sync");
@@ -659,6 +763,7 @@ int qms_set_fields(qms_handle *h, const double *in) {
 
 int qms_get_h(qms_handle *h, int unit_mu, double *out) {
     if (!h || !out) return QMS_ERR_ARG;
+    if (busy(h)) return QMS_ERR_STATE;
     QMS_TRY(h)
     const int64_t nc = (int64_t)h->cfg.cells[0] * h->cfg.cells[1] * h->cfg.cells[2];
     launch_hfield(h->g, op_coeffs(h), h->x, h->tmp, unit_mu, h->s);
@@ -670,6 +775,7 @@ int qms_get_h(qms_handle *h, int unit_mu, double *out) {
 
 int qms_apply(qms_handle *h, const double v[3], double dt, const double *x, double *y) {
     if (!h || !v || !x || !y) return QMS_ERR_ARG;
+    if (busy(h)) return QMS_ERR_STATE;
     QMS_TRY(h)
     const int tr = dt > 0 ? 1 : 0;
     const OpParams op = make_op(h, v, dt, tr);
@@ -687,6 +793,7 @@ int qms_apply(qms_handle *h, const double v[3], double dt, const double *x, doub
 
 int qms_residual(qms_handle *h, const double v[3], double dt, const double *x, const double *b, double *r) {
     if (!h || !v || !x || !b || !r) return QMS_ERR_ARG;
+    if (busy(h)) return QMS_ERR_STATE;
     QMS_TRY(h)
     const int tr = dt > 0 ? 1 : 0;
     const OpParams op = make_op(h, v, dt, tr);
@@ -701,6 +808,7 @@ int qms_residual(qms_handle *h, const double v[3], double dt, const double *x, c
 
 int qms_rhs(qms_handle *h, const double v[3], double dt, const double *aprev, double *b) {
     if (!h || !v || !aprev || !b) return QMS_ERR_ARG;
+    if (busy(h)) return QMS_ERR_STATE;
     QMS_TRY(h)
     const OpParams op = make_op(h, v, dt, dt > 0 ? 1 : 0);
     check(hipMemcpyAsync(h->r, aprev, sizeof(double) * h->n, hipMemcpyHostToDevice, h->s), "h2d");
@@ -748,6 +856,7 @@ int qms_set_colwidth(qms_handle *h, int nodes) {
 int qms_time_kernel(qms_handle *h, int which, const double v[3], int reps, double *ms_per_launch,
                     double *bytes_per_launch) {
     if (!h || reps < 1 || !ms_per_launch || !bytes_per_launch) return QMS_ERR_ARG;
+    if (busy(h)) return QMS_ERR_STATE;
     const bool cold = (which & QMS_TIME_COLD) != 0;
     which &= ~QMS_TIME_COLD;
     if (which < 0 || which > 7) return QMS_ERR_ARG;
@@ -856,6 +965,7 @@ int qms_time_kernel(qms_handle *h, int which, const double v[3], int reps, doubl
 
 int qms_gram_schmidt(qms_handle *h, int nv, const double *basis, const double *w, double *hcoef, double *wout) {
     if (!h || !basis || !w || !hcoef || !wout) return QMS_ERR_ARG;
+    if (busy(h)) return QMS_ERR_STATE;
     if (nv < 1 || nv > h->cfg.restart + 1) {
         h->err = "qms_gram_schmidt: nv must be in [1, restart + 1]";
         return QMS_ERR_ARG;
@@ -876,6 +986,7 @@ int qms_gram_schmidt(qms_handle *h, int nv, const double *basis, const double *w
 
 int qms_get_coeffs(qms_handle *h, double *out) {
     if (!h || !out) return QMS_ERR_ARG;
+    if (busy(h)) return QMS_ERR_STATE;
     const int64_t n1 = h->g.n1;
     for (int q = 0; q < 6; ++q)
         for (int64_t p = 0; p < n1; ++p) {
@@ -887,6 +998,7 @@ int qms_get_coeffs(qms_handle *h, double *out) {
 
 int qms_get_work(qms_handle *h, double *out) {
     if (!h || !out) return QMS_ERR_ARG;
+    if (busy(h)) return QMS_ERR_STATE;
     QMS_TRY(h)
     check(hipMemcpyAsync(out, h->w, sizeof(double) * h->n, hipMemcpyDeviceToHost, h->s), "d2h");
     check(hipStreamSynchronize(h->s), "sync");

@@ -20,6 +20,7 @@ QMS_ERR_HIP = -2
 QMS_ERR_NOCONV = -3
 QMS_ERR_NOMEM = -4
 QMS_ERR_STATE = -5
+QMS_BUSY = 1
 QMS_FP64 = 0
 QMS_MIXED = 1
 
@@ -28,6 +29,7 @@ EXPORTS = (
     "qms_config_default", "qms_create", "qms_steady", "qms_step", "qms_run",
     "qms_get_fields", "qms_set_fields", "qms_get_h", "qms_get_coeffs", "qms_apply", "qms_residual", "qms_rhs",
     "qms_get_stats", "qms_set_zchunk", "qms_set_colwidth", "qms_use_coef_classes", "qms_time_kernel", "qms_gram_schmidt", "qms_get_work", "qms_destroy", "qms_last_error",
+    "qms_run_async", "qms_poll", "qms_wait", "qms_estimate_bytes",
     "qms_batch_create", "qms_batch_set_blocks", "qms_batch_apply", "qms_batch_tune", "qms_batch_time",
     "qms_batch_plan", "qms_batch_last_error", "qms_batch_destroy",
 )
@@ -101,6 +103,11 @@ def load(path: Optional[str] = None):
     L.qms_steady.argtypes = [H, d]
     L.qms_step.argtypes = [H, d]
     L.qms_run.argtypes = [H, ctypes.c_int, d]
+    L.qms_run_async.argtypes = [H, ctypes.c_int, d]
+    L.qms_poll.argtypes = [H]
+    L.qms_wait.argtypes = [H]
+    L.qms_estimate_bytes.argtypes = [ctypes.POINTER(Config)]
+    L.qms_estimate_bytes.restype = ctypes.c_int64
     L.qms_get_fields.argtypes = [H, d]
     L.qms_set_fields.argtypes = [H, d]
     L.qms_get_h.argtypes = [H, ctypes.c_int, d]
@@ -133,7 +140,8 @@ def load(path: Optional[str] = None):
     L.qms_last_error.argtypes = [H]
     L.qms_last_error.restype = ctypes.c_char_p
     for name in EXPORTS:
-        if name not in ("qms_destroy", "qms_last_error", "qms_batch_destroy", "qms_batch_last_error"):
+        if name not in ("qms_destroy", "qms_last_error", "qms_batch_destroy", "qms_batch_last_error",
+                        "qms_estimate_bytes"):
             getattr(L, name).restype = ctypes.c_int
     if path is None:
         _lib = L
@@ -150,6 +158,37 @@ def default_config() -> Config:
     return c
 
 
+def make_config(cells: Sequence[int], h: Sequence[float], dt: float, bc: Sequence[float],
+                precision: int = QMS_FP64, tol: float = 1e-12, restart: int = 30, max_iters: int = 3000,
+                mg_nu: int = 4, mg_omega: float = 0.7, mg_alpha: float = 1.8, use_graphs: bool = True) -> Config:
+    cfg = default_config()
+    for a in range(3):
+        cfg.cells[a] = int(cells[a])
+        cfg.h[a] = float(h[a])
+    cfg.dt = float(dt)
+    bcv = np.asarray(bc, dtype=np.float64).ravel()
+    if bcv.size != 12:
+        raise ValueError("bc needs 12 values (B0 and the 3x3 gradient)")
+    for a in range(12):
+        cfg.bc[a] = float(bcv[a])
+    cfg.precision = int(precision)
+    cfg.tol = float(tol)
+    cfg.restart = int(restart)
+    cfg.max_iters = int(max_iters)
+    cfg.mg_nu = int(mg_nu)
+    cfg.mg_omega = float(mg_omega)
+    cfg.mg_alpha = float(mg_alpha)
+    cfg.use_graphs = 1 if use_graphs else 0
+    return cfg
+
+
+def estimate_bytes(cells: Sequence[int], precision: int = QMS_FP64, restart: int = 30) -> int:
+    """Device bytes a handle of this size holds once it has stepped (host-only,
+    qms_estimate_bytes): the admission measure of the serving loop."""
+    cfg = make_config(cells, (1.0, 1.0, 1.0), 1.0, np.zeros(12), precision=precision, restart=restart)
+    return int(load().qms_estimate_bytes(ctypes.byref(cfg)))
+
+
 class Solver:
     """One design on the GPU (a ``qms_handle``)."""
 
@@ -158,24 +197,8 @@ class Solver:
                  tol: float = 1e-12, restart: int = 30, max_iters: int = 3000, mg_nu: int = 4,
                  mg_omega: float = 0.7, mg_alpha: float = 1.8, use_graphs: bool = True):
         L = load()
-        cfg = default_config()
-        for a in range(3):
-            cfg.cells[a] = int(cells[a])
-            cfg.h[a] = float(h[a])
-        cfg.dt = float(dt)
-        bcv = np.asarray(bc, dtype=np.float64).ravel()
-        if bcv.size != 12:
-            raise ValueError("bc needs 12 values (B0 and the 3x3 gradient)")
-        for a in range(12):
-            cfg.bc[a] = float(bcv[a])
-        cfg.precision = int(precision)
-        cfg.tol = float(tol)
-        cfg.restart = int(restart)
-        cfg.max_iters = int(max_iters)
-        cfg.mg_nu = int(mg_nu)
-        cfg.mg_omega = float(mg_omega)
-        cfg.mg_alpha = float(mg_alpha)
-        cfg.use_graphs = 1 if use_graphs else 0
+        cfg = make_config(cells, h, dt, bc, precision, tol, restart, max_iters, mg_nu, mg_omega, mg_alpha,
+                          use_graphs)
         self.cells = tuple(int(c) for c in cells)
         self.nodes = tuple(c + 1 for c in self.cells)
         self.dt = float(dt)
@@ -233,6 +256,22 @@ class Solver:
     def run(self, velocities: np.ndarray):
         vv = np.ascontiguousarray(velocities, dtype=np.float64).reshape(-1, 3)
         self._check(self._L.qms_run(self._h, vv.shape[0], _dp(vv)))
+
+    def run_async(self, velocities: np.ndarray):
+        """Start the steps on a library thread (returns at once)."""
+        vv = np.ascontiguousarray(velocities, dtype=np.float64).reshape(-1, 3)
+        self._check(self._L.qms_run_async(self._h, vv.shape[0], _dp(vv)))
+
+    def poll(self) -> bool:
+        """True while an asynchronous run is going; raises its error if it failed."""
+        rc = self._L.qms_poll(self._h)
+        if rc == QMS_BUSY:
+            return True
+        self._check(rc)
+        return False
+
+    def wait(self):
+        self._check(self._L.qms_wait(self._h))
 
     def get_fields(self) -> np.ndarray:
         nx, ny, nz = self.nodes

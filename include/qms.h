@@ -31,6 +31,7 @@ extern "C" {
 #define QMS_ERR_NOCONV (-3)   /* Krylov solve did not reach the tolerance */
 #define QMS_ERR_NOMEM (-4)    /* device allocation failed */
 #define QMS_ERR_STATE (-5)    /* call out of order (e.g. step before steady) */
+#define QMS_BUSY 1            /* qms_poll: an asynchronous run is still going */
 
 #define QMS_FP64 0            /* Krylov and multigrid in FP64 */
 #define QMS_MIXED 1           /* FP64 Krylov / operator, FP32 multigrid */
@@ -98,6 +99,25 @@ int qms_step(qms_handle *h, const double v[3]);
 /* nsteps time steps with velocities v[3*s..3*s+2], fields kept on the device
  * between steps (the device-resident time loop). */
 int qms_run(qms_handle *h, int nsteps, const double *v);
+
+/* Asynchronous form of qms_run for serving many designs at once: the steps
+ * run on a library thread (each handle has its own HIP stream, so the work of
+ * different handles overlaps on the GPU) and the call returns at once.  The
+ * velocities are copied.  qms_poll returns QMS_BUSY while the run is going,
+ * then its status; qms_wait blocks until it ends and returns its status.  No
+ * other call may use the handle while a run is going (QMS_ERR_STATE).  The
+ * reference steps one design per process (QMS_3D.f90:1424-2302); this has no
+ * counterpart there. */
+int qms_run_async(qms_handle *h, int nsteps, const double *v);
+int qms_poll(qms_handle *h);
+int qms_wait(qms_handle *h);
+
+/* Device bytes a handle with this configuration holds once it has solved a
+ * steady state and a transient step (state, Krylov basis, both multigrid
+ * hierarchies, coefficients; the class table at its maximum size), for
+ * admission by memory footprint.  Host-only: needs no GPU.  Returns 0 on an
+ * invalid configuration. */
+int64_t qms_estimate_bytes(const qms_config *cfg);
 
 /* Copy the state (4 node-sized arrays Ax, Ay, Az, V) to / from the host. */
 int qms_get_fields(qms_handle *h, double *out);

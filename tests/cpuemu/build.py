@@ -51,6 +51,10 @@ def build(verbose: bool = False, sanitize: bool = True) -> str:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
         objs.append(obj)
+    sw = os.path.join(OUTDIR, "qemu_switch.o")
+    procs.append(("qemu_switch.S", subprocess.Popen([cc, "-c", os.path.join(HERE, "qemu_switch.S"), "-o", sw],
+                                                    stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
+    objs.append(sw)
     failed = []
     for src, p in procs:
         out, _ = p.communicate()

@@ -34,7 +34,6 @@
 #include <type_traits>
 #include <utility>
 #include <vector>
-#include <ucontext.h>
 #if defined(__SANITIZE_ADDRESS__)
 #include <sanitizer/common_interface_defs.h>
 #endif
@@ -81,8 +80,14 @@ namespace qemu {
 // The threads of one block are fibers on the calling OS thread: each runs
 // until it reaches a barrier or returns, and a barrier is released when every
 // thread of the block that has not returned is waiting at it.
+// Minimal x86-64 stack switch (tests/cpuemu/qemu_switch.S): saves the
+// callee-saved registers on the current stack, stores its pointer in *save and
+// resumes the stack `to` (glibc's swapcontext also saves the signal mask, a
+// system call per switch).
+extern "C" void qemu_switch(void **save, void *to);
+
 struct Fiber {
-    ucontext_t ctx;
+    void *sp = nullptr;
     char *stack = nullptr;
     bool done = false, waiting = false;
 };
@@ -94,7 +99,7 @@ struct Block {
     std::vector<uint64_t> shfl;
     unsigned bx, by, bz;
     const std::function<void()> *body;
-    ucontext_t sched;
+    void *sched_sp = nullptr;
     unsigned cur = 0;
     void *fake = nullptr;       // AddressSanitizer fake-stack handle of the scheduler
     const void *main_bottom = nullptr;
@@ -114,15 +119,15 @@ inline void to_sched(Block &b, bool dying) {
 #if defined(__SANITIZE_ADDRESS__)
     void *fake = nullptr;
     __sanitizer_start_switch_fiber(dying ? nullptr : &fake, b.main_bottom, b.main_size);
-    swapcontext(&f.ctx, &b.sched);
+    qemu_switch(&f.sp, b.sched_sp);
     __sanitizer_finish_switch_fiber(fake, &b.main_bottom, &b.main_size);
 #else
     (void)dying;
-    swapcontext(&f.ctx, &b.sched);
+    qemu_switch(&f.sp, b.sched_sp);
 #endif
 }
 
-inline void fiber_entry() {
+extern "C" inline void fiber_entry() {
     Block &b = *cur_block;
 #if defined(__SANITIZE_ADDRESS__)
     __sanitizer_finish_switch_fiber(nullptr, &b.main_bottom, &b.main_size);
@@ -171,11 +176,14 @@ inline void run_block(Block &b) {
         Fiber &f = fib[t];
         f.done = false;
         f.waiting = false;
-        getcontext(&f.ctx);
-        f.ctx.uc_stack.ss_sp = f.stack;
-        f.ctx.uc_stack.ss_size = FIBER_STACK;
-        f.ctx.uc_link = nullptr;
-        makecontext(&f.ctx, fiber_entry, 0);
+        // initial frame: six zeroed callee-saved registers, then the return
+        // address fiber_entry, with the stack 16-byte aligned at its entry
+        uintptr_t top = ((uintptr_t)(f.stack + FIBER_STACK)) & ~(uintptr_t)15;
+        void **sp = (void **)top;
+        *--sp = nullptr;
+        *--sp = (void *)&fiber_entry;
+        for (int r = 0; r < 6; ++r) *--sp = nullptr;
+        f.sp = sp;
     }
     for (;;) {
         bool any = false;
@@ -188,7 +196,7 @@ inline void run_block(Block &b) {
 #if defined(__SANITIZE_ADDRESS__)
             __sanitizer_start_switch_fiber(&b.fake, f.stack, FIBER_STACK);
 #endif
-            swapcontext(&b.sched, &f.ctx);
+            qemu_switch(&b.sched_sp, f.sp);
 #if defined(__SANITIZE_ADDRESS__)
             __sanitizer_finish_switch_fiber(b.fake, nullptr, nullptr);
 #endif

@@ -75,3 +75,37 @@ def test_solve_matches_direct(cpu_lib, precision):
         print("ok")
     """)
     assert "ok" in out
+
+
+def test_async_runs_and_served_mix_match_runs_alone(cpu_lib):
+    """The asynchronous run and the serving loop with three designs in flight
+    on the CPU build (library threads, one stream per handle): each design's
+    result is bit for bit its result alone, and a handle refuses calls while
+    its run goes on."""
+    out = run_child(cpu_lib, """
+        from alpineqms_amd import serve
+        from alpineqms_amd.configs import Request
+        import tests.test_gpu_serve as T
+        ds = [T._design(80 + k, n=9 + k, steps=2, v=2.0 + k) for k in range(3)]
+        alone = {d.name: T._run_alone(d, native.QMS_MIXED) for d in ds}
+        got = {}
+        res = serve.serve([Request(0.0, d) for d in ds], precision=native.QMS_MIXED, max_active=3,
+                          on_result=lambda req, s: got.__setitem__(req.design.name, s.get_fields()))
+        assert all(r.error is None for r in res.records), [r.error for r in res.records]
+        assert res.peak_active > 1
+        for d in ds:
+            assert np.array_equal(got[d.name], alone[d.name]), d.name
+        d = ds[0]
+        with serve.make_solver(d, native.QMS_MIXED) as s:
+            s.steady(serve.program.steady_velocity(d.inputs.traj, d.inputs.dt))
+            s.run_async(serve.velocities(d, 1, d.inputs.steps))
+            try:
+                s.get_fields()
+                raise AssertionError("busy handle answered")
+            except native.QmsError:
+                pass
+            s.wait()
+            assert np.array_equal(s.get_fields(), alone[d.name])
+        print("ok")
+    """)
+    assert "ok" in out

@@ -17,7 +17,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def header_functions():
     txt = open(os.path.join(ROOT, "include", "qms.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:int|void|const char \*)\s*(qms_\w+)\s*\(", txt, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int64_t|int|void|const char \*)\s*(qms_\w+)\s*\(", txt, re.M)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -118,3 +118,16 @@ def test_design_generators_are_seeded_and_in_bounds():
     s = configs.config5_stream(20, rate=1.0)
     assert all(32 <= q.design.nodes[0] <= 160 and q.design.nodes[0] % 8 == 0 for q in s)
     assert all(s[i].arrival < s[i + 1].arrival for i in range(len(s) - 1))
+
+
+def test_footprint_estimate_is_host_only_and_grows_with_the_design():
+    """qms_estimate_bytes needs no GPU; it grows with the grid and with the
+    FP64 multigrid, and refuses a grid the library refuses."""
+    from alpineqms_amd import native
+    small = native.estimate_bytes((31, 31, 31), native.QMS_FP64)
+    big = native.estimate_bytes((127, 127, 127), native.QMS_FP64)
+    mixed = native.estimate_bytes((127, 127, 127), native.QMS_MIXED)
+    assert 0 < small < big and mixed < big
+    # the Krylov basis alone (31 FP64 vectors of 4 N^3) is a lower bound
+    assert big > 31 * 8 * 4 * 128 ** 3
+    assert native.estimate_bytes((3, 31, 31), native.QMS_FP64) == 0
