@@ -172,8 +172,18 @@ struct MgLevel {
     T *exc = nullptr;          // [ncomp][n1] excess diagonal
     T *cross = nullptr;        // [ncomp][8][n1] cross-component coefficients (A block)
     T *dinv = nullptr;         // [ncomp][n1] 1/diag, 0 on inactive rows
+    // The same rows as a 2-byte class per node and a table of the distinct
+    // row tuples (MG_CLS_RS values per component: cond[6], exc, dinv,
+    // cross[8]); when set, cond/exc/cross/dinv are not allocated.
+    uint16_t *cls = nullptr;
+    T *tab = nullptr;
+    int ncls = 0;
     T *x = nullptr, *b = nullptr, *r = nullptr, *t = nullptr;   // [ncomp][n1] (x, b unused on level 0)
 };
+// Table row layout of a class-compressed level: per component, values
+// [0..5] conductances, [6] excess diagonal, [7] 1/diag (0: inactive row),
+// [8..15] cross-component coefficients (A block only).
+constexpr int MG_CLS_RS_A = 16, MG_CLS_RS_V = 8;
 
 template <typename T>
 struct Hierarchy {

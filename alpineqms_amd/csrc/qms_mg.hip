@@ -34,6 +34,7 @@
 #include <algorithm>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 #include <vector>
 #include "qms_kernels.h"
 
@@ -166,15 +167,35 @@ QMS_HD void eliminate(const Grid &g, int i, int j, int k, Row<T> &r) {
     }
 }
 
+// Coefficient access at node (i,j,k) through the class table: the class of
+// the node at the offset selects a row (1/Mu_x, 1/Mu_y, 1/Mu_z, Sigma_x,
+// Sigma_y, Sigma_z) of the FP64 table.  The values are the arrays' values
+// (the FP32 arrays hold the same doubles rounded to float).
+template <typename T>
+struct CoefCls {
+    const uint16_t *cls;
+    const double *ctab;
+    int64_t p;
+    int sy, sz;
+    QMS_HD const double *tuple(int di, int dj, int dk) const {
+        return ctab + 6 * (int)cls[p + di + (int64_t)dj * sy + (int64_t)dk * sz];
+    }
+    QMS_HD T IM(int a, int di, int dj, int dk) const { return (T)tuple(di, dj, dk)[a]; }
+    QMS_HD T SG(int a, int di, int dj, int dk) const { return (T)tuple(di, dj, dk)[3 + a]; }
+};
+
 // ---------------------------------------------------------------------------
 // Row providers for the device kernels
 // ---------------------------------------------------------------------------
-// Level 0: rows evaluated from the coefficient arrays.
-template <typename T, int NC>
+// Level 0: rows evaluated from the coefficient arrays, or (CLS) from the
+// class table.
+template <typename T, int NC, bool CLS>
 struct FineProv {
     Grid g;
     const T *im[3];
     const T *sg[3];
+    const uint16_t *cls;
+    const double *ctab;
     FineConst<T> fk;
     __device__ __forceinline__ LevelDims dims() const { return LevelDims{g.nx, g.ny, g.nz, g.n1}; }
     template <int C>
@@ -182,11 +203,56 @@ struct FineProv {
     template <int C>
     __device__ __forceinline__ bool row(int i, int j, int k, Row<T> &r) const {
         if (!active<C>(i, j, k)) return false;
-        const CoefAt<T> ca{{im[0], im[1], im[2]}, {sg[0], sg[1], sg[2]}, gidx(g, i, j, k), g.nx, g.nx * g.ny};
-        if (NC == 3) row_a_raw<C, T>(ca, fk, r);
-        else row_v_raw<T>(ca, fk, r);
+        if constexpr (CLS) {
+            const CoefCls<T> ca{cls, ctab, gidx(g, i, j, k), g.nx, g.nx * g.ny};
+            if (NC == 3) row_a_raw<C, T>(ca, fk, r);
+            else row_v_raw<T>(ca, fk, r);
+        } else {
+            const CoefAt<T> ca{{im[0], im[1], im[2]}, {sg[0], sg[1], sg[2]}, gidx(g, i, j, k), g.nx, g.nx * g.ny};
+            if (NC == 3) row_a_raw<C, T>(ca, fk, r);
+            else row_v_raw<T>(ca, fk, r);
+        }
         eliminate<C, NC, T>(g, i, j, k, r);
         return true;
+    }
+};
+
+// Levels >= 1 compressed: a 2-byte class per node selects a row tuple
+// (layout MG_CLS_RS_*) of the level's table.
+template <typename T, int NC>
+struct ClsProv {
+    static constexpr int RS = NC == 3 ? MG_CLS_RS_A : MG_CLS_RS_V;
+    LevelDims d;
+    const uint16_t *cls;
+    const T *tab;
+    __device__ __forceinline__ LevelDims dims() const { return d; }
+    __device__ __forceinline__ const T *tuple(int C, int i, int j, int k) const {
+        return tab + (int64_t)cls[lidx(d, i, j, k)] * (NC * RS) + C * RS;
+    }
+    template <int C>
+    __device__ __forceinline__ bool active(int i, int j, int k) const { return tuple(C, i, j, k)[7] != (T)0; }
+    template <int C>
+    __device__ __forceinline__ bool row(int i, int j, int k, Row<T> &r) const {
+        const T *t = tuple(C, i, j, k);
+        if (t[7] == (T)0) return false;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) r.cond[q] = t[q];
+        r.exc = t[6];
+        if (NC == 3) {
+#pragma unroll
+            for (int s = 0; s < 8; ++s) r.cross[s] = t[8 + s];
+        }
+        return true;
+    }
+    template <int C>
+    __device__ __forceinline__ bool row_all(int i, int j, int k, Row<T> &r) const {
+        const T *t = tuple(C, i, j, k);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) r.cond[q] = t[q];
+        r.exc = t[6];
+#pragma unroll
+        for (int s = 0; s < 8; ++s) r.cross[s] = NC == 3 ? t[8 + s] : (T)0;
+        return t[7] != (T)0;
     }
 };
 
@@ -212,6 +278,17 @@ struct ExplProv {
             for (int s = 0; s < 8; ++s) r.cross[s] = cross[((int64_t)C * 8 + s) * d.n1 + p];
         }
         return true;
+    }
+    // The row loaded whatever its state; returns whether it is active.
+    template <int C>
+    __device__ __forceinline__ bool row_all(int i, int j, int k, Row<T> &r) const {
+        const int64_t p = lidx(d, i, j, k);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) r.cond[q] = cond[((int64_t)C * 6 + q) * d.n1 + p];
+        r.exc = exc[(int64_t)C * d.n1 + p];
+#pragma unroll
+        for (int s = 0; s < 8; ++s) r.cross[s] = NC == 3 ? cross[((int64_t)C * 8 + s) * d.n1 + p] : (T)0;
+        return dinv[(int64_t)C * d.n1 + p] != (T)0;
     }
 };
 
@@ -318,6 +395,66 @@ __global__ void __launch_bounds__(256) k_mg_resid(P pv, const T *__restrict__ x,
 }
 
 // ---------------------------------------------------------------------------
+// Coarse levels (explicit or class-compressed rows), branch-free: these
+// levels are small, so a launch costs the latency of one thread's chain of
+// loads rather than bandwidth.  Every neighbour index is clamped into the
+// level's box -- a coupling to a node outside the box is zero after the
+// Dirichlet elimination (the Galerkin build checks this), so the clamped
+// value is multiplied by 0 -- every load is unconditional, and an inactive
+// row is selected to 0 at the end.  The loads of all components can then be
+// in flight together.  The arithmetic is that of k_mg_smooth / k_mg_resid:
+// a skipped zero term there adds a zero product here.
+// MODE 0: damped Jacobi sweep, 1: first sweep from x = 0, 2: residual.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int64_t lidx_clamped(const LevelDims &d, int i, int j, int k) {
+    return lidx(d, max(0, min(i, d.nx - 1)), max(0, min(j, d.ny - 1)), max(0, min(k, d.nz - 1)));
+}
+
+template <int C, int NC, int MODE, typename T, typename P>
+__device__ __forceinline__ void cop_comp(const P &pv, const LevelDims &d, int64_t p, int i, int j, int k,
+                                         const T *__restrict__ x, const T *__restrict__ b, T *__restrict__ out,
+                                         T omega) {
+    Row<T> r;
+    const bool act = pv.template row_all<C>(i, j, k, r);
+    const int64_t q = (int64_t)C * d.n1 + p;
+    T v;
+    if (MODE == 1) {
+        v = omega * b[q] / row_diag(r);
+    } else {
+        const T x0 = x[q];
+        T y = r.exc * x0;
+        T diag = r.exc;
+#pragma unroll
+        for (int e = 0; e < 6; ++e) {
+            diag += r.cond[e];
+            y += r.cond[e] * (x0 - x[(int64_t)C * d.n1 + lidx_clamped(d, i + so_i(e), j + so_j(e), k + so_k(e))]);
+        }
+        if (NC == 3) {
+#pragma unroll
+            for (int s = 0; s < 8; ++s)
+                y += r.cross[s] * x[(int64_t)xo_c(C, s) * d.n1 +
+                                    lidx_clamped(d, i + xo_i(C, s), j + xo_j(C, s), k + xo_k(C, s))];
+        }
+        v = MODE == 0 ? x0 + omega * (b[q] - y) / diag : b[q] - y;
+    }
+    out[q] = act ? v : (T)0;
+}
+
+template <typename T, typename P, int NC, int MODE>
+__global__ void __launch_bounds__(256) k_mg_cop(P pv, const T *__restrict__ x, const T *__restrict__ b,
+                                                T *__restrict__ out, T omega) {
+    const LevelDims d = pv.dims();
+    int i, j, k;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (!node_of(d, p, i, j, k)) return;
+    cop_comp<0, NC, MODE, T>(pv, d, p, i, j, k, x, b, out, omega);
+    if (NC == 3) {
+        cop_comp<1, NC, MODE, T>(pv, d, p, i, j, k, x, b, out, omega);
+        cop_comp<2, NC, MODE, T>(pv, d, p, i, j, k, x, b, out, omega);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Level 0 of the A block on a z-march: a workgroup owns a 32 x 8 column of
 // nodes and walks it through zt.kz planes; the three components of planes
 // k-1, k, k+1 (with a one-node x/y halo) sit in an LDS ring, so every x value
@@ -377,14 +514,15 @@ __device__ __forceinline__ T apply_lds(const A0View<T> &xv, const Row<T> &r, T &
     x0 = xv.get(C, 0, 0, 0);
     T y = r.exc * x0;
     diag = r.exc;
+    // every LDS value is a finite field value, so a zero coupling adds a zero
+    // product: no branches
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
         diag += r.cond[q];
-        if (r.cond[q] != (T)0) y += r.cond[q] * (x0 - xv.get(C, so_i(q), so_j(q), so_k(q)));
+        y += r.cond[q] * (x0 - xv.get(C, so_i(q), so_j(q), so_k(q)));
     }
 #pragma unroll
-    for (int s = 0; s < 8; ++s)
-        if (r.cross[s] != (T)0) y += r.cross[s] * xv.get(xo_c(C, s), xo_i(C, s), xo_j(C, s), xo_k(C, s));
+    for (int s = 0; s < 8; ++s) y += r.cross[s] * xv.get(xo_c(C, s), xo_i(C, s), xo_j(C, s), xo_k(C, s));
     return y;
 }
 
@@ -505,15 +643,46 @@ __global__ void __launch_bounds__(256) k_mg_prolong(P pv, LevelDims cd, const T 
     }
 }
 
-// Coarsest level: x = Ainv b over its active unknowns (one workgroup; the
-// entries that are not unknowns were zeroed before).
+// Coarsest level: x = Ainv b over its active unknowns, every other entry 0
+// (one workgroup of MG_COARSE_THREADS).  b is gathered into LDS once.  Row r
+// is summed by G threads (G = min(8, threads / n)), thread g over columns
+// [g n / G, (g+1) n / G) in ascending order; the inverse is stored
+// column-major, so the threads of a wave read consecutive entries of a
+// column, and the G partial sums are added in order.
+constexpr int MG_COARSE_MAX = 1024, MG_COARSE_THREADS = 1024;
 template <typename T>
-__global__ void __launch_bounds__(256) k_mg_coarse(int n, const int *__restrict__ idx, const double *__restrict__ inv,
-                                                   const T *__restrict__ b, T *__restrict__ x) {
-    for (int r = threadIdx.x; r < n; r += blockDim.x) {
+__global__ void __launch_bounds__(MG_COARSE_THREADS) k_mg_coarse(int n, int64_t nv, const int *__restrict__ idx,
+                                                                 const double *__restrict__ invT,
+                                                                 const T *__restrict__ b, T *__restrict__ x) {
+    __shared__ double bs[MG_COARSE_MAX];
+    __shared__ double ps[MG_COARSE_THREADS];
+    for (int c = threadIdx.x; c < n; c += blockDim.x) bs[c] = (double)b[idx[c]];
+    for (int64_t q = threadIdx.x; q < nv; q += blockDim.x) x[q] = (T)0;
+    __syncthreads();
+    const int G = n > 0 ? max(1, min(8, (int)blockDim.x / n)) : 1;
+    for (int r0 = 0; r0 < n; r0 += (int)blockDim.x / G) {
+        const int t = (int)threadIdx.x;
+        const int r = r0 + t % ((int)blockDim.x / G), g = t / ((int)blockDim.x / G);
         double s = 0.0;
-        for (int c = 0; c < n; ++c) s += inv[(int64_t)r * n + c] * (double)b[idx[c]];
-        x[idx[r]] = (T)s;
+        if (r < n && g < G) {
+            const int c0 = g * n / G, c1 = (g + 1) * n / G;
+            double a[4] = {0.0, 0.0, 0.0, 0.0};
+            int c = c0;
+            for (; c + 4 <= c1; c += 4) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) a[u] += invT[(int64_t)(c + u) * n + r] * bs[c + u];
+            }
+            for (; c < c1; ++c) a[0] += invT[(int64_t)c * n + r] * bs[c];
+            s = (a[0] + a[1]) + (a[2] + a[3]);
+        }
+        ps[t] = s;
+        __syncthreads();
+        if (r < n && g == 0) {
+            double tot = 0.0;
+            for (int u = 0; u < G; ++u) tot += ps[t + u * ((int)blockDim.x / G)];
+            x[idx[r]] = (T)tot;
+        }
+        __syncthreads();
     }
 }
 
@@ -627,6 +796,15 @@ HostLevel galerkin(const LevelDims &fd, int nc, RowFn row) {
                         for (int s = 0; s < 8; ++s) C.cross[(size_t)((c * 8 + s) * n1 + P)] = cx[s];
                     if (!(diag > 0.0)) throw std::runtime_error("galerkin: coarse row without a positive diagonal");
                     C.dinv[(size_t)(c * n1 + P)] = 1.0 / diag;
+                    // the coarse kernels clamp neighbour reads into the box and rely on every
+                    // coupling that leaves it being zero
+                    for (int q = 0; q < 6; ++q)
+                        if (cc[q] != 0.0 && !inside(C.d, I + so_i(q), J + so_j(q), K + so_k(q)))
+                            throw std::logic_error("galerkin: coarse coupling outside the level's box");
+                    if (nc == 3)
+                        for (int s = 0; s < 8; ++s)
+                            if (cx[s] != 0.0 && !inside(C.d, I + xo_i(c, s), J + xo_j(c, s), K + xo_k(c, s)))
+                                throw std::logic_error("galerkin: coarse coupling outside the level's box");
                 }
             }
     return C;
@@ -661,6 +839,55 @@ void dense_inverse(std::vector<double> &A, int n, std::vector<double> &inv) {
             }
         }
     }
+}
+
+// Distinct row tuples of an explicit level, keyed by their bit patterns
+// (layout MG_CLS_RS_*).  Coarse rows are sums of fine rows in a fixed order,
+// so two aggregates with the same local configuration of materials and
+// Dirichlet shell give bit-identical tuples: a design built from a few boxes
+// has a few thousand.  Returns false (the level keeps its explicit arrays)
+// above 65535 classes.
+bool class_compress(const HostLevel &H, std::vector<uint16_t> &cls, std::vector<double> &tab) {
+    const int nc = H.nc, RS = nc == 3 ? MG_CLS_RS_A : MG_CLS_RS_V, W = nc * RS;
+    const int64_t n1 = H.d.n1;
+    cls.assign((size_t)n1, 0);
+    tab.clear();
+    std::vector<double> t((size_t)W);
+    std::unordered_multimap<uint64_t, uint32_t> seen;
+    seen.reserve(4096);
+    uint32_t ncls = 0;
+    for (int64_t p = 0; p < n1; ++p) {
+        for (int c = 0; c < nc; ++c) {
+            double *r = t.data() + c * RS;
+            for (int q = 0; q < 6; ++q) r[q] = H.cond[(size_t)((c * 6 + q) * n1 + p)];
+            r[6] = H.exc[(size_t)(c * n1 + p)];
+            r[7] = H.dinv[(size_t)(c * n1 + p)];
+            if (nc == 3)
+                for (int s = 0; s < 8; ++s) r[8 + s] = H.cross[(size_t)((c * 8 + s) * n1 + p)];
+        }
+        uint64_t hsh = 0xcbf29ce484222325ull;
+        for (int e = 0; e < W; ++e) {
+            uint64_t u;
+            memcpy(&u, &t[(size_t)e], 8);
+            hsh = (hsh ^ u) * 0x100000001b3ull;
+            hsh ^= hsh >> 31;
+        }
+        uint32_t id = UINT32_MAX;
+        auto range = seen.equal_range(hsh);
+        for (auto it = range.first; it != range.second; ++it)
+            if (memcmp(tab.data() + (size_t)it->second * W, t.data(), sizeof(double) * W) == 0) {
+                id = it->second;
+                break;
+            }
+        if (id == UINT32_MAX) {
+            if (ncls >= 65535) return false;
+            id = ncls++;
+            seen.emplace(hsh, id);
+            tab.insert(tab.end(), t.begin(), t.end());
+        }
+        cls[(size_t)p] = (uint16_t)id;
+    }
+    return true;
 }
 
 template <typename T>
@@ -746,10 +973,21 @@ void mg_setup(const Grid &g, const HostCoef &hc, const CoeffsT<T> &dc, double dt
         MgLevel<T> &L = h.lev[l];
         L.d = H.d;
         L.ncomp = nc;
-        L.cond = upload<T>(H.cond, stage, al);
-        L.exc = upload<T>(H.exc, stage, al);
-        L.dinv = upload<T>(H.dinv, stage, al);
-        L.cross = nc == 3 ? upload<T>(H.cross, stage, al) : nullptr;
+        std::vector<uint16_t> cls;
+        std::vector<double> tab;
+        if (class_compress(H, cls, tab)) {
+            L.ncls = (int)(tab.size() / (size_t)(nc * (nc == 3 ? MG_CLS_RS_A : MG_CLS_RS_V)));
+            L.tab = upload<T>(tab, stage, al);
+            L.cls = (uint16_t *)al.alloc(sizeof(uint16_t) * cls.size());
+            hip_check(hipMemcpyAsync(L.cls, cls.data(), sizeof(uint16_t) * cls.size(), hipMemcpyHostToDevice,
+                                     al.stream), "h2d");
+            hip_check(hipStreamSynchronize(al.stream), "sync");
+        } else {
+            L.cond = upload<T>(H.cond, stage, al);
+            L.exc = upload<T>(H.exc, stage, al);
+            L.dinv = upload<T>(H.dinv, stage, al);
+            L.cross = nc == 3 ? upload<T>(H.cross, stage, al) : nullptr;
+        }
         L.x = (T *)al.alloc(sizeof(T) * (size_t)(nc * H.d.n1));
         L.b = (T *)al.alloc(sizeof(T) * (size_t)(nc * H.d.n1));
         L.r = (T *)al.alloc(sizeof(T) * (size_t)(nc * H.d.n1));
@@ -789,7 +1027,13 @@ void mg_setup(const Grid &g, const HostCoef &hc, const CoeffsT<T> &dc, double dt
                 if (m >= 0) A[(size_t)r * n + m] += rw.cross[s];
             }
     }
+    if (n > MG_COARSE_MAX) throw std::runtime_error("mg_setup: coarsest level too large for the dense solve");
     if (n > 0) dense_inverse(A, n, inv);
+    // column-major for the device solve
+    std::vector<double> invT((size_t)n * n);
+    for (int r = 0; r < n; ++r)
+        for (int c = 0; c < n; ++c) invT[(size_t)c * n + r] = inv[(size_t)r * n + c];
+    inv.swap(invT);
     h.ncoarse = n;
     h.coarse_idx = (int *)al.alloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
     h.coarse_inv = (double *)al.alloc(sizeof(double) * (size_t)(n > 0 ? (size_t)n * n : 1));
@@ -806,11 +1050,13 @@ void mg_setup(const Grid &g, const HostCoef &hc, const CoeffsT<T> &dc, double dt
 // ---------------------------------------------------------------------------
 static inline dim3 g1(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
-template <typename T, int NC>
-static FineProv<T, NC> fine_prov(const Hierarchy<T> &h) {
-    FineProv<T, NC> p;
+template <typename T, int NC, bool CLS>
+static FineProv<T, NC, CLS> fine_prov(const Hierarchy<T> &h) {
+    FineProv<T, NC, CLS> p;
     p.g = h.g;
     for (int a = 0; a < 3; ++a) { p.im[a] = h.im[a]; p.sg[a] = h.sg[a]; }
+    p.cls = h.cls;
+    p.ctab = h.ctab;
     p.fk = h.fk;
     return p;
 }
@@ -818,16 +1064,39 @@ template <typename T, int NC>
 static ExplProv<T, NC> expl_prov(const MgLevel<T> &L) {
     return ExplProv<T, NC>{L.d, L.cond, L.exc, L.cross, L.dinv};
 }
+template <typename T, int NC>
+static ClsProv<T, NC> cls_prov(const MgLevel<T> &L) {
+    return ClsProv<T, NC>{L.d, L.cls, L.tab};
+}
 
-// Dispatch a kernel template on (level 0 or explicit, block).
+template <typename P>
+struct is_coarse_prov : std::false_type {};
+template <typename T, int NC>
+struct is_coarse_prov<ClsProv<T, NC>> : std::true_type {};
+template <typename T, int NC>
+struct is_coarse_prov<ExplProv<T, NC>> : std::true_type {};
+
+// Dispatch a kernel template on the level's row provider (level 0 from the
+// coefficient arrays or the class table; coarse levels class-compressed or
+// explicit) and the block.
 template <typename T, typename F>
 static void on_level(const Hierarchy<T> &h, int l, F f) {
+    using I3 = std::integral_constant<int, 3>;
+    using I1 = std::integral_constant<int, 1>;
     if (l == 0) {
-        if (h.block == 0) f(fine_prov<T, 3>(h), std::integral_constant<int, 3>());
-        else f(fine_prov<T, 1>(h), std::integral_constant<int, 1>());
+        if (h.cls) {
+            if (h.block == 0) f(fine_prov<T, 3, true>(h), I3());
+            else f(fine_prov<T, 1, true>(h), I1());
+        } else {
+            if (h.block == 0) f(fine_prov<T, 3, false>(h), I3());
+            else f(fine_prov<T, 1, false>(h), I1());
+        }
+    } else if (h.lev[l].cls) {
+        if (h.block == 0) f(cls_prov<T, 3>(h.lev[l]), I3());
+        else f(cls_prov<T, 1>(h.lev[l]), I1());
     } else {
-        if (h.block == 0) f(expl_prov<T, 3>(h.lev[l]), std::integral_constant<int, 3>());
-        else f(expl_prov<T, 1>(h.lev[l]), std::integral_constant<int, 1>());
+        if (h.block == 0) f(expl_prov<T, 3>(h.lev[l]), I3());
+        else f(expl_prov<T, 1>(h.lev[l]), I1());
     }
 }
 
@@ -857,8 +1126,13 @@ static void smooth(const Hierarchy<T> &h, int l, const T *x, const T *b, T *xo, 
     on_level(h, l, [&](auto pv, auto nc) {
         using P = decltype(pv);
         constexpr int NC = decltype(nc)::value;
-        if (first) hipLaunchKernelGGL((k_mg_smooth<T, P, NC, true>), g1(n1), dim3(256), 0, s, pv, x, b, xo, w);
-        else hipLaunchKernelGGL((k_mg_smooth<T, P, NC, false>), g1(n1), dim3(256), 0, s, pv, x, b, xo, w);
+        if constexpr (is_coarse_prov<P>::value) {
+            if (first) hipLaunchKernelGGL((k_mg_cop<T, P, NC, 1>), g1(n1), dim3(256), 0, s, pv, x, b, xo, w);
+            else hipLaunchKernelGGL((k_mg_cop<T, P, NC, 0>), g1(n1), dim3(256), 0, s, pv, x, b, xo, w);
+        } else {
+            if (first) hipLaunchKernelGGL((k_mg_smooth<T, P, NC, true>), g1(n1), dim3(256), 0, s, pv, x, b, xo, w);
+            else hipLaunchKernelGGL((k_mg_smooth<T, P, NC, false>), g1(n1), dim3(256), 0, s, pv, x, b, xo, w);
+        }
     });
 }
 template <typename T>
@@ -868,7 +1142,10 @@ static void resid(const Hierarchy<T> &h, int l, const T *x, const T *b, T *r, hi
     on_level(h, l, [&](auto pv, auto nc) {
         using P = decltype(pv);
         constexpr int NC = decltype(nc)::value;
-        hipLaunchKernelGGL((k_mg_resid<T, P, NC>), g1(n1), dim3(256), 0, s, pv, x, b, r);
+        if constexpr (is_coarse_prov<P>::value)
+            hipLaunchKernelGGL((k_mg_cop<T, P, NC, 2>), g1(n1), dim3(256), 0, s, pv, x, b, r, (T)0);
+        else
+            hipLaunchKernelGGL((k_mg_resid<T, P, NC>), g1(n1), dim3(256), 0, s, pv, x, b, r);
     });
 }
 template <typename T>
@@ -888,10 +1165,8 @@ static void vcycle_rec(Hierarchy<T> &h, const MGParams &mp, int l, T *x, const T
     const int nc = L.ncomp;
     const int64_t nv = (int64_t)nc * L.d.n1;
     if (l == h.nlev - 1) {
-        hipLaunchKernelGGL(k_fill<T>, g1(nv), dim3(256), 0, s, x, nv, (T)0);
-        if (h.ncoarse > 0)
-            hipLaunchKernelGGL(k_mg_coarse<T>, dim3(1), dim3(256), 0, s, h.ncoarse, (const int *)h.coarse_idx,
-                               (const double *)h.coarse_inv, b, x);
+        hipLaunchKernelGGL(k_mg_coarse<T>, dim3(1), dim3(MG_COARSE_THREADS), 0, s, h.ncoarse, nv, (const int *)h.coarse_idx,
+                           (const double *)h.coarse_inv, b, x);
         return;
     }
     const T w = (T)mp.omega;

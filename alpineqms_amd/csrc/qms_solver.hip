@@ -309,6 +309,8 @@ void ensure_solver(qms_handle *h, int tr) {
     check(hipStreamSynchronize(h->s), "sync");
     h->st.n_levels_a = ps.hA[tr].nlev;
     h->st.n_levels_v = ps.hV.nlev;
+    h->st.mg_classes_a = ps.hA[tr].nlev > 1 ? ps.hA[tr].lev[1].ncls : 0;
+    h->st.mg_classes_v = ps.hV.nlev > 1 ? ps.hV.lev[1].ncls : 0;
 }
 
 // z = P^-1 v: block lower-triangular [K_A 0; M_VA L_V]^-1 with one V-cycle

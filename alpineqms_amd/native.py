@@ -80,6 +80,8 @@ class Stats(ctypes.Structure):
         ("n_levels_v", ctypes.c_int),
         ("coef_classes", ctypes.c_int),
         ("queued_iters", ctypes.c_int64),
+        ("mg_classes_a", ctypes.c_int),
+        ("mg_classes_v", ctypes.c_int),
     ]
 
 

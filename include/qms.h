@@ -71,6 +71,8 @@ typedef struct {
     int coef_classes;        /* distinct coefficient tuples (0: operator streams the arrays) */
     int64_t queued_iters;    /* iterations queued since creation (>= total_iters: the solver
                                 queues iterations ahead and reads convergence between chunks) */
+    int mg_classes_a, mg_classes_v;   /* row classes of the first coarse multigrid level of the A and
+                                         V blocks (0: that level stores explicit stencil arrays) */
 } qms_stats;
 
 /* Fills defaults: FP64, tol 1e-12, restart 30, 3000 iterations, nu 4,
