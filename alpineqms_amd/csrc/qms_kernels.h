@@ -119,8 +119,16 @@ void launch_vrhs(const Grid &g, const CoeffsD &c, const OpParams &op, const floa
                  hipStream_t s);
 void launch_vrhs(const Grid &g, const CoeffsD &c, const OpParams &op, const double *z, const double *v, double *out,
                  hipStream_t s);
+// The same with v's V field given as its own FP64 array vv (n1 values): the
+// V segment of a mixed-precision Krylov vector.
+void launch_vrhs_seg(const Grid &g, const CoeffsD &c, const OpParams &op, const float *z, const double *vv,
+                     double *out, hipStream_t s);
 // y = D^-1 M x for an FP32-stored x (FP64 arithmetic and output)
 void launch_apply_f32in(const Grid &g, const CoeffsD &c, const OpParams &op, const float *x, double *y, hipStream_t s);
+// y = D^-1 M x with x stored in FP32 and y a mixed-precision Krylov vector
+// (A rows in FP32, the V row in FP64 from byte offset offv); FP64 arithmetic
+void launch_apply_f32(const Grid &g, const CoeffsD &c, const OpParams &op, const float *x, float *y, int64_t offv,
+                      hipStream_t s);
 void launch_hfield(const Grid &g, const CoeffsD &c, const double *x, double *H, int unit_mu, hipStream_t s);
 
 // ---------------------------------------------------------------------------
@@ -233,18 +241,28 @@ void launch_fill(T *x, int64_t n, T v, hipStream_t s);
 // conversions / vector kernels at the Krylov / multigrid boundary
 void launch_d2t(const double *src, const double *scale, float *dst, int64_t n, hipStream_t s);
 void launch_d2t(const double *src, const double *scale, double *dst, int64_t n, hipStream_t s);
+void launch_d2t(const float *src, const double *scale, float *dst, int64_t n, hipStream_t s);
 void launch_cvt(const double *src, float *dst, int64_t n, hipStream_t s);
 
-// Krylov helpers (FP64)
-void launch_multidot(const double *const *V, int nv, const double *w, int64_t n, double *partial, int nblk,
+// Krylov helpers.  A Krylov vector has two segments: the A components (3 n1
+// values of type TE from byte 0) and the V component (n1 FP64 values from
+// byte offset offv).  TE = double with offv = 24 n1 is a flat FP64 vector;
+// TE = float is the mixed-precision basis.  Dots, norms and coefficients are
+// FP64.
+template <typename TE>
+void launch_multidot(const TE *const *V, int nv, const TE *w, int64_t n1, int64_t offv, double *partial, int nblk,
                      hipStream_t s);
 void launch_reduce_partials(const double *partial, int nv, int nblk, double *out, hipStream_t s);
-void launch_orth_update(double *const *V, int nv, const double *h, double *w, int64_t n, double *partial, int nblk,
-                        hipStream_t s);
-void launch_scale(double *dst, const double *src, double a, int64_t n, hipStream_t s);
+template <typename TE>
+void launch_orth_update(TE *const *V, int nv, const double *h, TE *w, int64_t n1, int64_t offv, double *partial,
+                        int nblk, hipStream_t s);
+// dst (two-segment) = src (flat FP64, 4 n1 values) * a
+template <typename TE>
+void launch_scale(TE *dst, const double *src, double a, int64_t n1, int64_t offv, hipStream_t s);
 void launch_axpy_list(double *x, const double *const *Z, int nz, const double *y_dev, int64_t n, hipStream_t s);
 void launch_axpy_list(double *x, const float *const *Z, int nz, const double *y_dev, int64_t n, hipStream_t s);
-void launch_scale_dev(double *dst, const double *src, const double *a_dev, int64_t n, hipStream_t s);
+template <typename TE>
+void launch_scale_dev(TE *dst, const TE *src, const double *a_dev, int64_t n1, int64_t offv, hipStream_t s);
 // FGMRES cycle scalars on the device (st[KS_*])
 enum { KS_BNORM = 0, KS_TOL = 1, KS_INVH = 2, KS_EST = 3, KS_K = 4, KS_ITERS = 5, KS_COUNT = 8 };
 void launch_cycle_start(int m, double beta, double bnorm, double tol, double *g, double *st, hipStream_t s);

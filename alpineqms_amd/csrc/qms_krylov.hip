@@ -49,18 +49,31 @@ __device__ __forceinline__ void block_sum(double acc[K], double *sh /* [RB/64][K
     }
 }
 
+// Krylov vectors have two segments: the A components (3 n1 values of type
+// TE, from byte 0) and the V component (n1 FP64 values from byte offset
+// offv).  TE = double with offv = 24 n1 is a plain FP64 vector; TE = float
+// is the mixed-precision basis (the V component stays FP64: the conductors'
+// floating potentials are fixed only through the 1e-3 S/m background, far
+// below FP32 resolution of the conductor rows).  Every sum is FP64.
+//
 // Basis vectors are read through pointers loaded from memory; marking them
 // global (rather than generic) lets every vector load be a global load from
 // a wave-uniform base.
-typedef const __attribute__((address_space(1))) double *gcptr;
-typedef __attribute__((address_space(1))) double *gptr;
+template <typename TV>
+using gcp = const __attribute__((address_space(1))) TV *;
+template <typename TV>
+using gp = __attribute__((address_space(1))) TV *;
 
-// Element offsets: I = uint32_t when every byte offset fits 32 bits (then all
-// loads of an element share one 32-bit offset on top of wave-uniform base
-// addresses), int64_t otherwise.
-template <typename P, typename I>
-__device__ __forceinline__ auto &at(P p, I off) {
-    return *(P)((__attribute__((address_space(1))) char *)p + off);
+// Element at byte offset off of base: I = uint32_t when every byte offset
+// fits 32 bits (then all loads of an element share one 32-bit offset on top of
+// wave-uniform base addresses), int64_t otherwise.
+template <typename TE, typename P, typename I>
+__device__ __forceinline__ TE ld(P base, I off) {
+    return *(gcp<TE>)((const __attribute__((address_space(1))) char *)base + off);
+}
+template <typename TE, typename P, typename I>
+__device__ __forceinline__ void st(P base, I off, TE v) {
+    *(gp<TE>)((__attribute__((address_space(1))) char *)base + off) = v;
 }
 
 // partial[i*nblk + blk] = sum over this block's elements of V_i . w for the
@@ -68,25 +81,36 @@ __device__ __forceinline__ auto &at(P p, I off) {
 // read once per chunk; the K loads of an element are independent and
 // unconditional (slots past the chunk re-read vector c0, from cache, and
 // their sums are dropped).
-template <int K, typename I>
-__global__ void __launch_bounds__(RB) k_multidot(const double *const *__restrict__ V, int nv, int c0,
-                                                 const double *__restrict__ w, int64_t n,
+template <int K, typename I, typename TE>
+__global__ void __launch_bounds__(RB) k_multidot(const TE *const *__restrict__ V, int nv, int c0,
+                                                 const TE *__restrict__ w, int64_t n1, int64_t offv,
                                                  double *__restrict__ partial) {
     __shared__ double sh[(RB / 64) * K];
     double acc[K];
-    gcptr vp[K];
+    gcp<char> vp[K];
     const int cn = min(K, nv - c0);
 #pragma unroll
     for (int c = 0; c < K; ++c) {
         acc[c] = 0.0;
-        vp[c] = (gcptr)V[c0 + (c < cn ? c : 0)];
+        vp[c] = (gcp<char>)V[c0 + (c < cn ? c : 0)];
     }
-    const gcptr wg = (gcptr)w;
-    const I nb = (I)n * (I)sizeof(double), step = (I)gridDim.x * RB * (I)sizeof(double);
-    for (I off = ((I)blockIdx.x * RB + threadIdx.x) * (I)sizeof(double); off < nb; off += step) {
-        const double we = at(wg, off);
+    const gcp<char> wg = (gcp<char>)w;
+    const I t0 = (I)blockIdx.x * RB + threadIdx.x, nt = (I)gridDim.x * RB;
+    {   // A components
+        const I nb = (I)(3 * n1) * (I)sizeof(TE), step = nt * (I)sizeof(TE);
+        for (I off = t0 * (I)sizeof(TE); off < nb; off += step) {
+            const double we = (double)ld<TE>(wg, off);
 #pragma unroll
-        for (int c = 0; c < K; ++c) acc[c] = fma(at(vp[c], off), we, acc[c]);
+            for (int c = 0; c < K; ++c) acc[c] = fma((double)ld<TE>(vp[c], off), we, acc[c]);
+        }
+    }
+    {   // V component
+        const I nb = (I)offv + (I)n1 * (I)sizeof(double), step = nt * (I)sizeof(double);
+        for (I off = (I)offv + t0 * (I)sizeof(double); off < nb; off += step) {
+            const double we = ld<double>(wg, off);
+#pragma unroll
+            for (int c = 0; c < K; ++c) acc[c] = fma(ld<double>(vp[c], off), we, acc[c]);
+        }
     }
     block_sum<K>(acc, sh);
     if (threadIdx.x == 0) {
@@ -109,34 +133,52 @@ __global__ void k_reduce_partials(const double *__restrict__ partial, int nblk, 
 // basis vectors per pass over w (the pointers and coefficients of the chunk
 // sit in scalar registers; slots past the chunk re-read vector c0 with a
 // zero coefficient).  The pass that ends the basis also writes
-// partial[blk] = sum of w^2 over the block.  Chunks run in order, so the
-// subtractions happen in basis order whatever the chunking.
-template <int K, typename I>
-__global__ void __launch_bounds__(RB) k_orth_update(double *const *__restrict__ V, int nv, int c0,
-                                                    const double *__restrict__ h, double *__restrict__ w, int64_t n,
-                                                    double *__restrict__ partial) {
+// partial[blk] = sum of w^2 over the block (of the stored values).  Chunks
+// run in order, so the subtractions happen in basis order whatever the
+// chunking.  The update is formed in FP64 and stored in each segment's type.
+template <int K, typename I, typename TE>
+__global__ void __launch_bounds__(RB) k_orth_update(TE *const *__restrict__ V, int nv, int c0,
+                                                    const double *__restrict__ h, TE *__restrict__ w, int64_t n1,
+                                                    int64_t offv, double *__restrict__ partial) {
     __shared__ double sh[RB / 64];
-    gcptr vp[K];
+    gcp<char> vp[K];
     double hc[K];
     const int cn = min(K, nv - c0);
 #pragma unroll
     for (int c = 0; c < K; ++c) {
-        vp[c] = (gcptr)V[c0 + (c < cn ? c : 0)];
+        vp[c] = (gcp<char>)V[c0 + (c < cn ? c : 0)];
         hc[c] = c < cn ? h[c0 + c] : 0.0;
     }
     const bool last = c0 + cn == nv;
     double acc[1] = {0.0};
-    const gptr wg = (gptr)w;
-    const I nb = (I)n * (I)sizeof(double), step = (I)gridDim.x * RB * (I)sizeof(double);
-    for (I off = ((I)blockIdx.x * RB + threadIdx.x) * (I)sizeof(double); off < nb; off += step) {
-        double ve[K];
+    const gp<char> wg = (gp<char>)w;
+    const I t0 = (I)blockIdx.x * RB + threadIdx.x, nt = (I)gridDim.x * RB;
+    {
+        const I nb = (I)(3 * n1) * (I)sizeof(TE), step = nt * (I)sizeof(TE);
+        for (I off = t0 * (I)sizeof(TE); off < nb; off += step) {
+            double ve[K];
 #pragma unroll
-        for (int c = 0; c < K; ++c) ve[c] = at(vp[c], off);
-        double we = at(wg, off);
+            for (int c = 0; c < K; ++c) ve[c] = (double)ld<TE>(vp[c], off);
+            double we = (double)ld<TE>(wg, off);
 #pragma unroll
-        for (int c = 0; c < K; ++c) we -= hc[c] * ve[c];
-        at(wg, off) = we;
-        acc[0] += we * we;
+            for (int c = 0; c < K; ++c) we -= hc[c] * ve[c];
+            const TE ws = (TE)we;
+            st<TE>(wg, off, ws);
+            acc[0] += (double)ws * (double)ws;
+        }
+    }
+    {
+        const I nb = (I)offv + (I)n1 * (I)sizeof(double), step = nt * (I)sizeof(double);
+        for (I off = (I)offv + t0 * (I)sizeof(double); off < nb; off += step) {
+            double ve[K];
+#pragma unroll
+            for (int c = 0; c < K; ++c) ve[c] = ld<double>(vp[c], off);
+            double we = ld<double>(wg, off);
+#pragma unroll
+            for (int c = 0; c < K; ++c) we -= hc[c] * ve[c];
+            st<double>(wg, off, we);
+            acc[0] += we * we;
+        }
     }
     if (last) {
         block_sum<1>(acc, sh);
@@ -155,11 +197,14 @@ __global__ void __launch_bounds__(RB) k_norm2(const double *__restrict__ w, int6
     if (threadIdx.x == 0) partial[blockIdx.x] = acc[0];
 }
 
-// dst = src * a
-__global__ void k_scale(double *__restrict__ dst, const double *__restrict__ src, double a, int64_t n) {
+// dst = src * a for a flat FP64 vector src of 4 n1 values into a two-segment
+// vector dst (FP64 product, stored in the segment's type)
+template <typename TE>
+__global__ void k_scale(TE *__restrict__ dst, const double *__restrict__ src, double a, int64_t n1, int64_t offv) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    dst[e] = src[e] * a;
+    if (e >= 4 * n1) return;
+    if (e < 3 * n1) dst[e] = (TE)(src[e] * a);
+    else ((double *)((char *)dst + offv))[e - 3 * n1] = src[e] * a;
 }
 
 // x += sum_i y_i Z_i (Z in the preconditioner's precision; FP64 sums)
@@ -173,12 +218,19 @@ __global__ void __launch_bounds__(256) k_axpy_list(double *__restrict__ x, const
     x[e] = s;
 }
 
-// dst = src * a[0] (a device scalar, so the launch needs no host value)
-__global__ void __launch_bounds__(256) k_scale_dev(double *__restrict__ dst, const double *__restrict__ src,
-                                                   const double *__restrict__ a, int64_t n) {
+// dst = src * a[0] for two-segment vectors (a device scalar, so the launch
+// needs no host value)
+template <typename TE>
+__global__ void __launch_bounds__(256) k_scale_dev(TE *__restrict__ dst, const TE *__restrict__ src,
+                                                   const double *__restrict__ a, int64_t n1, int64_t offv) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    dst[e] = src[e] * a[0];
+    if (e >= 4 * n1) return;
+    if (e < 3 * n1) {
+        dst[e] = (TE)((double)src[e] * a[0]);
+    } else {
+        const int64_t q = e - 3 * n1;
+        ((double *)((char *)dst + offv))[q] = ((const double *)((const char *)src + offv))[q] * a[0];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -273,53 +325,71 @@ void launch_copy(double *dst, const double *src, int64_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dst, src, n);
 }
 
-static bool offsets32(int64_t n) { return n < ((int64_t)1 << 28); }   // byte offsets of n doubles (+ one grid stride) < 2^32
 
-void launch_multidot(const double *const *V, int nv, const double *w, int64_t n, double *partial, int nblk,
+template <typename TE>
+void launch_multidot(const TE *const *V, int nv, const TE *w, int64_t n1, int64_t offv, double *partial, int nblk,
                      hipStream_t s) {
     // chunks of up to 16 vectors; the narrower kernel when the chunk fits it
+    const bool o32 = offv + 8 * n1 + (int64_t)nblk * RB * 8 < ((int64_t)1 << 32);
     for (int c0 = 0; c0 < nv; c0 += 16) {
         const int cn = nv - c0 < 16 ? nv - c0 : 16;
-        if (offsets32(n)) {
-            if (cn <= 8) hipLaunchKernelGGL((k_multidot<8, uint32_t>), dim3(nblk), dim3(RB), 0, s, V, nv, c0, w, n, partial);
-            else hipLaunchKernelGGL((k_multidot<16, uint32_t>), dim3(nblk), dim3(RB), 0, s, V, nv, c0, w, n, partial);
+        if (o32) {
+            if (cn <= 8) hipLaunchKernelGGL((k_multidot<8, uint32_t, TE>), dim3(nblk), dim3(RB), 0, s, V, nv, c0, w, n1, offv, partial);
+            else hipLaunchKernelGGL((k_multidot<16, uint32_t, TE>), dim3(nblk), dim3(RB), 0, s, V, nv, c0, w, n1, offv, partial);
         } else {
-            if (cn <= 8) hipLaunchKernelGGL((k_multidot<8, int64_t>), dim3(nblk), dim3(RB), 0, s, V, nv, c0, w, n, partial);
-            else hipLaunchKernelGGL((k_multidot<16, int64_t>), dim3(nblk), dim3(RB), 0, s, V, nv, c0, w, n, partial);
+            if (cn <= 8) hipLaunchKernelGGL((k_multidot<8, int64_t, TE>), dim3(nblk), dim3(RB), 0, s, V, nv, c0, w, n1, offv, partial);
+            else hipLaunchKernelGGL((k_multidot<16, int64_t, TE>), dim3(nblk), dim3(RB), 0, s, V, nv, c0, w, n1, offv, partial);
         }
     }
 }
 void launch_reduce_partials(const double *partial, int nv, int nblk, double *out, hipStream_t s) {
     hipLaunchKernelGGL(k_reduce_partials, dim3(nv), dim3(RB), 0, s, partial, nblk, out);
 }
-void launch_orth_update(double *const *V, int nv, const double *h, double *w, int64_t n, double *partial, int nblk,
-                        hipStream_t s) {
+template <typename TE>
+void launch_orth_update(TE *const *V, int nv, const double *h, TE *w, int64_t n1, int64_t offv, double *partial,
+                        int nblk, hipStream_t s) {
+    const bool o32 = offv + 8 * n1 + (int64_t)nblk * RB * 8 < ((int64_t)1 << 32);
     for (int c0 = 0; c0 < nv; c0 += 16) {
         const int cn = nv - c0 < 16 ? nv - c0 : 16;
-        if (offsets32(n)) {
-            if (cn <= 8) hipLaunchKernelGGL((k_orth_update<8, uint32_t>), dim3(nblk), dim3(RB), 0, s, V, nv, c0, h, w, n, partial);
-            else hipLaunchKernelGGL((k_orth_update<16, uint32_t>), dim3(nblk), dim3(RB), 0, s, V, nv, c0, h, w, n, partial);
+        if (o32) {
+            if (cn <= 8) hipLaunchKernelGGL((k_orth_update<8, uint32_t, TE>), dim3(nblk), dim3(RB), 0, s, V, nv, c0, h, w, n1, offv, partial);
+            else hipLaunchKernelGGL((k_orth_update<16, uint32_t, TE>), dim3(nblk), dim3(RB), 0, s, V, nv, c0, h, w, n1, offv, partial);
         } else {
-            if (cn <= 8) hipLaunchKernelGGL((k_orth_update<8, int64_t>), dim3(nblk), dim3(RB), 0, s, V, nv, c0, h, w, n, partial);
-            else hipLaunchKernelGGL((k_orth_update<16, int64_t>), dim3(nblk), dim3(RB), 0, s, V, nv, c0, h, w, n, partial);
+            if (cn <= 8) hipLaunchKernelGGL((k_orth_update<8, int64_t, TE>), dim3(nblk), dim3(RB), 0, s, V, nv, c0, h, w, n1, offv, partial);
+            else hipLaunchKernelGGL((k_orth_update<16, int64_t, TE>), dim3(nblk), dim3(RB), 0, s, V, nv, c0, h, w, n1, offv, partial);
         }
     }
 }
+template void launch_multidot<double>(const double *const *, int, const double *, int64_t, int64_t, double *, int,
+                                      hipStream_t);
+template void launch_multidot<float>(const float *const *, int, const float *, int64_t, int64_t, double *, int,
+                                     hipStream_t);
+template void launch_orth_update<double>(double *const *, int, const double *, double *, int64_t, int64_t, double *,
+                                         int, hipStream_t);
+template void launch_orth_update<float>(float *const *, int, const double *, float *, int64_t, int64_t, double *, int,
+                                        hipStream_t);
 void launch_norm2(const double *w, int64_t n, double *partial, int nblk, hipStream_t s) {
     hipLaunchKernelGGL(k_norm2, dim3(nblk), dim3(RB), 0, s, w, n, partial);
 }
-void launch_scale(double *dst, const double *src, double a, int64_t n, hipStream_t s) {
-    hipLaunchKernelGGL(k_scale, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dst, src, a, n);
+template <typename TE>
+void launch_scale(TE *dst, const double *src, double a, int64_t n1, int64_t offv, hipStream_t s) {
+    hipLaunchKernelGGL(k_scale<TE>, dim3((unsigned)((4 * n1 + 255) / 256)), dim3(256), 0, s, dst, src, a, n1, offv);
 }
+template void launch_scale<double>(double *, const double *, double, int64_t, int64_t, hipStream_t);
+template void launch_scale<float>(float *, const double *, double, int64_t, int64_t, hipStream_t);
 void launch_axpy_list(double *x, const double *const *Z, int nz, const double *y_dev, int64_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_axpy_list<double>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, Z, nz, y_dev, n);
 }
 void launch_axpy_list(double *x, const float *const *Z, int nz, const double *y_dev, int64_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_axpy_list<float>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, Z, nz, y_dev, n);
 }
-void launch_scale_dev(double *dst, const double *src, const double *a_dev, int64_t n, hipStream_t s) {
-    hipLaunchKernelGGL(k_scale_dev, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dst, src, a_dev, n);
+template <typename TE>
+void launch_scale_dev(TE *dst, const TE *src, const double *a_dev, int64_t n1, int64_t offv, hipStream_t s) {
+    hipLaunchKernelGGL(k_scale_dev<TE>, dim3((unsigned)((4 * n1 + 255) / 256)), dim3(256), 0, s, dst, src, a_dev, n1,
+                       offv);
 }
+template void launch_scale_dev<double>(double *, const double *, const double *, int64_t, int64_t, hipStream_t);
+template void launch_scale_dev<float>(float *, const float *, const double *, int64_t, int64_t, hipStream_t);
 void launch_cycle_start(int m, double beta, double bnorm, double tol, double *g, double *st, hipStream_t s) {
     hipLaunchKernelGGL(k_cycle_start, dim3(1), dim3(64), 0, s, m, beta, bnorm, tol, g, st);
 }

@@ -1222,11 +1222,11 @@ template void launch_fill<double>(double *, int64_t, double, hipStream_t);
 // ---------------------------------------------------------------------------
 // Vector conversions at the Krylov / multigrid boundary
 // ---------------------------------------------------------------------------
-template <typename TD>
-__global__ void __launch_bounds__(256) k_d2t(const double *__restrict__ src, const double *__restrict__ scale,
+template <typename TD, typename TS = double>
+__global__ void __launch_bounds__(256) k_d2t(const TS *__restrict__ src, const double *__restrict__ scale,
                                              TD *__restrict__ dst, int64_t n) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < n) dst[p] = (TD)(src[p] * scale[p]);
+    if (p < n) dst[p] = (TD)((double)src[p] * scale[p]);
 }
 __global__ void __launch_bounds__(256) k_cvt(const double *__restrict__ src, float *__restrict__ dst, int64_t n) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1237,6 +1237,9 @@ void launch_d2t(const double *src, const double *scale, float *dst, int64_t n, h
 }
 void launch_d2t(const double *src, const double *scale, double *dst, int64_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_d2t<double>, g1(n), dim3(256), 0, s, src, scale, dst, n);
+}
+void launch_d2t(const float *src, const double *scale, float *dst, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL((k_d2t<float, float>), g1(n), dim3(256), 0, s, src, scale, dst, n);
 }
 void launch_cvt(const double *src, float *dst, int64_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_cvt, g1(n), dim3(256), 0, s, src, dst, n);

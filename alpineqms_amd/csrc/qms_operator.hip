@@ -507,10 +507,16 @@ __device__ __forceinline__ ColSpan zspan(const Grid &g, const ZTiling &t, unsign
 // carried plane-(kb-1) values are recomputed by the same expressions), so any
 // split of a column into spans gives the same bits.  STG selects the staging
 // form (see `stage`); the results do not depend on it.
-template <typename TIN, typename TOUT, int MODE, int TX, bool CLS, bool STG = false>
+//
+// Output: with TOUT = float the output is a two-segment Krylov vector (the A
+// rows in FP32 from byte 0, the V row in FP64 from byte offset `offv`).
+// OP_VRHS reads the V field of b at element offset `bv` (3 n1 in a flat
+// vector, 0 when b points at a two-segment vector's V segment).
+template <typename TIN, typename TOUT, int MODE, int TX, bool CLS, bool STG = false, typename TB = double>
 __device__ __forceinline__ void op_column(const Grid &gin, const ColSpan cs, const CoeffsD &c, const OpK &k,
-                                          const TIN *__restrict__ x, const double *__restrict__ b,
-                                          TOUT *__restrict__ out, double *lds) {
+                                          const TIN *__restrict__ x, const TB *__restrict__ b,
+                                          TOUT *__restrict__ out, double *lds, int64_t offv = 0,
+                                          int64_t bv = -1) {
     // The node counts, held in scalar registers for the whole z-march: where
     // the grid comes from memory (the batched kernel) the compiler would
     // otherwise reload them in every plane, and each reload's wait also
@@ -607,7 +613,7 @@ __device__ __forceinline__ void op_column(const Grid &gin, const ColSpan cs, con
         if constexpr (MODE == OP_RESID && CLS) {
             if (live) {
 #pragma unroll
-                for (int f = 0; f < 4; ++f) bq[f] = b[f * n1 + p];
+                for (int f = 0; f < 4; ++f) bq[f] = (double)b[f * n1 + p];
             }
         }
         typename std::conditional<CLS, ZmClassView, ZmCView>::type cv;
@@ -637,7 +643,7 @@ __device__ __forceinline__ void op_column(const Grid &gin, const ColSpan cs, con
                 if (free_v(g, i, j, kk)) {
                     double d;
                     const double va = row_v(xv, cv, k, B_VA, d);
-                    rv = d * b[3 * n1 + p] - va;
+                    rv = d * (double)b[(bv < 0 ? 3 * n1 : bv) + p] - va;
                 }
                 out[p] = (TOUT)rv;
             }
@@ -651,14 +657,17 @@ __device__ __forceinline__ void op_column(const Grid &gin, const ColSpan cs, con
                     const int64_t q = f * n1 + p;
                     double rr;
                     if (free_f(g, f, i, j, kk)) {
-                        rr = ((MODE == OP_RESID) ? (CLS ? bq[f] : b[q]) - mx : mx) * recip(d);
+                        rr = ((MODE == OP_RESID) ? (CLS ? bq[f] : (double)b[q]) - mx : mx) * recip(d);
                     } else if (exists_f(g, f, i, j, kk)) {
                         const double xq = xv.get(f, 0, 0, 0);
-                        rr = (MODE == OP_RESID) ? (CLS ? bq[f] : b[q]) - xq : xq;
+                        rr = (MODE == OP_RESID) ? (CLS ? bq[f] : (double)b[q]) - xq : xq;
                     } else {
                         rr = 0.0;
                     }
-                    out[q] = (TOUT)rr;
+                    if (std::is_same<TOUT, float>::value && f == 3)
+                        ((double *)((char *)out + offv))[p] = rr;
+                    else
+                        out[q] = (TOUT)rr;
                 };
                 emit(0, rax, dax);
                 emit(1, ray, day);
@@ -682,17 +691,18 @@ __device__ __forceinline__ OpK op_consts(const OpK &k, const OpK *__restrict__ k
     else return k;
 }
 
-template <typename TIN, typename TOUT, int MODE, int TX, bool CLS, bool KP>
+template <typename TIN, typename TOUT, int MODE, int TX, bool CLS, bool KP, typename TB>
 __global__ void __launch_bounds__(256) k_operator(Grid g, ZTiling t, CoeffsD c, OpK k0, const OpK *__restrict__ kp,
-                                                  const TIN *__restrict__ x, const double *__restrict__ b,
-                                                  TOUT *__restrict__ out) {
+                                                  const TIN *__restrict__ x, const TB *__restrict__ b,
+                                                  TOUT *__restrict__ out, int64_t offv, int64_t bv) {
     __shared__ double lds[4 * 3 * ZShape<TX>::PL];
     const OpK k = op_consts<KP>(k0, kp);
     // The unrolled staging is used where the kernel still fits in 128 VGPRs
     // (four waves per SIMD) with it: the class-table apply on 32-wide
     // columns.  Elsewhere it costs a wave per SIMD and runs slower.
     constexpr bool STG = MODE == OP_APPLY && TX == 32 && CLS;
-    op_column<TIN, TOUT, MODE, TX, CLS, STG>(g, zspan(g, t, xcd_tile(blockIdx.x, gridDim.x)), c, k, x, b, out, lds);
+    op_column<TIN, TOUT, MODE, TX, CLS, STG, TB>(g, zspan(g, t, xcd_tile(blockIdx.x, gridDim.x)), c, k, x, b, out,
+                                                 lds, offv, bv);
 }
 
 // ---------------------------------------------------------------------------
@@ -735,17 +745,17 @@ __device__ __forceinline__ void batch_segment(const BatchItem *__restrict__ item
     const double *x = ci.x;
     double *y = ci.y;
     if constexpr (TX != 0) {
-        op_column<double, double, OP_APPLY, TX, CLS, STG>(it.g, cs, c, it.k, x, nullptr, y, lds);
+        op_column<double, double, OP_APPLY, TX, CLS, STG>(it.g, cs, c, it.k, x, (const double *)nullptr, y, lds);
     } else if (c.cls) {
         if (it.t.tx == 32)
-            op_column<double, double, OP_APPLY, 32, true>(it.g, cs, c, it.k, x, nullptr, y, lds);
+            op_column<double, double, OP_APPLY, 32, true>(it.g, cs, c, it.k, x, (const double *)nullptr, y, lds);
         else
-            op_column<double, double, OP_APPLY, 64, true>(it.g, cs, c, it.k, x, nullptr, y, lds);
+            op_column<double, double, OP_APPLY, 64, true>(it.g, cs, c, it.k, x, (const double *)nullptr, y, lds);
     } else {
         if (it.t.tx == 32)
-            op_column<double, double, OP_APPLY, 32, false>(it.g, cs, c, it.k, x, nullptr, y, lds);
+            op_column<double, double, OP_APPLY, 32, false>(it.g, cs, c, it.k, x, (const double *)nullptr, y, lds);
         else
-            op_column<double, double, OP_APPLY, 64, false>(it.g, cs, c, it.k, x, nullptr, y, lds);
+            op_column<double, double, OP_APPLY, 64, false>(it.g, cs, c, it.k, x, (const double *)nullptr, y, lds);
     }
 }
 
@@ -937,9 +947,9 @@ ZTiling ztiling(const Grid &g, int kz, int tx, unsigned &ncol) {
     return t;
 }
 
-template <typename TIN, typename TOUT, int MODE>
-static void launch_operator(const Grid &g, const CoeffsD &c, const OpParams &op, const TIN *x, const double *b,
-                            TOUT *out, hipStream_t s) {
+template <typename TIN, typename TOUT, int MODE, typename TB = double>
+static void launch_operator(const Grid &g, const CoeffsD &c, const OpParams &op, const TIN *x, const TB *b,
+                            TOUT *out, hipStream_t s, int64_t offv = 0, int64_t bv = -1) {
     unsigned ncol;
     const ZTiling t = ztiling(g, op.zchunk, op.colw, ncol);
     const OpK k = make_opk(g, op);
@@ -948,7 +958,8 @@ static void launch_operator(const Grid &g, const CoeffsD &c, const OpParams &op,
     auto go = [&](auto tx, auto cls, auto kpc) {
         constexpr int TX = decltype(tx)::value;
         constexpr bool CLS = decltype(cls)::value, KP = decltype(kpc)::value;
-        hipLaunchKernelGGL((k_operator<TIN, TOUT, MODE, TX, CLS, KP>), grid, block, 0, s, g, t, c, k, kp, x, b, out);
+        hipLaunchKernelGGL((k_operator<TIN, TOUT, MODE, TX, CLS, KP, TB>), grid, block, 0, s, g, t, c, k, kp, x, b,
+                           out, offv, bv);
     };
     using T32 = std::integral_constant<int, 32>;
     using T64 = std::integral_constant<int, 64>;
@@ -1025,10 +1036,10 @@ void launch_resid(const Grid &g, const CoeffsD &c, const OpParams &op, const dou
     launch_operator<double, double, OP_RESID>(g, c, op, x, b, r, s);
 }
 void launch_apply(const Grid &g, const CoeffsD &c, const OpParams &op, const double *x, double *y, hipStream_t s) {
-    launch_operator<double, double, OP_APPLY>(g, c, op, x, nullptr, y, s);
+    launch_operator<double, double, OP_APPLY>(g, c, op, x, (const double *)nullptr, y, s);
 }
 void launch_apply_f32in(const Grid &g, const CoeffsD &c, const OpParams &op, const float *x, double *y, hipStream_t s) {
-    launch_operator<float, double, OP_APPLY>(g, c, op, x, nullptr, y, s);
+    launch_operator<float, double, OP_APPLY>(g, c, op, x, (const double *)nullptr, y, s);
 }
 void launch_vrhs(const Grid &g, const CoeffsD &c, const OpParams &op, const float *z, const double *v, double *out,
                  hipStream_t s) {
@@ -1037,6 +1048,14 @@ void launch_vrhs(const Grid &g, const CoeffsD &c, const OpParams &op, const floa
 void launch_vrhs(const Grid &g, const CoeffsD &c, const OpParams &op, const double *z, const double *v, double *out,
                  hipStream_t s) {
     launch_operator<double, double, OP_VRHS>(g, c, op, z, v, out, s);
+}
+void launch_vrhs_seg(const Grid &g, const CoeffsD &c, const OpParams &op, const float *z, const double *vv,
+                     double *out, hipStream_t s) {
+    launch_operator<float, double, OP_VRHS>(g, c, op, z, vv, out, s, 0, 0);
+}
+void launch_apply_f32(const Grid &g, const CoeffsD &c, const OpParams &op, const float *x, float *y, int64_t offv,
+                      hipStream_t s) {
+    launch_operator<float, float, OP_APPLY>(g, c, op, x, (const double *)nullptr, y, s, offv);
 }
 void launch_hfield(const Grid &g, const CoeffsD &c, const double *x, double *H, int unit_mu, hipStream_t s) {
     const int64_t nc = (int64_t)(g.nx - 1) * (g.ny - 1) * (g.nz - 1);

@@ -67,6 +67,22 @@ double time_on(hipStream_t s, int reps, const std::function<void()> &f) {
 
 }  // namespace
 
+// Krylov basis: V_0..V_m and the vector w being orthogonalised (the
+// operator's output).  Each vector holds the A components as 3 n1 values of
+// type TE from byte 0 and the V component as n1 FP64 values from byte offset
+// offv: TE = double is a flat FP64 vector (offv = 24 n1); TE = float is the
+// mixed-precision basis, whose V component stays FP64 because the
+// conductors' floating potentials, fixed only through the 1e-3 S/m
+// background, are below FP32 resolution of the conductor rows (an FP32 V
+// component stalls the transient solves near a 1e-8 relative residual).
+template <typename TE>
+struct KBasis {
+    std::vector<TE *> V;
+    TE **dV = nullptr;
+    TE *w = nullptr;
+    int64_t offv = 0;
+};
+
 // Preconditioner state.  The A block's multigrid and the FGMRES Z basis
 // (z_j = P^-1 v_j) are in TA (float in mixed precision, double otherwise);
 // the V block's multigrid is always FP64: the conductors' floating
@@ -110,8 +126,10 @@ struct qms_handle {
     double *Dst = nullptr, *Dtr = nullptr;   // row scaling, steady / transient
     double *xh1 = nullptr, *xh2 = nullptr;   // solutions of the last two steps (initial guess)
     int nhist = 0;                           // how many of them are valid
-    std::vector<double *> V;
-    double **dV = nullptr;
+    // Krylov bases (allocated on first use): FP64 for the FP64 solve, FP32
+    // for the mixed-precision inner solve
+    KBasis<double> kb64;
+    KBasis<float> kb32;
     double *partial = nullptr, *dsmall = nullptr;
     double *hsmall = nullptr;   // pinned
     // FGMRES bookkeeping on the device
@@ -273,22 +291,47 @@ template <> CoeffsT<double> dev_coef<double>(const qms_handle *h) {
                            h->use_cls ? h->cls : nullptr, h->use_cls ? h->ctab : nullptr};
 }
 
-void launch_apply_t(qms_handle *h, const OpParams &op, const float *x, double *y) {
-    launch_apply_f32in(h->g, op_coeffs(h), op, x, y, h->s);
+// The Krylov basis of storage type TV (allocated on first use, outside any
+// graph capture).  The FP64 basis's w is the handle's work vector.
+template <typename TV> KBasis<TV> &kbasis(qms_handle *h);
+template <> KBasis<double> &kbasis<double>(qms_handle *h) { return h->kb64; }
+template <> KBasis<float> &kbasis<float>(qms_handle *h) { return h->kb32; }
+
+// Byte offset of the V segment of a Krylov vector with A components of type TE
+// (256-byte aligned when it does not follow on directly).
+template <typename TE>
+int64_t seg_offv(int64_t n1) {
+    if (std::is_same<TE, double>::value) return 24 * n1;
+    return (int64_t)((3 * n1 * (int64_t)sizeof(TE) + 255) / 256 * 256);
 }
-void launch_apply_t(qms_handle *h, const OpParams &op, const double *x, double *y) {
-    launch_apply(h->g, op_coeffs(h), op, x, y, h->s);
+
+template <typename TE>
+KBasis<TE> &ensure_basis(qms_handle *h) {
+    KBasis<TE> &kb = kbasis<TE>(h);
+    if (kb.V.empty()) {
+        const int m = h->cfg.restart;
+        kb.offv = seg_offv<TE>(h->g.n1);
+        const size_t nb = (size_t)kb.offv + 8 * (size_t)h->g.n1;
+        for (int j = 0; j <= m; ++j) kb.V.push_back((TE *)h->raw_alloc(nb));
+        kb.dV = h->dalloc<TE *>(m + 1);
+        check(hipMemcpyAsync(kb.dV, kb.V.data(), sizeof(TE *) * (m + 1), hipMemcpyHostToDevice, h->s), "h2d");
+        if constexpr (std::is_same<TE, double>::value) kb.w = h->w;
+        else kb.w = (TE *)h->raw_alloc(nb);
+        check(hipStreamSynchronize(h->s), "sync");
+    }
+    return kb;
 }
 
 // Everything the solve in precision TA needs, allocated and built before any
 // kernel of the solve is queued: the multigrid hierarchies (host-built
-// Galerkin levels, uploaded), the Z basis and the preconditioner's work
-// vectors.
+// Galerkin levels, uploaded), the Krylov basis, the Z basis and the
+// preconditioner's work vectors.
 template <typename TA>
 void ensure_solver(qms_handle *h, int tr) {
     PrecondState<TA> &ps = pstate<TA>(h);
     const HostCoef hc = h->host_coef();
     const MgAlloc al = h->mg_alloc();
+    ensure_basis<TA>(h);
     if (ps.Z.empty()) {
         const int m = h->cfg.restart;
         for (int j = 0; j < m; ++j) ps.Z.push_back(h->dalloc<TA>(h->n));
@@ -314,16 +357,21 @@ void ensure_solver(qms_handle *h, int tr) {
 }
 
 // z = P^-1 v: block lower-triangular [K_A 0; M_VA L_V]^-1 with one V-cycle
-// per diagonal block, applied to D v (unscaled residual units).  The V block's
-// right-hand side D_V v_V - M_VA z_A is formed in FP64 by the operator kernel.
+// per diagonal block, applied to D v (unscaled residual units).  The V
+// block's right-hand side D_V v_V - M_VA z_A is formed in FP64 by the
+// operator kernel and its V-cycle runs in FP64.  TA: the A block's multigrid,
+// z and the A segment of the Krylov vector v (V segment from byte offv).
 template <typename TA>
-void precond(qms_handle *h, const OpParams &op, int tr, const double *D, const double *v, TA *z) {
+void precond(qms_handle *h, const OpParams &op, int tr, const double *D, const TA *v, int64_t offv, TA *z) {
     PrecondState<TA> &ps = pstate<TA>(h);
     const int64_t n1 = h->g.n1;
     const MGParams mp = mgp(h);
     launch_d2t(v, D, ps.rT, 3 * n1, h->s);
     mg_vcycle<TA>(ps.hA[tr], mp, z, ps.rT, h->s);
-    launch_vrhs(h->g, op_coeffs(h), op, z, v, ps.rV, h->s);
+    if constexpr (std::is_same<TA, double>::value)
+        launch_vrhs(h->g, op_coeffs(h), op, z, v, ps.rV, h->s);
+    else
+        launch_vrhs_seg(h->g, op_coeffs(h), op, z, (const double *)((const char *)v + offv), ps.rV, h->s);
     if constexpr (std::is_same<TA, double>::value) {
         mg_vcycle<double>(ps.hV, mp, z + 3 * n1, ps.rV, h->s);
     } else {
@@ -341,14 +389,22 @@ double norm_dev(qms_handle *h, const double *w) {
     return sqrt(h->hsmall[0]);
 }
 
-// Classical Gram-Schmidt of h->w against basis vectors V[0..nv-1]: the dots
+// Classical Gram-Schmidt of kb.w against basis vectors V[0..nv-1]: the dots
 // land in dsmall[0..nv-1], w is updated in place and |w|^2 lands in
-// dsmall[nv] (one fused pass for the dots, one for the update).
-void gram_schmidt(qms_handle *h, int nv) {
-    launch_multidot((const double *const *)h->dV, nv, h->w, h->n, h->partial, h->nblk, h->s);
+// dsmall[nv] (one fused pass for the dots, one for the update; FP64 sums).
+template <typename TE>
+void gram_schmidt(qms_handle *h, KBasis<TE> &kb, int nv) {
+    launch_multidot<TE>((const TE *const *)kb.dV, nv, kb.w, h->g.n1, kb.offv, h->partial, h->nblk, h->s);
     launch_reduce_partials(h->partial, nv, h->nblk, h->dsmall, h->s);
-    launch_orth_update(h->dV, nv, h->dsmall, h->w, h->n, h->partial, h->nblk, h->s);
+    launch_orth_update<TE>(kb.dV, nv, h->dsmall, kb.w, h->g.n1, kb.offv, h->partial, h->nblk, h->s);
     launch_reduce_partials(h->partial, 1, h->nblk, h->dsmall + nv, h->s);
+}
+
+void launch_apply_t(qms_handle *h, const OpParams &op, const double *x, double *y, int64_t) {
+    launch_apply(h->g, op_coeffs(h), op, x, y, h->s);
+}
+void launch_apply_t(qms_handle *h, const OpParams &op, const float *x, float *y, int64_t offv) {
+    launch_apply_f32(h->g, op_coeffs(h), op, x, y, offv, h->s);
 }
 
 // One FGMRES iteration j, queued with no host round trip: z_j = P^-1 v_j,
@@ -356,12 +412,13 @@ void gram_schmidt(qms_handle *h, int nv) {
 template <typename T>
 void fgmres_iteration(qms_handle *h, const OpParams &op, int tr, const double *D, int j) {
     PrecondState<T> &ps = pstate<T>(h);
+    KBasis<T> &kb = kbasis<T>(h);
     const int m = h->cfg.restart;
-    precond<T>(h, op, tr, D, h->V[j], ps.Z[j]);
-    launch_apply_t(h, op, ps.Z[j], h->w);
-    gram_schmidt(h, j + 1);
+    precond<T>(h, op, tr, D, kb.V[j], kb.offv, ps.Z[j]);
+    launch_apply_t(h, op, ps.Z[j], kb.w, kb.offv);
+    gram_schmidt<T>(h, kb, j + 1);
     launch_givens(j, m, h->dsmall, h->Hd, h->csd, h->snd, h->gd, h->std_, h->s);
-    launch_scale_dev(h->V[j + 1], h->w, h->std_ + KS_INVH, h->n, h->s);
+    launch_scale_dev<T>(kb.V[j + 1], kb.w, h->std_ + KS_INVH, h->g.n1, kb.offv, h->s);
 }
 
 // Iteration j as a captured graph (captured on first use; every argument it
@@ -410,12 +467,23 @@ void read_cycle(qms_handle *h, double out[KS_COUNT]) {
 }
 
 // Restarted FGMRES on D^-1 M x = D^-1 b with an FP64 residual at every
-// restart.  Iterations are queued ahead in chunks (the first chunk as long as
-// the last solve took), and the host reads the device's convergence record
+// restart.  T is the storage precision of the Krylov basis, the Z basis and
+// the A block's multigrid:
+//   double: plain FP64; a cycle runs until the tolerance is met or the
+//           restart length is reached;
+//   float:  FP64 iterative refinement around an inner FGMRES whose Krylov
+//           vectors hold the A components in FP32 (V in FP64) and whose Z basis
+//           and A-block multigrid are FP32: each cycle solves for the
+//           correction of the current FP64 residual until that residual is
+//           reduced by the factor inner_tol (or the tolerance or the restart
+//           length is reached); x accumulates in FP64.
+// Iterations are queued ahead in chunks (the first chunk three quarters as
+// long as the last cycle), and the host reads the device's convergence record
 // only between chunks.
 template <typename T>
 int fgmres(qms_handle *h, const OpParams &op, int tr) {
     PrecondState<T> &ps = pstate<T>(h);
+    KBasis<T> &kb = kbasis<T>(h);
     const int64_t n = h->n;
     const int m = h->cfg.restart;
     const double *D = tr ? h->Dtr : h->Dst;
@@ -425,6 +493,8 @@ int fgmres(qms_handle *h, const OpParams &op, int tr) {
     launch_resid(h->g, op_coeffs(h), op, h->w, h->b, h->r, s);
     const double bnorm = norm_dev(h, h->r);
     const double tol = h->cfg.tol;
+    const bool inner = !std::is_same<T, double>::value;
+    const double eta = h->cfg.inner_tol > 0 ? h->cfg.inner_tol : 1e-6;
     int used = 0, queued = 0, restarts = 0;
     double rel = 0.0, rel0 = -1.0;
     double cyc[KS_COUNT];
@@ -444,10 +514,12 @@ int fgmres(qms_handle *h, const OpParams &op, int tr) {
             throw HipError(buf, QMS_ERR_NOCONV);
         }
         ++restarts;
-        launch_scale(h->V[0], h->r, 1.0 / beta, n, s);
-        launch_cycle_start(m, beta, bnorm, tol, h->gd, h->std_, s);
+        launch_scale<T>(kb.V[0], h->r, 1.0 / beta, h->g.n1, kb.offv, s);
+        // the cycle stops at an estimated residual of half its target
+        const double target = inner ? std::max(tol, 2.0 * eta * rel) : tol;
+        launch_cycle_start(m, beta, bnorm, target, h->gd, h->std_, s);
         int j = 0, k = 0;
-        // first chunk: three quarters of the last solve's count (the count
+        // first chunk: three quarters of the last cycle's count (the count
         // varies by a few iterations between steps), then two at a time
         int chunk = h->pred[tr] > 0 ? std::max(2, std::min(3 * h->pred[tr] / 4, m)) : 4;
         while (j < m) {
@@ -592,9 +664,6 @@ int qms_create(const qms_config *cfg, const uint16_t *cell_slot, const double *s
     launch_diag(g, h->c64, make_op(h.get(), v0, 0.0, 0), h->Dst, h->s);
     launch_diag(g, h->c64, make_op(h.get(), v0, cfg->dt, cfg->dt > 0 ? 1 : 0), h->Dtr, h->s);
     const int m = cfg->restart;
-    for (int j = 0; j <= m; ++j) h->V.push_back(h->dalloc<double>(h->n));
-    h->dV = h->dalloc<double *>(m + 1);
-    check(hipMemcpyAsync(h->dV, h->V.data(), sizeof(double *) * (m + 1), hipMemcpyHostToDevice, h->s), "h2d");
     h->nblk = reduce_blocks(h->n);
     h->partial = h->dalloc<double>((int64_t)(m + 2) * h->nblk);
     h->dsmall = h->dalloc<double>(m + 8);
@@ -704,17 +773,19 @@ int64_t qms_estimate_bytes(const qms_config *cfg) {
     int64_t b = 0, allocs = 0;
     auto add = [&](int64_t bytes) { b += bytes; ++allocs; };
     // qms_create: cell slots, material table (at most 65536 slots), coefficients in FP64 and FP32,
-    // class indices and table, state and work vectors, row scalings, step history, Krylov basis
+    // class indices and table, state and work vectors, row scalings, step history
     add(2 * nc); add(48 * 65536);
     for (int a = 0; a < 6; ++a) { add(8 * n1); add(4 * n1); }
     add(2 * n1); add(48 * 65535);
     for (int v = 0; v < 9; ++v) add(8 * n);
-    for (int j = 0; j <= m; ++j) add(8 * n);
-    add(8 * (m + 1));
     add(8 * (m + 2) * std::max<int64_t>(1, std::min<int64_t>(1024, (n + 2047) / 2048)));
     add(8 * (m + 8)); add(8 * (m + 1) * m); add(8 * m); add(8 * m); add(8 * (m + 1)); add(8 * m);
     add(8 * KS_COUNT); add((int64_t)sizeof(OpK));
-    // first solve: Z basis and preconditioner vectors
+    // first solve: Krylov basis and its w (precision ta), Z basis and preconditioner vectors
+    const int64_t kvec = (3 * n1 * ta + 255) / 256 * 256 + 8 * n1;    // A segment in precision ta, V in FP64
+    for (int j = 0; j <= m; ++j) add(kvec);
+    add(8 * (m + 1));
+    if (ta != 8) add(kvec);
     for (int j = 0; j < m; ++j) add(ta * n);
     add(8 * m); add(ta * 3 * n1); add(8 * n1); add(8 * n1);
     // hierarchies: A block steady and transient (precision ta), V block (FP64)
@@ -727,6 +798,7 @@ int64_t qms_estimate_bytes(const qms_config *cfg) {
             if (std::max(x, std::max(y, z)) <= 4 && nlev >= 2) break;
             x = (x + 1) / 2; y = (y + 1) / 2; z = (z + 1) / 2;
             const int64_t l1 = x * y * z;
+            // explicit stencil arrays (an upper bound of the class-compressed form)
             add(s * nc_ * 6 * l1); add(s * nc_ * l1); add(s * nc_ * l1);
             if (nc_ == 3) add(s * nc_ * 8 * l1);
             for (int v = 0; v < 4; ++v) add(s * nc_ * l1);
@@ -871,6 +943,7 @@ int qms_time_kernel(qms_handle *h, int which, const double v[3], int reps, doubl
     const bool mixed = h->cfg.precision == QMS_MIXED;
     const double s = mixed ? 4.0 : 8.0;
     if (which == 1 || which == 2) prepare_solve(h, tr);
+    if (which == 7) ensure_basis<double>(h);
     // The arrays a launch reads (x, b) and writes (w): the handle's own, or
     // one of the copies the cold timing rotates through.
     struct Arrays {
@@ -890,7 +963,7 @@ int qms_time_kernel(qms_handle *h, int which, const double v[3], int reps, doubl
         } else if (which == 6) {
             launch_resid(h->g, op_coeffs(h), op, cur.x, cur.b, cur.w, h->s);
         } else if (which == 7) {
-            gram_schmidt(h, h->cfg.restart + 1);
+            gram_schmidt<double>(h, h->kb64, h->cfg.restart + 1);
         } else {
             const MGParams mp = mgp(h);
             auto sweep = [&](auto &ps) {
@@ -974,10 +1047,11 @@ int qms_gram_schmidt(qms_handle *h, int nv, const double *basis, const double *w
     }
     QMS_TRY(h)
     const int64_t n = h->n;
+    KBasis<double> &kb = ensure_basis<double>(h);
     for (int i = 0; i < nv; ++i)
-        check(hipMemcpyAsync(h->V[i], basis + (size_t)i * n, sizeof(double) * n, hipMemcpyHostToDevice, h->s), "h2d");
-    check(hipMemcpyAsync(h->w, w, sizeof(double) * n, hipMemcpyHostToDevice, h->s), "h2d");
-    gram_schmidt(h, nv);
+        check(hipMemcpyAsync(kb.V[i], basis + (size_t)i * n, sizeof(double) * n, hipMemcpyHostToDevice, h->s), "h2d");
+    check(hipMemcpyAsync(kb.w, w, sizeof(double) * n, hipMemcpyHostToDevice, h->s), "h2d");
+    gram_schmidt<double>(h, kb, nv);
     check(hipMemcpyAsync(h->hsmall, h->dsmall, sizeof(double) * (nv + 1), hipMemcpyDeviceToHost, h->s), "d2h");
     check(hipMemcpyAsync(wout, h->w, sizeof(double) * n, hipMemcpyDeviceToHost, h->s), "d2h");
     check(hipStreamSynchronize(h->s), "sync");

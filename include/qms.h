@@ -34,7 +34,8 @@ extern "C" {
 #define QMS_BUSY 1            /* qms_poll: an asynchronous run is still going */
 
 #define QMS_FP64 0            /* Krylov and multigrid in FP64 */
-#define QMS_MIXED 1           /* FP64 Krylov / operator, FP32 multigrid */
+#define QMS_MIXED 1           /* FP64 refinement around an FP32 inner solve (Krylov basis, operator
+                                 input/output, A-block multigrid); V-block multigrid in FP64 */
 
 #define QMS_FIELD_AX 0
 #define QMS_FIELD_AY 1
@@ -57,6 +58,8 @@ typedef struct {
     double mg_alpha;     /* coarse-correction scaling */
     int use_graphs;      /* 1: each FGMRES iteration replays a captured hipGraph (default);
                             0: its kernels are launched one by one (same results bit for bit) */
+    double inner_tol;    /* QMS_MIXED: each FP32 inner FGMRES cycle stops once it has reduced the
+                            FP64 residual it started from by this factor (<= 0: 1e-6) */
 } qms_config;
 
 typedef struct {
