@@ -273,6 +273,8 @@ void launch_sqrt_to(const double *src, double *dst, hipStream_t s);
 void launch_extrapolate(double *x, double *h1, double *h2, int nh, int64_t n, hipStream_t s);
 void launch_norm2(const double *w, int64_t n, double *partial, int nblk, hipStream_t s);
 int reduce_blocks(int64_t n);
+// Sets the message qms_last_error(NULL) returns (for C-ABI calls without a handle).
+void set_last_error(const char *msg);
 void launch_copy(double *dst, const double *src, int64_t n, hipStream_t s);
 
 }  // namespace qms

@@ -3,7 +3,11 @@
 // for a given vector length: a design gives the same iterates alone or in a
 // batch.
 #include <hip/hip_runtime.h>
+#include <algorithm>
+#include <mutex>
+#include <string>
 #include "qms_kernels.h"
+#include "../../include/qms.h"
 
 namespace qms {
 
@@ -49,10 +53,11 @@ __device__ __forceinline__ void block_sum(double acc[K], double *sh /* [RB/64][K
     }
 }
 
-// Krylov vectors have two segments: the A components (3 n1 values of type
-// TE, from byte 0) and the V component (n1 FP64 values from byte offset
-// offv).  TE = double with offv = 24 n1 is a plain FP64 vector; TE = float
-// is the mixed-precision basis (the V component stays FP64: the conductors'
+// Krylov vectors have two segments: na values of type TE from byte 0 and nb
+// FP64 values from byte offset offv.  In the solver the first segment holds
+// the A components (na = 3 n1) and the second the V component (nb = n1):
+// TE = double with offv = 24 n1 is a plain FP64 vector; TE = float is the
+// mixed-precision basis (the V component stays FP64: the conductors'
 // floating potentials are fixed only through the 1e-3 S/m background, far
 // below FP32 resolution of the conductor rows).  Every sum is FP64.
 //
@@ -83,7 +88,7 @@ __device__ __forceinline__ void st(P base, I off, TE v) {
 // their sums are dropped).
 template <int K, typename I, typename TE>
 __global__ void __launch_bounds__(RB) k_multidot(const TE *const *__restrict__ V, int nv, int c0,
-                                                 const TE *__restrict__ w, int64_t n1, int64_t offv,
+                                                 const TE *__restrict__ w, int64_t na, int64_t nb, int64_t offv,
                                                  double *__restrict__ partial) {
     __shared__ double sh[(RB / 64) * K];
     double acc[K];
@@ -97,16 +102,16 @@ __global__ void __launch_bounds__(RB) k_multidot(const TE *const *__restrict__ V
     const gcp<char> wg = (gcp<char>)w;
     const I t0 = (I)blockIdx.x * RB + threadIdx.x, nt = (I)gridDim.x * RB;
     {   // A components
-        const I nb = (I)(3 * n1) * (I)sizeof(TE), step = nt * (I)sizeof(TE);
-        for (I off = t0 * (I)sizeof(TE); off < nb; off += step) {
+        const I end = (I)na * (I)sizeof(TE), step = nt * (I)sizeof(TE);
+        for (I off = t0 * (I)sizeof(TE); off < end; off += step) {
             const double we = (double)ld<TE>(wg, off);
 #pragma unroll
             for (int c = 0; c < K; ++c) acc[c] = fma((double)ld<TE>(vp[c], off), we, acc[c]);
         }
     }
     {   // V component
-        const I nb = (I)offv + (I)n1 * (I)sizeof(double), step = nt * (I)sizeof(double);
-        for (I off = (I)offv + t0 * (I)sizeof(double); off < nb; off += step) {
+        const I end = (I)offv + (I)nb * (I)sizeof(double), step = nt * (I)sizeof(double);
+        for (I off = (I)offv + t0 * (I)sizeof(double); off < end; off += step) {
             const double we = ld<double>(wg, off);
 #pragma unroll
             for (int c = 0; c < K; ++c) acc[c] = fma(ld<double>(vp[c], off), we, acc[c]);
@@ -138,8 +143,8 @@ __global__ void k_reduce_partials(const double *__restrict__ partial, int nblk, 
 // chunking.  The update is formed in FP64 and stored in each segment's type.
 template <int K, typename I, typename TE>
 __global__ void __launch_bounds__(RB) k_orth_update(TE *const *__restrict__ V, int nv, int c0,
-                                                    const double *__restrict__ h, TE *__restrict__ w, int64_t n1,
-                                                    int64_t offv, double *__restrict__ partial) {
+                                                    const double *__restrict__ h, TE *__restrict__ w, int64_t na,
+                                                    int64_t nb, int64_t offv, double *__restrict__ partial) {
     __shared__ double sh[RB / 64];
     gcp<char> vp[K];
     double hc[K];
@@ -154,8 +159,8 @@ __global__ void __launch_bounds__(RB) k_orth_update(TE *const *__restrict__ V, i
     const gp<char> wg = (gp<char>)w;
     const I t0 = (I)blockIdx.x * RB + threadIdx.x, nt = (I)gridDim.x * RB;
     {
-        const I nb = (I)(3 * n1) * (I)sizeof(TE), step = nt * (I)sizeof(TE);
-        for (I off = t0 * (I)sizeof(TE); off < nb; off += step) {
+        const I end = (I)na * (I)sizeof(TE), step = nt * (I)sizeof(TE);
+        for (I off = t0 * (I)sizeof(TE); off < end; off += step) {
             double ve[K];
 #pragma unroll
             for (int c = 0; c < K; ++c) ve[c] = (double)ld<TE>(vp[c], off);
@@ -168,8 +173,8 @@ __global__ void __launch_bounds__(RB) k_orth_update(TE *const *__restrict__ V, i
         }
     }
     {
-        const I nb = (I)offv + (I)n1 * (I)sizeof(double), step = nt * (I)sizeof(double);
-        for (I off = (I)offv + t0 * (I)sizeof(double); off < nb; off += step) {
+        const I end = (I)offv + (I)nb * (I)sizeof(double), step = nt * (I)sizeof(double);
+        for (I off = (I)offv + t0 * (I)sizeof(double); off < end; off += step) {
             double ve[K];
 #pragma unroll
             for (int c = 0; c < K; ++c) ve[c] = ld<double>(vp[c], off);
@@ -180,10 +185,10 @@ __global__ void __launch_bounds__(RB) k_orth_update(TE *const *__restrict__ V, i
             acc[0] += we * we;
         }
     }
-    if (last) {
-        block_sum<1>(acc, sh);
-        if (threadIdx.x == 0) partial[blockIdx.x] = acc[0];
-    }
+    // every launch reduces (one barrier whatever the chunk, so the kernel's
+    // synchronisation does not depend on its arguments); the last one writes
+    block_sum<1>(acc, sh);
+    if (last && threadIdx.x == 0) partial[blockIdx.x] = acc[0];
 }
 
 __global__ void __launch_bounds__(RB) k_norm2(const double *__restrict__ w, int64_t n, double *__restrict__ partial) {
@@ -326,21 +331,45 @@ void launch_copy(double *dst, const double *src, int64_t n, hipStream_t s) {
 }
 
 
+// Splits the element range of a Gram-Schmidt pass into na values of type TE
+// and nb FP64 values from byte offv; uint32_t byte offsets when they fit.
 template <typename TE>
-void launch_multidot(const TE *const *V, int nv, const TE *w, int64_t n1, int64_t offv, double *partial, int nblk,
-                     hipStream_t s) {
+void launch_multidot_seg(const TE *const *V, int nv, const TE *w, int64_t na, int64_t nb, int64_t offv,
+                         double *partial, int nblk, hipStream_t s) {
     // chunks of up to 16 vectors; the narrower kernel when the chunk fits it
-    const bool o32 = offv + 8 * n1 + (int64_t)nblk * RB * 8 < ((int64_t)1 << 32);
+    const bool o32 = std::max<int64_t>(na * (int64_t)sizeof(TE), offv + 8 * nb) + (int64_t)nblk * RB * 8 <
+                     ((int64_t)1 << 32);
     for (int c0 = 0; c0 < nv; c0 += 16) {
         const int cn = nv - c0 < 16 ? nv - c0 : 16;
         if (o32) {
-            if (cn <= 8) hipLaunchKernelGGL((k_multidot<8, uint32_t, TE>), dim3(nblk), dim3(RB), 0, s, V, nv, c0, w, n1, offv, partial);
-            else hipLaunchKernelGGL((k_multidot<16, uint32_t, TE>), dim3(nblk), dim3(RB), 0, s, V, nv, c0, w, n1, offv, partial);
+            if (cn <= 8) hipLaunchKernelGGL((k_multidot<8, uint32_t, TE>), dim3(nblk), dim3(RB), 0, s, V, nv, c0, w, na, nb, offv, partial);
+            else hipLaunchKernelGGL((k_multidot<16, uint32_t, TE>), dim3(nblk), dim3(RB), 0, s, V, nv, c0, w, na, nb, offv, partial);
         } else {
-            if (cn <= 8) hipLaunchKernelGGL((k_multidot<8, int64_t, TE>), dim3(nblk), dim3(RB), 0, s, V, nv, c0, w, n1, offv, partial);
-            else hipLaunchKernelGGL((k_multidot<16, int64_t, TE>), dim3(nblk), dim3(RB), 0, s, V, nv, c0, w, n1, offv, partial);
+            if (cn <= 8) hipLaunchKernelGGL((k_multidot<8, int64_t, TE>), dim3(nblk), dim3(RB), 0, s, V, nv, c0, w, na, nb, offv, partial);
+            else hipLaunchKernelGGL((k_multidot<16, int64_t, TE>), dim3(nblk), dim3(RB), 0, s, V, nv, c0, w, na, nb, offv, partial);
         }
     }
+}
+template <typename TE>
+void launch_orth_update_seg(TE *const *V, int nv, const double *h, TE *w, int64_t na, int64_t nb, int64_t offv,
+                            double *partial, int nblk, hipStream_t s) {
+    const bool o32 = std::max<int64_t>(na * (int64_t)sizeof(TE), offv + 8 * nb) + (int64_t)nblk * RB * 8 <
+                     ((int64_t)1 << 32);
+    for (int c0 = 0; c0 < nv; c0 += 16) {
+        const int cn = nv - c0 < 16 ? nv - c0 : 16;
+        if (o32) {
+            if (cn <= 8) hipLaunchKernelGGL((k_orth_update<8, uint32_t, TE>), dim3(nblk), dim3(RB), 0, s, V, nv, c0, h, w, na, nb, offv, partial);
+            else hipLaunchKernelGGL((k_orth_update<16, uint32_t, TE>), dim3(nblk), dim3(RB), 0, s, V, nv, c0, h, w, na, nb, offv, partial);
+        } else {
+            if (cn <= 8) hipLaunchKernelGGL((k_orth_update<8, int64_t, TE>), dim3(nblk), dim3(RB), 0, s, V, nv, c0, h, w, na, nb, offv, partial);
+            else hipLaunchKernelGGL((k_orth_update<16, int64_t, TE>), dim3(nblk), dim3(RB), 0, s, V, nv, c0, h, w, na, nb, offv, partial);
+        }
+    }
+}
+template <typename TE>
+void launch_multidot(const TE *const *V, int nv, const TE *w, int64_t n1, int64_t offv, double *partial, int nblk,
+                     hipStream_t s) {
+    launch_multidot_seg<TE>(V, nv, w, 3 * n1, n1, offv, partial, nblk, s);
 }
 void launch_reduce_partials(const double *partial, int nv, int nblk, double *out, hipStream_t s) {
     hipLaunchKernelGGL(k_reduce_partials, dim3(nv), dim3(RB), 0, s, partial, nblk, out);
@@ -348,17 +377,7 @@ void launch_reduce_partials(const double *partial, int nv, int nblk, double *out
 template <typename TE>
 void launch_orth_update(TE *const *V, int nv, const double *h, TE *w, int64_t n1, int64_t offv, double *partial,
                         int nblk, hipStream_t s) {
-    const bool o32 = offv + 8 * n1 + (int64_t)nblk * RB * 8 < ((int64_t)1 << 32);
-    for (int c0 = 0; c0 < nv; c0 += 16) {
-        const int cn = nv - c0 < 16 ? nv - c0 : 16;
-        if (o32) {
-            if (cn <= 8) hipLaunchKernelGGL((k_orth_update<8, uint32_t, TE>), dim3(nblk), dim3(RB), 0, s, V, nv, c0, h, w, n1, offv, partial);
-            else hipLaunchKernelGGL((k_orth_update<16, uint32_t, TE>), dim3(nblk), dim3(RB), 0, s, V, nv, c0, h, w, n1, offv, partial);
-        } else {
-            if (cn <= 8) hipLaunchKernelGGL((k_orth_update<8, int64_t, TE>), dim3(nblk), dim3(RB), 0, s, V, nv, c0, h, w, n1, offv, partial);
-            else hipLaunchKernelGGL((k_orth_update<16, int64_t, TE>), dim3(nblk), dim3(RB), 0, s, V, nv, c0, h, w, n1, offv, partial);
-        }
-    }
+    launch_orth_update_seg<TE>(V, nv, h, w, 3 * n1, n1, offv, partial, nblk, s);
 }
 template void launch_multidot<double>(const double *const *, int, const double *, int64_t, int64_t, double *, int,
                                       hipStream_t);
@@ -408,3 +427,86 @@ void launch_sqrt_to(const double *src, double *dst, hipStream_t s) {
 }
 
 }  // namespace qms
+
+// ---------------------------------------------------------------------------
+// Allocation-free Gram-Schmidt step, qms_gram_schmidt_noalloc (declared in
+// include/qms.h).  Its kernels are qms_gram_schmidt's; their scratch -- the
+// device table of basis pointers and the per-block partial sums -- lives in
+// `scratch`, a fixed array in device memory: every GPU the library runs on
+// holds its own copy until the process ends.  Host code never passes the
+// array's name to a kernel: the host-side symbol is not a device address.
+// The device address is looked up once per device (hipGetSymbolAddress, with
+// that device current) and kept.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[QMS_NOALLOC_SCRATCH_BYTES];
+
+namespace qms {
+
+// scratch layout: [basis pointer table, 1 KiB][partial sums: (nv+1) x nblk doubles]
+constexpr size_t NOALLOC_TAB_BYTES = 1024;
+static_assert(QMS_NOALLOC_MAX_VECTORS * sizeof(void *) <= NOALLOC_TAB_BYTES, "pointer table");
+static_assert(NOALLOC_TAB_BYTES + (size_t)(QMS_NOALLOC_MAX_VECTORS + 1) * 1024 * sizeof(double) <=
+                  QMS_NOALLOC_SCRATCH_BYTES,
+              "partial sums of the largest call (reduce_blocks is at most 1024)");
+
+static std::mutex g_scratch_mu;
+static void *g_scratch_dev[256];       // device address of `scratch` per device id (null: not looked up yet)
+
+static hipError_t scratch_address(int dev, void **out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_dev[dev] = p;
+    }
+    *out = g_scratch_dev[dev];
+    return hipSuccess;
+}
+
+// tab[i] = base + i*n for the nv basis vectors (one workgroup)
+__global__ void k_basis_table(const double **__restrict__ tab, const double *base, int64_t n, int nv) {
+    const int i = (int)threadIdx.x;
+    if (i < nv) tab[i] = base + (int64_t)i * n;
+}
+
+}  // namespace qms
+
+extern "C" int qms_gram_schmidt_noalloc(int64_t n, int nv, const double *basis_dev, double *w_dev, double *hcoef_dev,
+                                        void *stream) {
+    using namespace qms;
+    if (n < 1 || nv < 1 || nv > QMS_NOALLOC_MAX_VECTORS || !basis_dev || !w_dev || !hcoef_dev) {
+        set_last_error("qms_gram_schmidt_noalloc: need n >= 1, 1 <= nv <= QMS_NOALLOC_MAX_VECTORS and device buffers");
+        return QMS_ERR_ARG;
+    }
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess || dev < 0 || dev >= 256) {
+        set_last_error("qms_gram_schmidt_noalloc: no current device");
+        return QMS_ERR_HIP;
+    }
+    void *base = nullptr;
+    e = scratch_address(dev, &base);
+    if (e != hipSuccess) {
+        set_last_error((std::string("qms_gram_schmidt_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e)).c_str());
+        return QMS_ERR_HIP;
+    }
+    const double **tab = (const double **)base;
+    double *partial = (double *)((char *)base + NOALLOC_TAB_BYTES);
+    const int nblk = reduce_blocks(n);
+    hipStream_t s = (hipStream_t)stream;
+    // the same element split as the solver's FP64 basis when n is a multiple
+    // of 4 (so the sums are those of qms_gram_schmidt), else one segment
+    const int64_t na = n % 4 == 0 ? 3 * (n / 4) : n, nb = n - na, offv = 8 * na;
+    hipLaunchKernelGGL(k_basis_table, dim3(1), dim3(64), 0, s, tab, basis_dev, n, nv);
+    launch_multidot_seg<double>(tab, nv, w_dev, na, nb, offv, partial, nblk, s);
+    launch_reduce_partials(partial, nv, nblk, hcoef_dev, s);
+    launch_orth_update_seg<double>((double *const *)tab, nv, hcoef_dev, w_dev, na, nb, offv, partial, nblk, s);
+    launch_reduce_partials(partial, 1, nblk, hcoef_dev + nv, s);
+    e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_last_error((std::string("qms_gram_schmidt_noalloc: launch: ") + hipGetErrorString(e)).c_str());
+        return QMS_ERR_HIP;
+    }
+    return QMS_OK;
+}

@@ -67,6 +67,9 @@ double time_on(hipStream_t s, int reps, const std::function<void()> &f) {
 
 }  // namespace
 
+// The message qms_last_error(NULL) returns, for the C-ABI calls of other files.
+void qms::set_last_error(const char *msg) { g_last_error = msg; }
+
 // Krylov basis: V_0..V_m and the vector w being orthogonalised (the
 // operator's output).  Each vector holds the A components as 3 n1 values of
 // type TE from byte 0 and the V component as n1 FP64 values from byte offset

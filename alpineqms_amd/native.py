@@ -31,7 +31,7 @@ EXPORTS = (
     "qms_get_stats", "qms_set_zchunk", "qms_set_colwidth", "qms_use_coef_classes", "qms_time_kernel", "qms_gram_schmidt", "qms_get_work", "qms_destroy", "qms_last_error",
     "qms_run_async", "qms_poll", "qms_wait", "qms_estimate_bytes",
     "qms_batch_create", "qms_batch_set_blocks", "qms_batch_apply", "qms_batch_tune", "qms_batch_time",
-    "qms_batch_plan", "qms_batch_last_error", "qms_batch_destroy",
+    "qms_batch_plan", "qms_batch_last_error", "qms_batch_destroy", "qms_gram_schmidt_noalloc",
 )
 
 # qms_time_kernel selectors
@@ -123,6 +123,8 @@ def load(path: Optional[str] = None):
     L.qms_get_work.argtypes = [H, d]
     L.qms_gram_schmidt.argtypes = [H, ctypes.c_int, d, d, d, d]
     L.qms_residual.argtypes = [H, d, ctypes.c_double, d, d, d]
+    L.qms_gram_schmidt_noalloc.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                           ctypes.c_void_p, ctypes.c_void_p]
     L.qms_batch_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
     L.qms_batch_set_blocks.argtypes = [H, ctypes.c_int]
     L.qms_batch_apply.argtypes = [H, d, ctypes.c_int]
@@ -185,6 +187,22 @@ def make_config(cells: Sequence[int], h: Sequence[float], dt: float, bc: Sequenc
     cfg.use_graphs = 1 if use_graphs else 0
     cfg.inner_tol = float(inner_tol)
     return cfg
+
+
+# qms_gram_schmidt_noalloc limits (include/qms.h)
+NOALLOC_MAX_VECTORS = 64
+NOALLOC_SCRATCH_BYTES = 1 << 20
+
+
+def gram_schmidt_noalloc(n: int, nv: int, basis_dev: int, w_dev: int, hcoef_dev: int, stream: int = 0) -> None:
+    """qms_gram_schmidt_noalloc on device addresses the caller owns (queued on
+    `stream`, a hipStream_t as an integer; 0 is the null stream).  Allocates
+    nothing; raises QmsError on a refused argument or launch."""
+    L = load()
+    rc = L.qms_gram_schmidt_noalloc(int(n), int(nv), ctypes.c_void_p(basis_dev), ctypes.c_void_p(w_dev),
+                                    ctypes.c_void_p(hcoef_dev), ctypes.c_void_p(stream))
+    if rc != QMS_OK:
+        raise QmsError(rc, L.qms_last_error(None).decode())
 
 
 def estimate_bytes(cells: Sequence[int], precision: int = QMS_FP64, restart: int = 30) -> int:

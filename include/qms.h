@@ -211,6 +211,25 @@ int qms_time_kernel(qms_handle *h, int which, const double v[3], int reps, doubl
  * Writes h[0..nv] and the updated w.  Exposed for parity tests. */
 int qms_gram_schmidt(qms_handle *h, int nv, const double *basis, const double *w, double *hcoef, double *wout);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the step of qms_gram_schmidt on buffers the caller
+ * already holds on the current device, with no allocation of any kind.
+ * basis_dev: nv vectors of n doubles, vector i at basis_dev + i*n;
+ * w_dev: n doubles, orthogonalised in place; hcoef_dev: nv + 1 doubles out
+ * (h_i = V_i . w, then |w|^2 after the update).  Queued on `stream` (a
+ * hipStream_t; NULL: the null stream); returns once queued.  The kernels'
+ * scratch (basis pointer table, partial sums) is a fixed static device
+ * buffer of QMS_NOALLOC_SCRATCH_BYTES per GPU, kept until the process ends;
+ * every call on one GPU uses that GPU's buffer, so calls that overlap on one
+ * GPU (other streams, other threads) race: order them.  QMS_ERR_ARG for
+ * n < 1, nv outside [1, QMS_NOALLOC_MAX_VECTORS] or a null buffer.  For
+ * n = 4 n1 the results are bit for bit those of qms_gram_schmidt. */
+#define QMS_NOALLOC_MAX_VECTORS 64
+#define QMS_NOALLOC_SCRATCH_BYTES (1u << 20)
+int qms_gram_schmidt_noalloc(int64_t n, int nv, const double *basis_dev, double *w_dev, double *hcoef_dev,
+                             void *stream);
+
 /* Copy the handle's work vector (4 node-sized arrays) to the host: the last
  * operator output D^-1 M x written by qms_apply, qms_time_kernel or
  * qms_batch_apply, where D = diag(M) and fixed rows are identity rows. */
