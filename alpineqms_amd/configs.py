@@ -121,7 +121,12 @@ def config3(N: int = 128, steps: int = 200, seed: int = 1003, ramp: bool = True,
     return Design(f"config3_N{N}" + ("_ramp" if ramp else ""), inp, geom)
 
 
-def config4_designs(count: int = 64, steps: int = 50, sizes=None) -> List[Design]:
+def config4_designs(count: int = 64, steps: int = 50, sizes=None, ramp_v: float = 0.0) -> List[Design]:
+    """Config 4's designs.  ramp_v != 0: the dynamic variant, a frame velocity
+    ramped from 0 to ramp_v m/s along x over the run (the reference's
+    trajectory grammar, QMS_3D.f90:134-151, :2331-2350), so that every step
+    carries a solve; 0 is the literal config (every step after the steady
+    state reproduces it)."""
     sizes = sizes or list(range(24, 97, 8))
     out = []
     for d in range(count):
@@ -131,8 +136,8 @@ def config4_designs(count: int = 64, steps: int = 50, sizes=None) -> List[Design
         nbox = int(rng.integers(4, 13))
         geom, mats = _random_boxes(rng, cells, nbox)
         B0 = float(rng.uniform(0.01, 0.2)) * _unit(rng)
-        inp = _inputs(cells, 5e-4, 5e-6, steps, B0, mats=mats)
-        out.append(Design(f"config4_d{d}", inp, geom))
+        inp = _inputs(cells, 5e-4, 5e-6, steps, B0, v_axis_x=ramp_v, ramp=ramp_v != 0.0, mats=mats)
+        out.append(Design(f"config4_d{d}" + (f"_ramp{ramp_v:g}" if ramp_v else ""), inp, geom))
     return out
 
 
@@ -143,7 +148,10 @@ class Request:
 
 
 def config5_stream(count: int, rate: float, seed: int = 5000, nmin: int = 32, nmax: int = 160,
-                   smin: int = 20, smax: int = 200) -> List[Request]:
+                   smin: int = 20, smax: int = 200, ramp_v: float = 0.0) -> List[Request]:
+    """Config 5's request stream (Poisson arrivals at `rate` per second).
+    ramp_v != 0: the dynamic variant, each request's frame velocity ramped
+    from 0 to ramp_v m/s along x over its steps (as config4_designs)."""
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
@@ -155,8 +163,8 @@ def config5_stream(count: int, rate: float, seed: int = 5000, nmin: int = 32, nm
         c = N - 1
         geom, mats = _random_boxes(rng, (c, c, c), int(rng.integers(4, 13)))
         B0 = 0.05 * _unit(rng)
-        inp = _inputs((c, c, c), 5e-4, 5e-6, steps, B0, mats=mats)
-        out.append(Request(t, Design(f"config5_r{r}_N{N}", inp, geom)))
+        inp = _inputs((c, c, c), 5e-4, 5e-6, steps, B0, v_axis_x=ramp_v, ramp=ramp_v != 0.0, mats=mats)
+        out.append(Request(t, Design(f"config5_r{r}_N{N}" + (f"_ramp{ramp_v:g}" if ramp_v else ""), inp, geom)))
     return out
 
 

@@ -498,8 +498,9 @@ int fgmres(qms_handle *h, const OpParams &op, int tr) {
     const double tol = h->cfg.tol;
     const bool inner = !std::is_same<T, double>::value;
     const double eta = h->cfg.inner_tol > 0 ? h->cfg.inner_tol : 1e-6;
-    int used = 0, queued = 0, restarts = 0;
+    int used = 0, queued = 0, restarts = 0, stall = 0;
     double rel = 0.0, rel0 = -1.0;
+    bool cycle_stalled = false;
     double cyc[KS_COUNT];
     while (true) {
         launch_resid(h->g, op_coeffs(h), op, h->x, h->b, h->r, s);
@@ -508,6 +509,22 @@ int fgmres(qms_handle *h, const OpParams &op, int tr) {
         if (rel0 < 0) rel0 = rel;
         if (!(rel == rel)) throw HipError("residual is NaN", QMS_ERR_NOCONV);
         if (rel <= tol || beta == 0.0) break;
+        // Stagnation: a cycle whose own residual estimate ended above half the
+        // residual it started from made no progress (restarted FGMRES can
+        // stagnate on slowly converging modes, such as the conductors'
+        // floating potentials near the tolerance).  Within 10 x tol the solve
+        // is accepted there (and counted); two such cycles in a row above it
+        // fail at once instead of running to the iteration cap.
+        if (cycle_stalled && rel <= 10.0 * tol) { h->st.stalled_solves += 1; break; }
+        stall = cycle_stalled ? stall + 1 : 0;
+        if (stall >= 2) {
+            h->st.last_iters = used; h->st.last_restarts = restarts; h->st.last_relres = rel;
+            h->st.last_relres0 = rel0;
+            char buf[160];
+            snprintf(buf, sizeof buf, "FGMRES stalled after %d iterations at relative residual %.3e (tol %.1e)",
+                     used, rel, tol);
+            throw HipError(buf, QMS_ERR_NOCONV);
+        }
         if (used >= h->cfg.max_iters) {
             h->st.last_iters = used; h->st.last_restarts = restarts; h->st.last_relres = rel;
             h->st.last_relres0 = rel0;
@@ -536,6 +553,7 @@ int fgmres(qms_handle *h, const OpParams &op, int tr) {
             chunk = 2;
         }
         if (k == 0) k = m;
+        cycle_stalled = cyc[KS_EST] > 0.5 * rel;
         used += k;
         launch_backsub(k, m, h->Hd, h->gd, h->yd, s);
         launch_axpy_list(h->x, (const T *const *)ps.dZ, k, h->yd, n, s);

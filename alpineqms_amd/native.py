@@ -83,6 +83,7 @@ class Stats(ctypes.Structure):
         ("queued_iters", ctypes.c_int64),
         ("mg_classes_a", ctypes.c_int),
         ("mg_classes_v", ctypes.c_int),
+        ("stalled_solves", ctypes.c_int64),
     ]
 
 

@@ -16,22 +16,27 @@ multigrid preconditioner, every iteration a replayed hipGraph.  Fields stay on t
 
 Timed: `--warmup` untimed steps after the steady state, then `--steps` steps,
 each a blocking call; `ms_per_step` is the mean of the timed steps.  The
-mixed-precision path (FP32 A-block multigrid and Z basis, FP64 V block,
-FP64 Krylov basis, residuals and fields) is `value`; the plain FP64 path
-runs the same steps beside it and every timed step's A, B and sigma*grad(V)
-are compared with it.
+mixed-precision path (FP64 refinement around an inner FGMRES with the A
+components of its Krylov vectors, its Z basis and its A-block multigrid in
+FP32; V components, V-block multigrid, residuals and fields FP64) is
+`value`; the plain FP64 path runs the same steps beside it and every timed
+step's A, B and sigma*grad(V) are compared with it.
 
 `roofline` is the dominant kernel of the mixed step, the level-0 A-block
 smoother sweep, timed with HIP events on the handle's stream; `kernels` holds
-the other kernels of the step at 128^3 (cold-cache operator figures as
-defined in DESIGN.md).  `cpu_baseline` is the oracle (the C restatement of
+the other kernels of the step at 128^3 (operator figures with cold caches:
+each launch reads copies of its arrays that the other launches' traffic has
+evicted).  `batch` applies config 4's 64 operators in one launch;
+`batch_steps` steps config 4's designs together; `serving` serves a config-5
+request stream and reports p50/p99 latency.  `cpu_baseline` is the oracle (the C restatement of
 the reference's CSR assembly with SciPy's SuperLU in place of PARDISO) on one
 core: config 1's design at 24^3 nodes (its 32^3 factorisation takes over two
 minutes on one core), steady factorisation then steps that reuse it, as the
 reference's phase 33 does for a constant velocity; the GPU runs the same
 design and steps and its errors against the port are reported there.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--nodes N] [--no-cpu-baseline] [--no-batch]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--nodes N] [--paths both|mixed|fp64]
+                       [--no-kernels] [--no-graphs] [--no-cpu-baseline] [--no-batch] [--no-serve]
 """
 import argparse
 import json
@@ -179,7 +184,47 @@ def batch_config4(reps: int, ndesigns: int = 64):
             "frac": gbs / HBM_PEAK_GBS, "speedup_vs_sequential": seq0 / ms_b, "blocks": blocks}
 
 
-def main():
+def serving_config5(count: int, rate: float, max_active: int, ramp_v: float = 0.0, smax: int = 200,
+                    tol: float = TOL):
+    """Config 5 as a served request stream: Poisson arrivals, cubic designs of
+    32-160 nodes per axis, 20-smax steps each, admitted by device footprint
+    under 288 GB with at most `max_active` in flight, mixed precision.
+    ramp_v != 0: the dynamic variant (frame velocity ramped 0 -> ramp_v m/s
+    over each request's steps, so every step carries a solve)."""
+    from alpineqms_amd import configs, native, serve
+    reqs = configs.config5_stream(count, rate, smax=smax, ramp_v=ramp_v)
+    res = serve.serve(reqs, precision=native.QMS_MIXED, max_active=max_active, tol=tol)
+    ok = [r for r in res.records if r.error is None]
+    vdesc = (f"frame velocity ramped 0 -> {ramp_v:g} m/s along x over each request's steps (the dynamic variant)"
+             if ramp_v else "frame velocity 0 (the literal config: each step after the steady state reproduces it)")
+    return {"workload": f"config 5: {count} requests, Poisson arrivals at {rate:g}/s (rng 5000), N^3 with N "
+                        f"log-uniform over multiples of 8 in [32, 160], 20-{smax} steps of 5 us each, {vdesc}, "
+                        f"mixed precision to {tol:g} scaled residual, at most {max_active} in flight, 288 GB budget",
+            "requests": count, "completed": len(ok), "errors": [r.error for r in res.records if r.error],
+            "p50_latency_s": res.percentile(50), "p99_latency_s": res.percentile(99),
+            "mean_wait_s": sum(r.wait for r in ok) / max(1, len(ok)),
+            "dof_steps_per_s": res.dof_steps_per_s, "wall_s": res.wall_s,
+            "peak_active": res.peak_active, "peak_device_bytes": res.peak_bytes,
+            "sizes": [r.name.split("_")[-1] for r in res.records],
+            "iterations": [r.iters for r in res.records]}
+
+
+def batch_steps_config4(ndesigns: int, steps: int, ramp_v: float = 0.0, tol: float = TOL):
+    """Config 4's designs stepped together (each on its own stream from a
+    library thread) against the same steps one design at a time."""
+    from alpineqms_amd import configs, native, serve
+    ds = configs.config4_designs(ndesigns, steps=steps, ramp_v=ramp_v)
+    r = serve.run_batch(ds, native.QMS_MIXED, steps, tol=tol)
+    vdesc = (f"frame velocity ramped 0 -> {ramp_v:g} m/s along x (the dynamic variant)" if ramp_v
+             else "frame velocity 0 (the literal config)")
+    return {"workload": f"config 4: {ndesigns} designs, 24-96 nodes per axis, {sum(d.dof for d in ds)} DoF, "
+                        f"{steps} steps each after the steady state, {vdesc}, mixed precision to {tol:g} scaled "
+                        f"residual",
+            "together_s": r.batched_s, "one_at_a_time_s": r.sequential_s, "speedup": r.speedup,
+            "dof_steps_per_s": r.dof_steps / r.batched_s, "iterations": r.iters}
+
+
+def parser() -> argparse.ArgumentParser:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
@@ -189,9 +234,17 @@ def main():
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch each FGMRES iteration's kernels one by one instead of replaying captured graphs "
                          "(the same kernels and results; for profilers that do not follow graph launches)")
+    ap.add_argument("--paths", choices=("both", "mixed", "fp64"), default="both",
+                    help="which precision paths to step (both: mixed is value, FP64 beside it)")
+    ap.add_argument("--no-kernels", action="store_true", help="skip the per-kernel timings")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
-    a = ap.parse_args()
+    ap.add_argument("--no-serve", action="store_true")
+    return ap
+
+
+def main():
+    a = parser().parse_args()
     if a.gpus != 1:
         print("bench.py runs on one GPU only: every design of the benchmark set fits one MI355X, and the "
               "multi-GPU path (whole designs per GPU) is not built", file=sys.stderr)
@@ -201,10 +254,12 @@ def main():
     inp = d.inputs
     nx, ny, nz = d.nodes
     print(f"bench: {d.name} nodes={d.nodes} dof={d.dof}", file=sys.stderr, flush=True)
+    names = {"both": ("fp64", "mixed"), "mixed": ("mixed",), "fp64": ("fp64",)}[a.paths]
+    precs = {"fp64": native.QMS_FP64, "mixed": native.QMS_MIXED}
     runs = {}
     solvers = {}
-    for name, prec in (("fp64", native.QMS_FP64), ("mixed", native.QMS_MIXED)):
-        s = make_solver(d, prec, tol=a.tol, use_graphs=not a.no_graphs)
+    for name in names:
+        s = make_solver(d, precs[name], tol=a.tol, use_graphs=not a.no_graphs)
         t0 = time.perf_counter()
         s.steady(program.steady_velocity(inp.traj, inp.dt))
         st = s.stats()
@@ -212,8 +267,8 @@ def main():
               file=sys.stderr, flush=True)
         solvers[name] = s
         runs[name] = {"steady_iters": st.last_iters, "step_ms": [], "iters": []}
-    coef = solvers["fp64"].get_coeffs()
-    errors = {"A": 0.0, "B": 0.0, "sigma_grad_V": 0.0}
+    coef = next(iter(solvers.values())).get_coeffs()
+    errors = {"A": 0.0, "B": 0.0, "sigma_grad_V": 0.0} if len(names) == 2 else None
     for n in range(1, a.warmup + a.steps + 1):
         v = program.step_velocity(inp.traj, inp.dt, n)
         for name, s in solvers.items():
@@ -225,76 +280,91 @@ def main():
                 runs[name]["step_ms"].append(t * 1e3)
                 runs[name]["iters"].append(int(s.stats().total_iters - q0))
         if n > a.warmup:
-            f64, fmx = solvers["fp64"].get_fields(), solvers["mixed"].get_fields()
-            B64, Bmx = solvers["fp64"].get_h(unit_mu=True), solvers["mixed"].get_h(unit_mu=True)
-            e = state_errors(d, coef, fmx, f64, Bmx, B64)
-            for k in errors:
-                errors[k] = max(errors[k], e[k])
-            print(f"bench: step {n}: fp64 {runs['fp64']['step_ms'][-1]:.2f} ms ({runs['fp64']['iters'][-1]} it), "
-                  f"mixed {runs['mixed']['step_ms'][-1]:.2f} ms ({runs['mixed']['iters'][-1]} it), "
-                  f"mixed vs fp64: A {e['A']:.1e} B {e['B']:.1e} sgV {e['sigma_grad_V']:.1e}",
-                  file=sys.stderr, flush=True)
+            msg = ", ".join(f"{k} {runs[k]['step_ms'][-1]:.2f} ms ({runs[k]['iters'][-1]} it)" for k in names)
+            if errors is not None:
+                f64, fmx = solvers["fp64"].get_fields(), solvers["mixed"].get_fields()
+                B64, Bmx = solvers["fp64"].get_h(unit_mu=True), solvers["mixed"].get_h(unit_mu=True)
+                e = state_errors(d, coef, fmx, f64, Bmx, B64)
+                for k in errors:
+                    errors[k] = max(errors[k], e[k])
+                msg += f", mixed vs fp64: A {e['A']:.1e} B {e['B']:.1e} sgV {e['sigma_grad_V']:.1e}"
+            print(f"bench: step {n}: {msg}", file=sys.stderr, flush=True)
     for name in runs:
         r = runs[name]
+        r["stalled_solves"] = int(solvers[name].stats().stalled_solves)
         r["ms_per_step"] = sum(r["step_ms"]) / len(r["step_ms"])
         r["steps_per_s"] = 1e3 / r["ms_per_step"]
         r["dof_steps_per_s"] = d.dof * r["steps_per_s"]
         r["mean_iters"] = sum(r["iters"]) / len(r["iters"])
         del r["step_ms"]
-    # kernels of the step (mixed handle for the smoother, FP64 for the operator figures)
-    smx = solvers["mixed"]
-    ms_s, nb_s, gbs_s = smoother_roofline(smx, True, 50)
-    s64 = solvers["fp64"]
-    ms_s64, nb_s64, gbs_s64 = smoother_roofline(s64, False, 50)
-    vlast = program.step_velocity(inp.traj, inp.dt, a.warmup + a.steps)
-    kernels = {"smooth_A_fp64": {"ms_per_launch": ms_s64, "algorithmic_bytes": nb_s64, "achieved_GBs": gbs_s64,
-                                 "frac": gbs_s64 / HBM_PEAK_GBS}}
-    for kname, which in (("operator_apply", native.K_APPLY), ("operator_resid", native.K_RESID),
-                         ("rhs", native.K_RHS), ("hfield", native.K_HFIELD), ("copy", native.K_COPY),
-                         ("smooth_V_fp64", native.K_SMOOTH_V), ("gram_schmidt_31", native.K_GRAM_SCHMIDT)):
-        sel = which | (native.K_COLD if which in (native.K_APPLY, native.K_RESID, native.K_RHS, native.K_HFIELD,
-                                                   native.K_COPY) else 0)
-        s64.time_kernel(sel, 2, v=vlast)
-        ms, nb = s64.time_kernel(sel, 20, v=vlast)
-        gbs = nb / (ms * 1e-3) / 1e9
-        kernels[kname] = {"ms_per_launch": ms, "algorithmic_bytes": nb, "achieved_GBs": gbs,
-                          "frac": gbs / HBM_PEAK_GBS, "cold_caches": bool(sel & native.K_COLD)}
+    # the dominant kernel of the step: the level-0 A-block smoother sweep (FP32 on the mixed path)
+    main_name = "mixed" if "mixed" in solvers else "fp64"
+    ms_s, nb_s, gbs_s = smoother_roofline(solvers[main_name], main_name == "mixed", 50)
+    kernels = None
+    if not a.no_kernels:
+        kernels = {}
+        vlast = program.step_velocity(inp.traj, inp.dt, a.warmup + a.steps)
+        s64 = solvers.get("fp64")
+        if s64 is not None:
+            ms_s64, nb_s64, gbs_s64 = smoother_roofline(s64, False, 50)
+            kernels["smooth_A_fp64"] = {"ms_per_launch": ms_s64, "algorithmic_bytes": nb_s64,
+                                        "achieved_GBs": gbs_s64, "frac": gbs_s64 / HBM_PEAK_GBS}
+            for kname, which in (("operator_apply", native.K_APPLY), ("operator_resid", native.K_RESID),
+                                 ("rhs", native.K_RHS), ("hfield", native.K_HFIELD), ("copy", native.K_COPY),
+                                 ("smooth_V_fp64", native.K_SMOOTH_V), ("gram_schmidt_31", native.K_GRAM_SCHMIDT)):
+                sel = which | (native.K_COLD if which in (native.K_APPLY, native.K_RESID, native.K_RHS,
+                                                           native.K_HFIELD, native.K_COPY) else 0)
+                s64.time_kernel(sel, 2, v=vlast)
+                ms, nb = s64.time_kernel(sel, 20, v=vlast)
+                gbs = nb / (ms * 1e-3) / 1e9
+                kernels[kname] = {"ms_per_launch": ms, "algorithmic_bytes": nb, "achieved_GBs": gbs,
+                                  "frac": gbs / HBM_PEAK_GBS, "cold_caches": bool(sel & native.K_COLD)}
     for s in solvers.values():
         s.close()
-    mx, fp = runs["mixed"], runs["fp64"]
-    traffic_s, traffic_src = traffic_from_profile(f"k_mg_smooth_A_fp32_config3_{a.nodes}")
+    head = runs[main_name]
+    traffic_s, traffic_src = traffic_from_profile(f"k_a0_zm_{'fp32' if main_name == 'mixed' else 'fp64'}_sweep_"
+                                                  f"config3_{a.nodes}")
+    sw = "FP32" if main_name == "mixed" else "FP64"
+    ssz = 4 if main_name == "mixed" else 8
     line = {
         "metric": "A-V time steps/s at 128^3, mixed vs FP64 @1e-8 rel err; served p50/p99 latency",
-        "value": mx["steps_per_s"],
+        "value": head["steps_per_s"],
         "unit": "steps/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": mx["ms_per_step"],
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64",
+        "dtype": "f32+f64" if main_name == "mixed" else "f64",
         "data": "synthetic",
         "config": {"workload": f"config 3: {nx}x{ny}x{nz} nodes, 0.25 mm, 24 boxes + 40x40x8 slab, frame velocity "
                                f"ramp 0 -> 5 m/s over 200 steps of 2 us, steps {a.warmup + 1}-"
-                               f"{a.warmup + a.steps}, FGMRES(30) to {a.tol:g} scaled residual",
+                               f"{a.warmup + a.steps}, {main_name} path, FGMRES(30) to {a.tol:g} scaled residual",
                    "dof": d.dof},
-        "paths": {"mixed": mx, "fp64": fp, "mixed_over_fp64": mx["steps_per_s"] / fp["steps_per_s"]},
-        "dof_steps_per_s": mx["dof_steps_per_s"],
+        "paths": dict(runs, **({"mixed_over_fp64": runs["mixed"]["steps_per_s"] / runs["fp64"]["steps_per_s"]}
+                               if len(names) == 2 else {})),
+        "dof_steps_per_s": head["dof_steps_per_s"],
         "max_rel_error_mixed_vs_fp64": errors,
-        "roofline": {"kernel": "k_mg_smooth (level-0 A-block damped Jacobi sweep, FP32, 3 components per node, "
-                               "rows from the Yee coefficients on the fly)",
+        "roofline": {"kernel": f"k_a0_zm (level-0 A-block damped Jacobi sweep, {sw}, 3 components per node, "
+                               f"LDS z-march, rows from the coefficient class table)",
                      "bound": "hbm", "achieved": gbs_s, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": gbs_s / HBM_PEAK_GBS, "traffic": traffic_s, "traffic_source": traffic_src,
                      "algorithmic_bytes": nb_s,
-                     "algorithmic_bytes_definition": "4*(9*N^3 + 6*N^3): x, b and x_out (3 components each) and "
-                                                     "the six FP32 coefficient arrays",
+                     "algorithmic_bytes_definition": f"{ssz}*(9*N^3 + 6*N^3): x, b and x_out (3 components each) "
+                                                     f"and the six {sw} coefficient arrays",
                      "ms_per_launch": ms_s, "launches": 50,
                      "caches": "back to back on the handle's arrays: the sweeps of a V-cycle reuse the same b and "
                                "coefficients, so this is how they run inside a step"},
         "kernels": kernels,
         "batch": None if a.no_batch else batch_config4(20),
+        # the ramp variants at 1e-10: on these 0.5 mm designs restarted FGMRES(30) can stagnate between
+        # 1e-12 and 1e-11 (the conductors' floating potentials), which the headline's 0.25 mm design does not
+        "batch_steps": None if a.no_batch else {"literal": batch_steps_config4(64, 10),
+                                                "ramp": batch_steps_config4(64, 10, ramp_v=1.0, tol=1e-10)},
+        "serving": None if a.no_serve else {"literal": serving_config5(24, 4.0, 8),
+                                            "ramp": serving_config5(24, 4.0, 8, ramp_v=1.0, smax=60, tol=1e-10)},
         "cpu_baseline": None if a.no_cpu_baseline else cpu_baseline(),
     }
     print(json.dumps(line), flush=True)

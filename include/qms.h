@@ -76,6 +76,9 @@ typedef struct {
                                 queues iterations ahead and reads convergence between chunks) */
     int mg_classes_a, mg_classes_v;   /* row classes of the first coarse multigrid level of the A and
                                          V blocks (0: that level stores explicit stencil arrays) */
+    int64_t stalled_solves;  /* solves accepted on stagnation: a restart cycle did not halve the residual
+                                it started from, which then was within 10 x tol (two stagnant cycles
+                                above 10 x tol fail the solve with QMS_ERR_NOCONV) */
 } qms_stats;
 
 /* Fills defaults: FP64, tol 1e-12, restart 30, 3000 iterations, nu 4,

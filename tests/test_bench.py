@@ -32,3 +32,22 @@ def test_committed_traffic_summaries_parse_and_name_their_command():
         nbytes, src = bench.traffic_from_profile(name)
         assert src == f"profiles/{fn}" and nbytes > 0, fn
         assert "rocprofv3" in open(os.path.join(pdir, fn)).read(), fn
+
+
+def test_committed_profiles_name_commands_bench_accepts():
+    """Every bench.py command line quoted in profiles/ parses with the current
+    bench.py arguments, so each summary can be reproduced from the tree."""
+    import re
+    import shlex
+    pdir = os.path.join(ROOT, "profiles")
+    seen = 0
+    for fn in sorted(os.listdir(pdir)):
+        text = open(os.path.join(pdir, fn)).read().replace("\\\n", " ")
+        for line in text.splitlines():
+            m = re.search(r"bench\.py(.*)$", line)
+            if not m:
+                continue
+            args = [a for a in shlex.split(m.group(1)) if a not in (">", "2>&1")]
+            bench.parser().parse_args(args)
+            seen += 1
+    assert seen > 0
