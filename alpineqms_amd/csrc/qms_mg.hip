@@ -410,10 +410,29 @@ __device__ __forceinline__ int64_t lidx_clamped(const LevelDims &d, int i, int j
     return lidx(d, max(0, min(i, d.nx - 1)), max(0, min(j, d.ny - 1)), max(0, min(k, d.nz - 1)));
 }
 
-template <int C, int NC, int MODE, typename T, typename P>
+// The coarse-grid correction of the next level, applied where a value is
+// read: x + alpha * xc[parent].  It is added whatever the node's state: an
+// inactive node's value only meets zero couplings (a coupling to a node of
+// no active row is zero after the elimination), and an inactive row's own
+// output is 0.  For active nodes the value is the prolongation's own.
+template <typename T>
+struct Prolong {
+    const T *xc;       // null: no correction
+    LevelDims cd;
+    T alpha;
+    // PRO: the kernel variant that applies the correction (compile time, so
+    // the plain sweeps carry no test or registers for it)
+    template <bool PRO>
+    __device__ __forceinline__ T at(const T *__restrict__ x, int64_t q, int c, int i, int j, int k) const {
+        if constexpr (PRO) return x[q] + alpha * xc[(int64_t)c * cd.n1 + lidx(cd, i >> 1, j >> 1, k >> 1)];
+        else return x[q];
+    }
+};
+
+template <int C, int NC, int MODE, bool PRO, typename T, typename P>
 __device__ __forceinline__ void cop_comp(const P &pv, const LevelDims &d, int64_t p, int i, int j, int k,
                                          const T *__restrict__ x, const T *__restrict__ b, T *__restrict__ out,
-                                         T omega) {
+                                         T omega, const Prolong<T> &pr) {
     Row<T> r;
     const bool act = pv.template row_all<C>(i, j, k, r);
     const int64_t q = (int64_t)C * d.n1 + p;
@@ -421,43 +440,45 @@ __device__ __forceinline__ void cop_comp(const P &pv, const LevelDims &d, int64_
     if (MODE == 1) {
         v = omega * b[q] / row_diag(r);
     } else {
-        const T x0 = x[q];
+        auto X = [&](int c, int ni, int nj, int nk) {
+            ni = max(0, min(ni, d.nx - 1)); nj = max(0, min(nj, d.ny - 1)); nk = max(0, min(nk, d.nz - 1));
+            return pr.template at<PRO>(x, (int64_t)c * d.n1 + lidx(d, ni, nj, nk), c, ni, nj, nk);
+        };
+        const T x0 = X(C, i, j, k);
         T y = r.exc * x0;
         T diag = r.exc;
 #pragma unroll
         for (int e = 0; e < 6; ++e) {
             diag += r.cond[e];
-            y += r.cond[e] * (x0 - x[(int64_t)C * d.n1 + lidx_clamped(d, i + so_i(e), j + so_j(e), k + so_k(e))]);
+            y += r.cond[e] * (x0 - X(C, i + so_i(e), j + so_j(e), k + so_k(e)));
         }
         if (NC == 3) {
 #pragma unroll
-            for (int s = 0; s < 8; ++s)
-                y += r.cross[s] * x[(int64_t)xo_c(C, s) * d.n1 +
-                                    lidx_clamped(d, i + xo_i(C, s), j + xo_j(C, s), k + xo_k(C, s))];
+            for (int s = 0; s < 8; ++s) y += r.cross[s] * X(xo_c(C, s), i + xo_i(C, s), j + xo_j(C, s), k + xo_k(C, s));
         }
         v = MODE == 0 ? x0 + omega * (b[q] - y) / diag : b[q] - y;
     }
     out[q] = act ? v : (T)0;
 }
 
-template <typename T, typename P, int NC, int MODE>
+template <typename T, typename P, int NC, int MODE, bool PRO>
 __global__ void __launch_bounds__(256) k_mg_cop(P pv, const T *__restrict__ x, const T *__restrict__ b,
-                                                T *__restrict__ out, T omega) {
+                                                T *__restrict__ out, T omega, Prolong<T> pr) {
     const LevelDims d = pv.dims();
     int i, j, k;
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (!node_of(d, p, i, j, k)) return;
-    cop_comp<0, NC, MODE, T>(pv, d, p, i, j, k, x, b, out, omega);
+    cop_comp<0, NC, MODE, PRO, T>(pv, d, p, i, j, k, x, b, out, omega, pr);
     if (NC == 3) {
-        cop_comp<1, NC, MODE, T>(pv, d, p, i, j, k, x, b, out, omega);
-        cop_comp<2, NC, MODE, T>(pv, d, p, i, j, k, x, b, out, omega);
+        cop_comp<1, NC, MODE, PRO, T>(pv, d, p, i, j, k, x, b, out, omega, pr);
+        cop_comp<2, NC, MODE, PRO, T>(pv, d, p, i, j, k, x, b, out, omega, pr);
     }
 }
 
 // ---------------------------------------------------------------------------
-// Level 0 of the A block on a z-march: a workgroup owns a 32 x 8 column of
-// nodes and walks it through zt.kz planes; the three components of planes
-// k-1, k, k+1 (with a one-node x/y halo) sit in an LDS ring, so every x value
+// Level 0 of either block on a z-march: a workgroup owns a 32 x 8 column of
+// nodes and walks it through zt.kz planes; the components of planes k-1, k,
+// k+1 (with a one-node x/y halo) sit in an LDS ring, so every x value
 // is read from memory about once per column instead of once per stencil use
 // (the per-node kernel issues about 70 loads per node).  Coefficients come
 // from the class table when there is one (2 bytes per node instead of six
@@ -478,11 +499,13 @@ struct A0Z {
 // Coefficient tuples of a node and of its -x, -y and -z neighbours.
 template <typename T>
 struct ZmTuple {
-    T im0[3], imx[3], imy[3], imz[3], sg0[3];
+    T im0[3], imx[3], imy[3], imz[3], sg0[3], sgx[3], sgy[3], sgz[3];
     __device__ __forceinline__ T IM(int a, int di, int dj, int dk) const {
         return dk ? imz[a] : di ? imx[a] : dj ? imy[a] : im0[a];
     }
-    __device__ __forceinline__ T SG(int a, int, int, int) const { return sg0[a]; }
+    __device__ __forceinline__ T SG(int a, int di, int dj, int dk) const {
+        return dk ? sgz[a] : di ? sgx[a] : dj ? sgy[a] : sg0[a];
+    }
 };
 
 template <typename T, bool CLS>
@@ -509,7 +532,7 @@ struct A0View {
     }
 };
 
-template <int C, typename T>
+template <int C, int NC, typename T>
 __device__ __forceinline__ T apply_lds(const A0View<T> &xv, const Row<T> &r, T &diag, T &x0) {
     x0 = xv.get(C, 0, 0, 0);
     T y = r.exc * x0;
@@ -521,31 +544,37 @@ __device__ __forceinline__ T apply_lds(const A0View<T> &xv, const Row<T> &r, T &
         diag += r.cond[q];
         y += r.cond[q] * (x0 - xv.get(C, so_i(q), so_j(q), so_k(q)));
     }
+    if (NC == 3) {
 #pragma unroll
-    for (int s = 0; s < 8; ++s) y += r.cross[s] * xv.get(xo_c(C, s), xo_i(C, s), xo_j(C, s), xo_k(C, s));
+        for (int s = 0; s < 8; ++s) y += r.cross[s] * xv.get(xo_c(C, s), xo_i(C, s), xo_j(C, s), xo_k(C, s));
+    }
     return y;
 }
 
-template <int C, int MODE, typename T>
+// Row of component C of an NC-component block (NC = 3: the A block's Ax, Ay,
+// Az rows; NC = 1: the V block's row) at a node of the z-march.
+template <int C, int NC, int MODE, typename T>
 __device__ __forceinline__ void a0_row(const Grid &g, const FineConst<T> &fk, const ZmTuple<T> &cf,
                                        const A0View<T> &xv, int i, int j, int k, int64_t p, const T *bq,
                                        T *__restrict__ out, T omega) {
     const int64_t q = (int64_t)C * g.n1 + p;
-    if (!free_f(g, C, i, j, k)) { out[q] = (T)0; return; }
+    if (!free_f(g, NC == 1 ? FV : C, i, j, k)) { out[q] = (T)0; return; }
     Row<T> r;
-    row_a_raw<C, T>(cf, fk, r);
-    eliminate<C, 3, T>(g, i, j, k, r);
+    if (NC == 3) row_a_raw<C, T>(cf, fk, r);
+    else row_v_raw<T>(cf, fk, r);
+    eliminate<C, NC, T>(g, i, j, k, r);
     T diag, x0;
-    const T y = apply_lds<C, T>(xv, r, diag, x0);
+    const T y = apply_lds<C, NC, T>(xv, r, diag, x0);
     out[q] = MODE == 0 ? x0 + omega * (bq[C] - y) / diag : bq[C] - y;
 }
 
+// Level 0 of either block (NC = 3: A block, NC = 1: V block) on the z-march.
 // MODE 0: damped Jacobi sweep out = x + w (b - A x)/diag; MODE 1: residual
 // out = b - A x.  Inactive rows get 0.
-template <typename T, int MODE, bool CLS>
+template <typename T, int NC, int MODE, bool CLS, bool PRO>
 __global__ void __launch_bounds__(256) k_a0_zm(A0Z<T> z, const T *__restrict__ x, const T *__restrict__ b,
-                                               T *__restrict__ out, T omega) {
-    __shared__ T lds[3 * 3 * A0_PL];
+                                               T *__restrict__ out, T omega, Prolong<T> pr) {
+    __shared__ T lds[NC * 3 * A0_PL];
     Grid g = z.g;
     g.nx = __builtin_amdgcn_readfirstlane(z.g.nx);
     g.ny = __builtin_amdgcn_readfirstlane(z.g.ny);
@@ -569,7 +598,7 @@ __global__ void __launch_bounds__(256) k_a0_zm(A0Z<T> z, const T *__restrict__ x
             const int gi = max(0, min(i0 - 1 + ei, g.nx - 1)), gj = max(0, min(j0 - 1 + ej, g.ny - 1));
             const int64_t q = gi + (int64_t)g.nx * gj + plane * kp;
 #pragma unroll
-            for (int c = 0; c < 3; ++c) lds[(c * 3 + slot) * A0_PL + e] = x[c * n1 + q];
+            for (int c = 0; c < NC; ++c) lds[(c * 3 + slot) * A0_PL + e] = pr.template at<PRO>(x, c * n1 + q, c, gi, gj, kp);
         }
     };
     stage(kb - 1, 0);
@@ -578,33 +607,29 @@ __global__ void __launch_bounds__(256) k_a0_zm(A0Z<T> z, const T *__restrict__ x
     const int dxm = ic > 0 ? -1 : 0;
     const int64_t dym = jc > 0 ? -(int64_t)g.nx : 0;
     ZmTuple<T> cf;
-    {
-        T sgd[3];
-        load_tuple<T, CLS>(z, pc + plane * max(kb - 1, 0), cf.imz, sgd);
-    }
+    load_tuple<T, CLS>(z, pc + plane * max(kb - 1, 0), cf.imz, cf.sgz);
     for (int kk = kb; kk < ke; ++kk) {
         const int r = kk - kb;
         const int sm = r % 3, s0 = (r + 1) % 3, sp = (r + 2) % 3;
         const int64_t p = pc + plane * kk;
-        T sgd[3];
         load_tuple<T, CLS>(z, p, cf.im0, cf.sg0);
-        load_tuple<T, CLS>(z, p + dxm, cf.imx, sgd);
-        load_tuple<T, CLS>(z, p + dym, cf.imy, sgd);
-        T bq[3] = {(T)0, (T)0, (T)0};
-        if (live) {
+        load_tuple<T, CLS>(z, p + dxm, cf.imx, cf.sgx);
+        load_tuple<T, CLS>(z, p + dym, cf.imy, cf.sgy);
+        T bq[NC];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) bq[c] = b[c * n1 + p];
-        }
+        for (int c = 0; c < NC; ++c) bq[c] = live ? b[c * n1 + p] : (T)0;
         stage(kk + 1, sp);
         __syncthreads();
         if (live) {
             const A0View<T> xv{lds, {sm * A0_PL, s0 * A0_PL, sp * A0_PL}, ctr};
-            a0_row<0, MODE, T>(g, z.fk, cf, xv, i, j, kk, p, bq, out, omega);
-            a0_row<1, MODE, T>(g, z.fk, cf, xv, i, j, kk, p, bq, out, omega);
-            a0_row<2, MODE, T>(g, z.fk, cf, xv, i, j, kk, p, bq, out, omega);
+            a0_row<0, NC, MODE, T>(g, z.fk, cf, xv, i, j, kk, p, bq, out, omega);
+            if constexpr (NC == 3) {
+                a0_row<1, NC, MODE, T>(g, z.fk, cf, xv, i, j, kk, p, bq, out, omega);
+                a0_row<2, NC, MODE, T>(g, z.fk, cf, xv, i, j, kk, p, bq, out, omega);
+            }
         }
 #pragma unroll
-        for (int a = 0; a < 3; ++a) cf.imz[a] = cf.im0[a];
+        for (int a = 0; a < 3; ++a) { cf.imz[a] = cf.im0[a]; cf.sgz[a] = cf.sg0[a]; }
         __syncthreads();
     }
 }
@@ -624,22 +649,6 @@ __global__ void __launch_bounds__(256) k_mg_restrict(LevelDims fd, LevelDims cd,
             if (inside(fd, i, j, k)) s += r[(int64_t)c * fd.n1 + lidx(fd, i, j, k)];
         }
         bc[(int64_t)c * cd.n1 + pc] = s;
-    }
-}
-
-// x += alpha * P xc on the active fine rows.
-template <typename T, typename P, int NC>
-__global__ void __launch_bounds__(256) k_mg_prolong(P pv, LevelDims cd, const T *__restrict__ xc, T *__restrict__ x,
-                                                    T alpha) {
-    const LevelDims d = pv.dims();
-    int i, j, k;
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (!node_of(d, p, i, j, k)) return;
-    const int64_t pc = lidx(cd, i >> 1, j >> 1, k >> 1);
-    if (pv.template active<0>(i, j, k)) x[p] += alpha * xc[pc];
-    if (NC == 3) {
-        if (pv.template active<1>(i, j, k)) x[d.n1 + p] += alpha * xc[cd.n1 + pc];
-        if (pv.template active<2>(i, j, k)) x[2 * d.n1 + p] += alpha * xc[2 * cd.n1 + pc];
     }
 }
 
@@ -932,11 +941,14 @@ void mg_setup(const Grid &g, const HostCoef &hc, const CoeffsT<T> &dc, double dt
     h.ctab = dc.ctab;
     {
         // level-0 z-march: 32 x 8 columns, the deepest z-chunk of 1..16 planes
-        // that leaves at least 768 workgroups (three per CU)
+        // that leaves at least 768 workgroups (three per CU) for the A block
+        // and 1536 for the one-component V block, whose workgroups are
+        // lighter (measured fastest at 128^3: 8 and 4 planes)
         const int64_t cols2d = (int64_t)((g.nx + A0_TX - 1) / A0_TX) * ((g.ny + A0_TY - 1) / A0_TY);
+        const int64_t want = block == 0 ? 768 : 1536;
         int kz = 1;
         for (int k : {2, 4, 8, 16})
-            if (cols2d * ((g.nz + k - 1) / k) >= 768) kz = k;
+            if (cols2d * ((g.nz + k - 1) / k) >= want) kz = k;
         h.zt = ztiling(g, kz, A0_TX, h.zcols);
     }
     // level 0 work vectors
@@ -1112,23 +1124,45 @@ static A0Z<T> a0z(const Hierarchy<T> &h) {
     return z;
 }
 
-// Level 0 of the A block, z-march form (MODE 0: sweep, 1: residual).
+// Level 0 of either block, z-march form (MODE 0: sweep, 1: residual).
 template <typename T, int MODE>
-static void a0_launch(const Hierarchy<T> &h, const T *x, const T *b, T *out, T w, hipStream_t s) {
-    if (h.cls) hipLaunchKernelGGL((k_a0_zm<T, MODE, true>), dim3(h.zcols), dim3(256), 0, s, a0z(h), x, b, out, w);
-    else hipLaunchKernelGGL((k_a0_zm<T, MODE, false>), dim3(h.zcols), dim3(256), 0, s, a0z(h), x, b, out, w);
+static void a0_launch(const Hierarchy<T> &h, const T *x, const T *b, T *out, T w, const Prolong<T> &pr,
+                      hipStream_t s) {
+    const dim3 grid(h.zcols), block(256);
+    auto go = [&](auto nc, auto cls, auto pro) {
+        constexpr int NC = decltype(nc)::value;
+        constexpr bool CLS = decltype(cls)::value, PRO = decltype(pro)::value;
+        hipLaunchKernelGGL((k_a0_zm<T, NC, MODE, CLS, PRO>), grid, block, 0, s, a0z(h), x, b, out, w, pr);
+    };
+    using I3 = std::integral_constant<int, 3>;
+    using I1 = std::integral_constant<int, 1>;
+    using Y = std::true_type;
+    using N = std::false_type;
+    const bool pro = MODE == 0 && pr.xc != nullptr;
+    if (h.block == 0) {
+        if (h.cls) { if (pro) go(I3(), Y(), Y()); else go(I3(), Y(), N()); }
+        else { if (pro) go(I3(), N(), Y()); else go(I3(), N(), N()); }
+    } else {
+        if (h.cls) { if (pro) go(I1(), Y(), Y()); else go(I1(), Y(), N()); }
+        else { if (pro) go(I1(), N(), Y()); else go(I1(), N(), N()); }
+    }
 }
 
+// One smoother sweep of level l: from x = 0 (first), or from x plus the
+// next level's correction xc scaled by alpha (pr.xc non-null: the
+// prolongation fused into the first post-smoothing sweep).
 template <typename T>
-static void smooth(const Hierarchy<T> &h, int l, const T *x, const T *b, T *xo, T w, bool first, hipStream_t s) {
+static void smooth(const Hierarchy<T> &h, int l, const T *x, const T *b, T *xo, T w, bool first, hipStream_t s,
+                   const Prolong<T> &pr = Prolong<T>{nullptr, LevelDims{0, 0, 0, 0}, (T)0}) {
     const int64_t n1 = h.lev[l].d.n1;
-    if (l == 0 && h.block == 0 && !first) { a0_launch<T, 0>(h, x, b, xo, w, s); return; }
+    if (l == 0 && !first) { a0_launch<T, 0>(h, x, b, xo, w, pr, s); return; }
     on_level(h, l, [&](auto pv, auto nc) {
         using P = decltype(pv);
         constexpr int NC = decltype(nc)::value;
         if constexpr (is_coarse_prov<P>::value) {
-            if (first) hipLaunchKernelGGL((k_mg_cop<T, P, NC, 1>), g1(n1), dim3(256), 0, s, pv, x, b, xo, w);
-            else hipLaunchKernelGGL((k_mg_cop<T, P, NC, 0>), g1(n1), dim3(256), 0, s, pv, x, b, xo, w);
+            if (first) hipLaunchKernelGGL((k_mg_cop<T, P, NC, 1, false>), g1(n1), dim3(256), 0, s, pv, x, b, xo, w, pr);
+            else if (pr.xc) hipLaunchKernelGGL((k_mg_cop<T, P, NC, 0, true>), g1(n1), dim3(256), 0, s, pv, x, b, xo, w, pr);
+            else hipLaunchKernelGGL((k_mg_cop<T, P, NC, 0, false>), g1(n1), dim3(256), 0, s, pv, x, b, xo, w, pr);
         } else {
             if (first) hipLaunchKernelGGL((k_mg_smooth<T, P, NC, true>), g1(n1), dim3(256), 0, s, pv, x, b, xo, w);
             else hipLaunchKernelGGL((k_mg_smooth<T, P, NC, false>), g1(n1), dim3(256), 0, s, pv, x, b, xo, w);
@@ -1138,27 +1172,17 @@ static void smooth(const Hierarchy<T> &h, int l, const T *x, const T *b, T *xo, 
 template <typename T>
 static void resid(const Hierarchy<T> &h, int l, const T *x, const T *b, T *r, hipStream_t s) {
     const int64_t n1 = h.lev[l].d.n1;
-    if (l == 0 && h.block == 0) { a0_launch<T, 1>(h, x, b, r, (T)0, s); return; }
+    const Prolong<T> none{nullptr, LevelDims{0, 0, 0, 0}, (T)0};
+    if (l == 0) { a0_launch<T, 1>(h, x, b, r, (T)0, none, s); return; }
     on_level(h, l, [&](auto pv, auto nc) {
         using P = decltype(pv);
         constexpr int NC = decltype(nc)::value;
         if constexpr (is_coarse_prov<P>::value)
-            hipLaunchKernelGGL((k_mg_cop<T, P, NC, 2>), g1(n1), dim3(256), 0, s, pv, x, b, r, (T)0);
+            hipLaunchKernelGGL((k_mg_cop<T, P, NC, 2, false>), g1(n1), dim3(256), 0, s, pv, x, b, r, (T)0, none);
         else
             hipLaunchKernelGGL((k_mg_resid<T, P, NC>), g1(n1), dim3(256), 0, s, pv, x, b, r);
     });
 }
-template <typename T>
-static void prolong(const Hierarchy<T> &h, int l, const T *xc, T *x, T alpha, hipStream_t s) {
-    const int64_t n1 = h.lev[l].d.n1;
-    const LevelDims cd = h.lev[l + 1].d;
-    on_level(h, l, [&](auto pv, auto nc) {
-        using P = decltype(pv);
-        constexpr int NC = decltype(nc)::value;
-        hipLaunchKernelGGL((k_mg_prolong<T, P, NC>), g1(n1), dim3(256), 0, s, pv, cd, xc, x, alpha);
-    });
-}
-
 template <typename T>
 static void vcycle_rec(Hierarchy<T> &h, const MGParams &mp, int l, T *x, const T *b, hipStream_t s) {
     MgLevel<T> &L = h.lev[l];
@@ -1181,11 +1205,12 @@ static void vcycle_rec(Hierarchy<T> &h, const MGParams &mp, int l, T *x, const T
     MgLevel<T> &C = h.lev[l + 1];
     hipLaunchKernelGGL(k_mg_restrict<T>, g1(C.d.n1), dim3(256), 0, s, L.d, C.d, nc, (const T *)L.r, C.b);
     vcycle_rec<T>(h, mp, l + 1, C.x, C.b, s);
-    prolong<T>(h, l, C.x, x, (T)mp.alpha, s);
+    // post-smoothing; the first sweep reads x + alpha * P xc (the prolongation)
     cur = x;
     oth = L.t;
     for (int it = 0; it < mp.nu; ++it) {
-        smooth<T>(h, l, cur, b, oth, w, false, s);
+        if (it == 0) smooth<T>(h, l, cur, b, oth, w, false, s, Prolong<T>{C.x, C.d, (T)mp.alpha});
+        else smooth<T>(h, l, cur, b, oth, w, false, s);
         std::swap(cur, oth);
     }
     if (cur != x) hipLaunchKernelGGL(k_copy_t<T>, g1(nv), dim3(256), 0, s, x, (const T *)cur, nv);

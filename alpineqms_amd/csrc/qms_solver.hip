@@ -954,7 +954,7 @@ int qms_time_kernel(qms_handle *h, int which, const double v[3], int reps, doubl
     if (busy(h)) return QMS_ERR_STATE;
     const bool cold = (which & QMS_TIME_COLD) != 0;
     which &= ~QMS_TIME_COLD;
-    if (which < 0 || which > 7) return QMS_ERR_ARG;
+    if (which < 0 || which > 10) return QMS_ERR_ARG;
     QMS_TRY(h)
     const int tr = h->cfg.dt > 0 ? 1 : 0;
     const double v0[3] = {0, 0, 0};
@@ -963,8 +963,9 @@ int qms_time_kernel(qms_handle *h, int which, const double v[3], int reps, doubl
     const int64_t nc = (int64_t)h->cfg.cells[0] * h->cfg.cells[1] * h->cfg.cells[2];
     const bool mixed = h->cfg.precision == QMS_MIXED;
     const double s = mixed ? 4.0 : 8.0;
-    if (which == 1 || which == 2) prepare_solve(h, tr);
+    if (which == 1 || which == 2 || which >= 8) prepare_solve(h, tr);
     if (which == 7) ensure_basis<double>(h);
+    const OpParams opd = which == 10 ? device_op(h, op) : op;
     // The arrays a launch reads (x, b) and writes (w): the handle's own, or
     // one of the copies the cold timing rotates through.
     struct Arrays {
@@ -985,6 +986,21 @@ int qms_time_kernel(qms_handle *h, int which, const double v[3], int reps, doubl
             launch_resid(h->g, op_coeffs(h), op, cur.x, cur.b, cur.w, h->s);
         } else if (which == 7) {
             gram_schmidt<double>(h, h->kb64, h->cfg.restart + 1);
+        } else if (which >= 8) {
+            // a whole A-block or V-block V-cycle, or one FGMRES iteration
+            // (iteration index min(10, restart - 1): preconditioner, operator,
+            // Gram-Schmidt against up to 11 vectors, Givens, next basis vector)
+            const MGParams mp = mgp(h);
+            const double *D = tr ? h->Dtr : h->Dst;
+            const int j = std::min(10, h->cfg.restart - 1);
+            auto part = [&](auto &ps, auto tag) {
+                using T = decltype(tag);
+                if (which == 8) mg_vcycle<T>(ps.hA[tr], mp, ps.Z[0], ps.rT, h->s);
+                else if (which == 9) mg_vcycle<double>(ps.hV, mp, ps.zV, ps.rV, h->s);
+                else fgmres_iteration<T>(h, opd, tr, D, j);
+            };
+            if (mixed) part(h->p32, 0.0f);
+            else part(h->p64, 0.0);
         } else {
             const MGParams mp = mgp(h);
             auto sweep = [&](auto &ps) {
@@ -1053,6 +1069,7 @@ int qms_time_kernel(qms_handle *h, int which, const double v[3], int reps, doubl
         break;
     }
     case 1: *bytes_per_launch = s * (3.0 * 3.0 * N1 + 6.0 * N1); break;
+    case 8: case 9: case 10: *bytes_per_launch = 0.0; break;             // composite: no single stream
     default: *bytes_per_launch = s * (3.0 * N1 + 3.0 * N1); break;
     }
     return QMS_OK;

@@ -187,7 +187,10 @@ int qms_set_colwidth(qms_handle *h, int nodes);
  * 3 plain copy of the 4-field state (calibration of byte counters),
  * 4 step right-hand side, 5 cell-centred H, 6 FP64 residual b - M x,
  * 7 Gram-Schmidt step of FGMRES against restart + 1 basis vectors (the
- * largest of a restart cycle: multi-dot, reduction, update, norm).
+ * largest of a restart cycle: multi-dot, reduction, update, norm),
+ * 8 one A-block V-cycle, 9 one V-block V-cycle, 10 one FGMRES iteration of
+ * the handle's precision (preconditioner, operator, Gram-Schmidt against 11
+ * basis vectors, Givens update); 8-10 are composite and report 0 bytes.
  * which | QMS_TIME_COLD: the same kernel with cold caches, as inside a solve
  * where the Krylov basis streams through the caches between two launches:
  * the launches rotate through copies of the kernel's arrays, enough that at

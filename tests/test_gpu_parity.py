@@ -182,7 +182,13 @@ def test_time_kernel_selectors():
             ms_c, nbytes_c = s.time_kernel(which | native.K_COLD, 2, v=[1.0, 0.0, 0.0])
             assert ms_c > 0 and nbytes_c == nbytes, which
         with pytest.raises(native.QmsError):
-            s.time_kernel(8, 1)
+            s.time_kernel(11, 1)
+    # composite selectors (whole V-cycles, one FGMRES iteration): timed, no byte count
+    for prec in (native.QMS_FP64, native.QMS_MIXED):
+        with solver_for(d, precision=prec) as s:
+            for which in (native.K_VCYCLE_A, native.K_VCYCLE_V, native.K_ITERATION):
+                ms, nbytes = s.time_kernel(which, 2, v=[1.0, 0.0, 0.0])
+                assert ms > 0 and nbytes == 0, (prec, which)
 
 
 @pytest.mark.parametrize("name,mk", CASES, ids=[c[0] for c in CASES])
