@@ -212,7 +212,8 @@ struct Hierarchy {
 };
 
 struct MGParams {
-    int nu;          // pre/post smoothing sweeps
+    int nu;          // pre/post smoothing sweeps on level 0
+    int nu_coarse;   // pre/post smoothing sweeps on the coarse levels
     double omega;    // Jacobi damping
     double alpha;    // coarse-correction scaling
 };

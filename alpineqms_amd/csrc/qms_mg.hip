@@ -652,6 +652,42 @@ __global__ void __launch_bounds__(256) k_mg_restrict(LevelDims fd, LevelDims cd,
     }
 }
 
+// The restriction bc = P^T r together with the coarse level's first sweep
+// from zero, xo = omega bc / diag (0 on inactive rows): the same sums and
+// the same expression as k_mg_restrict followed by k_mg_cop's first sweep,
+// in one launch.
+template <int C, typename T, typename P>
+__device__ __forceinline__ void restrict_first_comp(const P &pv, const LevelDims &fd, const LevelDims &cd, int I,
+                                                    int J, int K, int64_t pc, const T *__restrict__ r,
+                                                    T *__restrict__ bc, T *__restrict__ xo, T omega) {
+    T s = (T)0;
+#pragma unroll
+    for (int ch = 0; ch < 8; ++ch) {
+        const int i = 2 * I + (ch & 1), j = 2 * J + ((ch >> 1) & 1), k = 2 * K + (ch >> 2);
+        if (inside(fd, i, j, k)) s += r[(int64_t)C * fd.n1 + lidx(fd, i, j, k)];
+    }
+    const int64_t q = (int64_t)C * cd.n1 + pc;
+    bc[q] = s;
+    Row<T> row;
+    const bool act = pv.template row_all<C>(I, J, K, row);
+    const T v = omega * s / row_diag(row);
+    xo[q] = act ? v : (T)0;
+}
+
+template <typename T, typename P, int NC>
+__global__ void __launch_bounds__(256) k_mg_restrict_first(P pv, LevelDims fd, const T *__restrict__ r,
+                                                           T *__restrict__ bc, T *__restrict__ xo, T omega) {
+    const LevelDims cd = pv.dims();
+    int I, J, K;
+    const int64_t pc = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (!node_of(cd, pc, I, J, K)) return;
+    restrict_first_comp<0, T>(pv, fd, cd, I, J, K, pc, r, bc, xo, omega);
+    if (NC == 3) {
+        restrict_first_comp<1, T>(pv, fd, cd, I, J, K, pc, r, bc, xo, omega);
+        restrict_first_comp<2, T>(pv, fd, cd, I, J, K, pc, r, bc, xo, omega);
+    }
+}
+
 // Coarsest level: x = Ainv b over its active unknowns, every other entry 0
 // (one workgroup of MG_COARSE_THREADS).  b is gathered into LDS once.  Row r
 // is summed by G threads (G = min(8, threads / n)), thread g over columns
@@ -1183,8 +1219,21 @@ static void resid(const Hierarchy<T> &h, int l, const T *x, const T *b, T *r, hi
             hipLaunchKernelGGL((k_mg_resid<T, P, NC>), g1(n1), dim3(256), 0, s, pv, x, b, r);
     });
 }
+// Smoothing sweeps before and after the coarse correction on level l.
+static inline int level_nu(const MGParams &mp, int l) { return l == 0 ? mp.nu : mp.nu_coarse; }
+
+// Where level l's first sweep from zero writes: pre-smoothing ping-pongs
+// between x and L.t and must end in x.
 template <typename T>
-static void vcycle_rec(Hierarchy<T> &h, const MGParams &mp, int l, T *x, const T *b, hipStream_t s) {
+static T *first_target(Hierarchy<T> &h, const MGParams &mp, int l) {
+    return (level_nu(mp, l) % 2) ? h.lev[l].x : h.lev[l].t;
+}
+
+// first_done: the caller's restriction already wrote the first sweep from
+// zero into first_target (coarse levels).
+template <typename T>
+static void vcycle_rec(Hierarchy<T> &h, const MGParams &mp, int l, T *x, const T *b, hipStream_t s,
+                       bool first_done = false) {
     MgLevel<T> &L = h.lev[l];
     const int nc = L.ncomp;
     const int64_t nv = (int64_t)nc * L.d.n1;
@@ -1194,21 +1243,35 @@ static void vcycle_rec(Hierarchy<T> &h, const MGParams &mp, int l, T *x, const T
         return;
     }
     const T w = (T)mp.omega;
+    const int nu = level_nu(mp, l);
     // pre-smoothing from zero, ping-pong between x and L.t, ending in x
-    T *cur = (mp.nu % 2) ? x : L.t, *oth = (mp.nu % 2) ? L.t : x;
-    smooth<T>(h, l, nullptr, b, cur, w, true, s);
-    for (int it = 1; it < mp.nu; ++it) {
+    T *cur = (nu % 2) ? x : L.t, *oth = (nu % 2) ? L.t : x;
+    if (!first_done) smooth<T>(h, l, nullptr, b, cur, w, true, s);
+    for (int it = 1; it < nu; ++it) {
         smooth<T>(h, l, cur, b, oth, w, false, s);
         std::swap(cur, oth);
     }
     resid<T>(h, l, x, b, L.r, s);
     MgLevel<T> &C = h.lev[l + 1];
-    hipLaunchKernelGGL(k_mg_restrict<T>, g1(C.d.n1), dim3(256), 0, s, L.d, C.d, nc, (const T *)L.r, C.b);
-    vcycle_rec<T>(h, mp, l + 1, C.x, C.b, s);
+    if (l + 1 < h.nlev - 1) {
+        // restriction fused with the next level's first sweep
+        T *xf = first_target(h, mp, l + 1);
+        on_level(h, l + 1, [&](auto pv, auto ncc) {
+            using P = decltype(pv);
+            constexpr int NC = decltype(ncc)::value;
+            if constexpr (is_coarse_prov<P>::value)
+                hipLaunchKernelGGL((k_mg_restrict_first<T, P, NC>), g1(C.d.n1), dim3(256), 0, s, pv, L.d,
+                                   (const T *)L.r, C.b, xf, w);
+        });
+        vcycle_rec<T>(h, mp, l + 1, C.x, C.b, s, true);
+    } else {
+        hipLaunchKernelGGL(k_mg_restrict<T>, g1(C.d.n1), dim3(256), 0, s, L.d, C.d, nc, (const T *)L.r, C.b);
+        vcycle_rec<T>(h, mp, l + 1, C.x, C.b, s);
+    }
     // post-smoothing; the first sweep reads x + alpha * P xc (the prolongation)
     cur = x;
     oth = L.t;
-    for (int it = 0; it < mp.nu; ++it) {
+    for (int it = 0; it < nu; ++it) {
         if (it == 0) smooth<T>(h, l, cur, b, oth, w, false, s, Prolong<T>{C.x, C.d, (T)mp.alpha});
         else smooth<T>(h, l, cur, b, oth, w, false, s);
         std::swap(cur, oth);

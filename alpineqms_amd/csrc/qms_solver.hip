@@ -276,6 +276,7 @@ OpParams make_op(const qms_handle *h, const double v[3], double dt, int tr) {
 MGParams mgp(const qms_handle *h) {
     MGParams m;
     m.nu = h->cfg.mg_nu;
+    m.nu_coarse = h->cfg.mg_nu_coarse > 0 ? h->cfg.mg_nu_coarse : h->cfg.mg_nu;
     m.omega = h->cfg.mg_omega;
     m.alpha = h->cfg.mg_alpha;
     return m;
@@ -618,6 +619,8 @@ int qms_config_default(qms_config *cfg) {
     cfg->mg_omega = 0.7;
     cfg->mg_alpha = 1.8;
     cfg->use_graphs = 1;
+    cfg->inner_tol = 1e-6;
+    cfg->mg_nu_coarse = 6;
     return QMS_OK;
 }
 
@@ -635,7 +638,8 @@ int qms_create(const qms_config *cfg, const uint16_t *cell_slot, const double *s
         }
         if (!(cfg->h[a] > 0)) { g_last_error = "qms_create: grid steps must be positive"; return QMS_ERR_ARG; }
     }
-    if (cfg->restart < 1 || cfg->restart > 64 || cfg->mg_nu < 1 || !(cfg->tol > 0) || cfg->max_iters < 1 ||
+    if (cfg->restart < 1 || cfg->restart > 64 || cfg->mg_nu < 1 || cfg->mg_nu_coarse > 64 || !(cfg->tol > 0) ||
+        cfg->max_iters < 1 ||
         (cfg->precision != QMS_FP64 && cfg->precision != QMS_MIXED)) {
         g_last_error = "qms_create: invalid solver parameters";
         return QMS_ERR_ARG;

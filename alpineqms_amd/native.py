@@ -66,6 +66,7 @@ class Config(ctypes.Structure):
         ("mg_alpha", ctypes.c_double),
         ("use_graphs", ctypes.c_int),
         ("inner_tol", ctypes.c_double),
+        ("mg_nu_coarse", ctypes.c_int),
     ]
 
 
@@ -168,7 +169,7 @@ def default_config() -> Config:
 def make_config(cells: Sequence[int], h: Sequence[float], dt: float, bc: Sequence[float],
                 precision: int = QMS_FP64, tol: float = 1e-12, restart: int = 30, max_iters: int = 3000,
                 mg_nu: int = 4, mg_omega: float = 0.7, mg_alpha: float = 1.8, use_graphs: bool = True,
-                inner_tol: float = 0.0) -> Config:
+                inner_tol: float = 0.0, mg_nu_coarse: int = 6) -> Config:
     cfg = default_config()
     for a in range(3):
         cfg.cells[a] = int(cells[a])
@@ -188,6 +189,7 @@ def make_config(cells: Sequence[int], h: Sequence[float], dt: float, bc: Sequenc
     cfg.mg_alpha = float(mg_alpha)
     cfg.use_graphs = 1 if use_graphs else 0
     cfg.inner_tol = float(inner_tol)
+    cfg.mg_nu_coarse = int(mg_nu_coarse)
     return cfg
 
 
@@ -220,10 +222,11 @@ class Solver:
     def __init__(self, cells: Sequence[int], h: Sequence[float], dt: float, bc: Sequence[float],
                  cell_slot: np.ndarray, slot_props: np.ndarray, precision: int = QMS_FP64,
                  tol: float = 1e-12, restart: int = 30, max_iters: int = 3000, mg_nu: int = 4,
-                 mg_omega: float = 0.7, mg_alpha: float = 1.8, use_graphs: bool = True, inner_tol: float = 0.0):
+                 mg_omega: float = 0.7, mg_alpha: float = 1.8, use_graphs: bool = True, inner_tol: float = 0.0,
+                 mg_nu_coarse: int = 6):
         L = load()
         cfg = make_config(cells, h, dt, bc, precision, tol, restart, max_iters, mg_nu, mg_omega, mg_alpha,
-                          use_graphs, inner_tol)
+                          use_graphs, inner_tol, mg_nu_coarse)
         self.cells = tuple(int(c) for c in cells)
         self.nodes = tuple(c + 1 for c in self.cells)
         self.dt = float(dt)

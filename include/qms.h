@@ -60,6 +60,8 @@ typedef struct {
                             0: its kernels are launched one by one (same results bit for bit) */
     double inner_tol;    /* QMS_MIXED: each FP32 inner FGMRES cycle stops once it has reduced the
                             FP64 residual it started from by this factor (<= 0: 1e-6) */
+    int mg_nu_coarse;    /* Jacobi sweeps before and after the correction on the coarse levels
+                            (<= 0: mg_nu) */
 } qms_config;
 
 typedef struct {
@@ -81,8 +83,9 @@ typedef struct {
                                 above 10 x tol fail the solve with QMS_ERR_NOCONV) */
 } qms_stats;
 
-/* Fills defaults: FP64, tol 1e-12, restart 30, 3000 iterations, nu 4,
- * omega 0.7, alpha 1.8, graphs on. */
+/* Fills defaults: FP64, tol 1e-12, restart 30, 3000 iterations, nu 4 on
+ * level 0 and 6 on the coarse levels, omega 0.7, alpha 1.8, graphs on,
+ * inner_tol 1e-6. */
 int qms_config_default(qms_config *cfg);
 
 /* Setup and material coefficients: replaces QMS_3D.f90:172-269 (sizes,
