@@ -265,8 +265,15 @@ void launch_axpy_list(double *x, const float *const *Z, int nz, const double *y_
 template <typename TE>
 void launch_scale_dev(TE *dst, const TE *src, const double *a_dev, int64_t n1, int64_t offv, hipStream_t s);
 // FGMRES cycle scalars on the device (st[KS_*])
-enum { KS_BNORM = 0, KS_TOL = 1, KS_INVH = 2, KS_EST = 3, KS_K = 4, KS_ITERS = 5, KS_COUNT = 8 };
-void launch_cycle_start(int m, double beta, double bnorm, double tol, double *g, double *st, hipStream_t s);
+// KS_FLOOR: below this estimate a cycle that stops improving ends
+// (KS_STALL = 1); KS_HIST..KS_HIST+3: the estimates of the last four
+// iterations (ring by iteration index).
+enum {
+    KS_BNORM = 0, KS_TOL = 1, KS_INVH = 2, KS_EST = 3, KS_K = 4, KS_ITERS = 5, KS_FLOOR = 6, KS_STALL = 7,
+    KS_HIST = 8, KS_COUNT = 12
+};
+void launch_cycle_start(int m, double beta, double bnorm, double tol, double floor, double *g, double *st,
+                        hipStream_t s);
 void launch_givens(int j, int m, const double *hcol, double *H, double *cs, double *sn, double *g, double *st,
                    hipStream_t s);
 void launch_backsub(int k, int m, const double *H, const double *g, double *y, hipStream_t s);

@@ -245,7 +245,7 @@ __global__ void __launch_bounds__(256) k_scale_dev(TE *__restrict__ dst, const T
 // round trip.  st holds the cycle's scalars (KS_* below).
 // ---------------------------------------------------------------------------
 // A new cycle from a residual of norm beta: g = beta e_1, no iteration done.
-__global__ void k_cycle_start(int m, double beta, double bnorm, double tol, double *__restrict__ g,
+__global__ void k_cycle_start(int m, double beta, double bnorm, double tol, double floor, double *__restrict__ g,
                               double *__restrict__ st) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     for (int i = 0; i <= m; ++i) g[i] = 0.0;
@@ -256,6 +256,9 @@ __global__ void k_cycle_start(int m, double beta, double bnorm, double tol, doub
     st[KS_EST] = bnorm > 0 ? beta / bnorm : beta;
     st[KS_K] = 0.0;
     st[KS_ITERS] = 0.0;
+    st[KS_FLOOR] = floor;
+    st[KS_STALL] = 0.0;
+    for (int i = 0; i < 4; ++i) st[KS_HIST + i] = 0.0;
 }
 
 // Iteration j: hcol[0..j] are the Gram-Schmidt coefficients of w against
@@ -287,6 +290,14 @@ __global__ void k_givens(int j, int m, const double *__restrict__ hcol, double *
     st[KS_EST] = est;
     st[KS_ITERS] = (double)(j + 1);
     if (st[KS_K] == 0.0 && (est <= 0.5 * st[KS_TOL] || hn == 0.0 || d == 0.0)) st[KS_K] = (double)(j + 1);
+    // stagnation below the floor: less than 20% progress over the last four
+    // iterations ends the cycle here
+    const double est4 = st[KS_HIST + (j & 3)];       // the estimate of iteration j - 4
+    if (st[KS_K] == 0.0 && j >= 4 && est <= st[KS_FLOOR] && est > 0.8 * est4) {
+        st[KS_K] = (double)(j + 1);
+        st[KS_STALL] = 1.0;
+    }
+    st[KS_HIST + (j & 3)] = est;
     st[KS_INVH] = hn > 0 ? 1.0 / hn : 0.0;
 }
 
@@ -409,8 +420,9 @@ void launch_scale_dev(TE *dst, const TE *src, const double *a_dev, int64_t n1, i
 }
 template void launch_scale_dev<double>(double *, const double *, const double *, int64_t, int64_t, hipStream_t);
 template void launch_scale_dev<float>(float *, const float *, const double *, int64_t, int64_t, hipStream_t);
-void launch_cycle_start(int m, double beta, double bnorm, double tol, double *g, double *st, hipStream_t s) {
-    hipLaunchKernelGGL(k_cycle_start, dim3(1), dim3(64), 0, s, m, beta, bnorm, tol, g, st);
+void launch_cycle_start(int m, double beta, double bnorm, double tol, double floor, double *g, double *st,
+                        hipStream_t s) {
+    hipLaunchKernelGGL(k_cycle_start, dim3(1), dim3(64), 0, s, m, beta, bnorm, tol, floor, g, st);
 }
 void launch_givens(int j, int m, const double *hcol, double *H, double *cs, double *sn, double *g, double *st,
                    hipStream_t s) {

@@ -67,6 +67,7 @@ class Config(ctypes.Structure):
         ("use_graphs", ctypes.c_int),
         ("inner_tol", ctypes.c_double),
         ("mg_nu_coarse", ctypes.c_int),
+        ("floor_tol", ctypes.c_double),
     ]
 
 
@@ -169,7 +170,7 @@ def default_config() -> Config:
 def make_config(cells: Sequence[int], h: Sequence[float], dt: float, bc: Sequence[float],
                 precision: int = QMS_FP64, tol: float = 1e-12, restart: int = 30, max_iters: int = 3000,
                 mg_nu: int = 4, mg_omega: float = 0.7, mg_alpha: float = 1.8, use_graphs: bool = True,
-                inner_tol: float = 0.0, mg_nu_coarse: int = 6) -> Config:
+                inner_tol: float = 0.0, mg_nu_coarse: int = 6, floor_tol: float = 0.0) -> Config:
     cfg = default_config()
     for a in range(3):
         cfg.cells[a] = int(cells[a])
@@ -190,6 +191,7 @@ def make_config(cells: Sequence[int], h: Sequence[float], dt: float, bc: Sequenc
     cfg.use_graphs = 1 if use_graphs else 0
     cfg.inner_tol = float(inner_tol)
     cfg.mg_nu_coarse = int(mg_nu_coarse)
+    cfg.floor_tol = float(floor_tol)
     return cfg
 
 
@@ -223,10 +225,10 @@ class Solver:
                  cell_slot: np.ndarray, slot_props: np.ndarray, precision: int = QMS_FP64,
                  tol: float = 1e-12, restart: int = 30, max_iters: int = 3000, mg_nu: int = 4,
                  mg_omega: float = 0.7, mg_alpha: float = 1.8, use_graphs: bool = True, inner_tol: float = 0.0,
-                 mg_nu_coarse: int = 6):
+                 mg_nu_coarse: int = 6, floor_tol: float = 0.0):
         L = load()
         cfg = make_config(cells, h, dt, bc, precision, tol, restart, max_iters, mg_nu, mg_omega, mg_alpha,
-                          use_graphs, inner_tol, mg_nu_coarse)
+                          use_graphs, inner_tol, mg_nu_coarse, floor_tol)
         self.cells = tuple(int(c) for c in cells)
         self.nodes = tuple(c + 1 for c in self.cells)
         self.dt = float(dt)

@@ -10,12 +10,16 @@ steady state, so the ramp is the variant whose steps carry work).
 
 Per step (QMS_3D.f90:1424-2234 restated on the GPU): right-hand side,
 Dirichlet values, extrapolated initial guess, FGMRES to a scaled relative
-residual of 1e-12 (the tolerance at which sigma*grad(V) agrees with the FP64
-path to about 1e-8; at 1e-10 it agrees to only 1e-6) with the block-triangular
-multigrid preconditioner, every iteration a replayed hipGraph.  Fields stay on the device between steps.
+residual of 1e-12 with the block-triangular multigrid preconditioner, every
+iteration a replayed hipGraph; a step whose FGMRES stagnates at the residual's
+rounding floor (which rises above 1e-12 as the fields grow over the run) is
+accepted within 100 x the tolerance and counted (`stalled_solves`).  Fields stay on the device between steps.
 
-Timed: `--warmup` untimed steps after the steady state, then `--steps` steps,
-each a blocking call; `ms_per_step` is the mean of the timed steps.  The
+Timed: the steady state, `--warmup` untimed steps, then (by default) every
+remaining step of the 200-step run, each a blocking call; `ms_per_step` is
+the mean of the timed steps.  The frame velocity grows over the run, and so
+do the FGMRES iterations per step: the early steps alone would overstate the
+rate.  The
 mixed-precision path (FP64 refinement around an inner FGMRES with the A
 components of its Krylov vectors, its Z basis and its A-block multigrid in
 FP32; V components, V-block multigrid, residuals and fields FP64) is
@@ -227,8 +231,11 @@ def batch_steps_config4(ndesigns: int, steps: int, ramp_v: float = 0.0, tol: flo
 def parser() -> argparse.ArgumentParser:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=0,
+                    help="timed steps after the warmup (0: every remaining step of the 200-step run)")
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--check-every", type=int, default=10,
+                    help="compare the paths' A, B and sigma*grad(V) every this many timed steps (and at the last)")
     ap.add_argument("--nodes", type=int, default=128)
     ap.add_argument("--tol", type=float, default=TOL, help="scaled relative residual of every solve")
     ap.add_argument("--no-graphs", action="store_true",
@@ -268,7 +275,14 @@ def main():
         solvers[name] = s
         runs[name] = {"steady_iters": st.last_iters, "step_ms": [], "iters": []}
     coef = next(iter(solvers.values())).get_coeffs()
+    nsteps = a.steps if a.steps > 0 else inp.steps - a.warmup
+    if a.warmup + nsteps > inp.steps:
+        print(f"bench: the run has {inp.steps} steps", file=sys.stderr)
+        return 2
+    a.steps = nsteps
     errors = {"A": 0.0, "B": 0.0, "sigma_grad_V": 0.0} if len(names) == 2 else None
+    checked = []
+    err_steps = []
     for n in range(1, a.warmup + a.steps + 1):
         v = program.step_velocity(inp.traj, inp.dt, n)
         for name, s in solvers.items():
@@ -281,17 +295,21 @@ def main():
                 runs[name]["iters"].append(int(s.stats().total_iters - q0))
         if n > a.warmup:
             msg = ", ".join(f"{k} {runs[k]['step_ms'][-1]:.2f} ms ({runs[k]['iters'][-1]} it)" for k in names)
-            if errors is not None:
+            last = n == a.warmup + a.steps
+            if errors is not None and ((n - a.warmup) % a.check_every == 0 or last):
+                checked.append(n)
                 f64, fmx = solvers["fp64"].get_fields(), solvers["mixed"].get_fields()
                 B64, Bmx = solvers["fp64"].get_h(unit_mu=True), solvers["mixed"].get_h(unit_mu=True)
                 e = state_errors(d, coef, fmx, f64, Bmx, B64)
                 for k in errors:
                     errors[k] = max(errors[k], e[k])
+                err_steps.append({"step": n, **e})
                 msg += f", mixed vs fp64: A {e['A']:.1e} B {e['B']:.1e} sgV {e['sigma_grad_V']:.1e}"
             print(f"bench: step {n}: {msg}", file=sys.stderr, flush=True)
     for name in runs:
         r = runs[name]
         r["stalled_solves"] = int(solvers[name].stats().stalled_solves)
+        r["total_s"] = sum(r["step_ms"]) / 1e3
         r["ms_per_step"] = sum(r["step_ms"]) / len(r["step_ms"])
         r["steps_per_s"] = 1e3 / r["ms_per_step"]
         r["dof_steps_per_s"] = d.dof * r["steps_per_s"]
@@ -341,12 +359,14 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"config 3: {nx}x{ny}x{nz} nodes, 0.25 mm, 24 boxes + 40x40x8 slab, frame velocity "
                                f"ramp 0 -> 5 m/s over 200 steps of 2 us, steps {a.warmup + 1}-"
-                               f"{a.warmup + a.steps}, {main_name} path, FGMRES(30) to {a.tol:g} scaled residual",
+                               f"{a.warmup + a.steps} timed, {main_name} path, FGMRES(30) to {a.tol:g} scaled "
+                               f"residual (accepted at stagnation within 100 x that)",
                    "dof": d.dof},
         "paths": dict(runs, **({"mixed_over_fp64": runs["mixed"]["steps_per_s"] / runs["fp64"]["steps_per_s"]}
                                if len(names) == 2 else {})),
         "dof_steps_per_s": head["dof_steps_per_s"],
         "max_rel_error_mixed_vs_fp64": errors,
+        "rel_error_mixed_vs_fp64_by_step": err_steps,
         "roofline": {"kernel": f"k_a0_zm (level-0 A-block damped Jacobi sweep, {sw}, 3 components per node, "
                                f"LDS z-march, rows from the coefficient class table)",
                      "bound": "hbm", "achieved": gbs_s, "peak": HBM_PEAK_GBS, "unit": "GB/s",

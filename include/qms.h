@@ -62,6 +62,10 @@ typedef struct {
                             FP64 residual it started from by this factor (<= 0: 1e-6) */
     int mg_nu_coarse;    /* Jacobi sweeps before and after the correction on the coarse levels
                             (<= 0: mg_nu) */
+    double floor_tol;    /* a solve whose FGMRES stagnates (the residual estimate improves by less
+                            than 20% over four iterations, or a cycle does not halve the residual)
+                            is accepted when its FP64 scaled residual is at most this, and counted
+                            in qms_stats.stalled_solves (<= 0: 100 x tol) */
 } qms_config;
 
 typedef struct {
@@ -78,9 +82,8 @@ typedef struct {
                                 queues iterations ahead and reads convergence between chunks) */
     int mg_classes_a, mg_classes_v;   /* row classes of the first coarse multigrid level of the A and
                                          V blocks (0: that level stores explicit stencil arrays) */
-    int64_t stalled_solves;  /* solves accepted on stagnation: a restart cycle did not halve the residual
-                                it started from, which then was within 10 x tol (two stagnant cycles
-                                above 10 x tol fail the solve with QMS_ERR_NOCONV) */
+    int64_t stalled_solves;  /* solves accepted on stagnation within floor_tol (two stagnant cycles
+                                above it fail the solve with QMS_ERR_NOCONV) */
 } qms_stats;
 
 /* Fills defaults: FP64, tol 1e-12, restart 30, 3000 iterations, nu 4 on
