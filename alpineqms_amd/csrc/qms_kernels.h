@@ -187,6 +187,7 @@ struct MgLevel {
     T *tab = nullptr;
     int ncls = 0;
     T *x = nullptr, *b = nullptr, *r = nullptr, *t = nullptr;   // [ncomp][n1] (x, b unused on level 0)
+    T *dg = nullptr;           // level 0: [ncomp][n1] row diagonals, 0 on inactive rows
 };
 // Table row layout of a class-compressed level: per component, values
 // [0..5] conductances, [6] excess diagonal, [7] 1/diag (0: inactive row),
@@ -216,6 +217,7 @@ struct MGParams {
     int nu_coarse;   // pre/post smoothing sweeps on the coarse levels
     double omega;    // Jacobi damping
     double alpha;    // coarse-correction scaling
+    int tail;        // run the level above the coarsest and the coarsest solve as one launch
 };
 
 // Device allocations of the hierarchy go through the owner (which frees them).
@@ -231,8 +233,22 @@ template <typename T>
 void mg_setup(const Grid &g, const HostCoef &hc, const CoeffsT<T> &dc, double dt, int block, const MgAlloc &al,
               Hierarchy<T> &h);
 // x0 <- one V-cycle applied to b0 (x0, b0: [ncomp][n1] of level 0).
+// first_done: level 0's first sweep from zero is already in
+// mg_first_target(h, mp, x0) (written by launch_d2t_first / launch_first_sweep).
 template <typename T>
-void mg_vcycle(Hierarchy<T> &h, const MGParams &mp, T *x0, const T *b0, hipStream_t s);
+void mg_vcycle(Hierarchy<T> &h, const MGParams &mp, T *x0, const T *b0, hipStream_t s, bool first_done = false);
+// Where the V-cycle reads level 0's first sweep from (with first_done).
+template <typename T>
+T *mg_first_target(Hierarchy<T> &h, const MGParams &mp, T *x0);
+// Level 0's first sweep from zero as a stream: xf = omega b / diag (0 on
+// inactive rows), b = the level-0 right-hand side.
+template <typename T>
+void launch_first_sweep(const Hierarchy<T> &h, const MGParams &mp, const T *b, T *xf, hipStream_t s);
+// dst = src * scale (the A block's right-hand side D_A v_A) together with
+// level 0's first sweep from zero on it, xf = omega dst / diag.
+template <typename TS, typename T>
+void launch_d2t_first(const Hierarchy<T> &h, const MGParams &mp, const TS *src, const double *scale, T *dst, T *xf,
+                      hipStream_t s);
 // One level-0 smoother sweep (for kernel timing).
 template <typename T>
 void mg_fine_sweep(Hierarchy<T> &h, const MGParams &mp, const T *x, const T *b, T *xo, hipStream_t s);
@@ -240,9 +256,6 @@ template <typename T>
 void launch_fill(T *x, int64_t n, T v, hipStream_t s);
 
 // conversions / vector kernels at the Krylov / multigrid boundary
-void launch_d2t(const double *src, const double *scale, float *dst, int64_t n, hipStream_t s);
-void launch_d2t(const double *src, const double *scale, double *dst, int64_t n, hipStream_t s);
-void launch_d2t(const float *src, const double *scale, float *dst, int64_t n, hipStream_t s);
 void launch_cvt(const double *src, float *dst, int64_t n, hipStream_t s);
 
 // Krylov helpers.  A Krylov vector has two segments: the A components (3 n1

@@ -370,6 +370,53 @@ __global__ void __launch_bounds__(256) k_mg_smooth(P pv, const T *__restrict__ x
     }
 }
 
+// Row diagonals of level 0 (0 on inactive rows), computed once per
+// hierarchy: the divisor of the first sweep from zero, which then reads only
+// b and this array.
+template <int C, typename T, typename P>
+__device__ __forceinline__ T diag_comp(const P &pv, int i, int j, int k) {
+    Row<T> r;
+    return pv.template row<C>(i, j, k, r) ? row_diag(r) : (T)0;
+}
+
+template <typename T, typename P, int NC>
+__global__ void __launch_bounds__(256) k_mg_diag(P pv, T *__restrict__ dg) {
+    const LevelDims d = pv.dims();
+    int i, j, k;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (!node_of(d, p, i, j, k)) return;
+    dg[p] = diag_comp<0, T>(pv, i, j, k);
+    if constexpr (NC == 3) {
+        dg[d.n1 + p] = diag_comp<1, T>(pv, i, j, k);
+        dg[2 * d.n1 + p] = diag_comp<2, T>(pv, i, j, k);
+    }
+}
+
+// The first sweep from zero, xf = omega b / diag (0 on inactive rows): the
+// expression of k_mg_smooth's FIRST form, as a stream.
+template <typename T>
+__global__ void __launch_bounds__(256) k_first_sweep(const T *__restrict__ b, const T *__restrict__ dg, T omega,
+                                                     T *__restrict__ xf, int64_t n) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const T d = dg[p];
+    xf[p] = d != (T)0 ? omega * b[p] / d : (T)0;
+}
+
+// dst = src * scale (the FP64 product, rounded to T) and the first sweep from
+// zero on it.
+template <typename TS, typename T>
+__global__ void __launch_bounds__(256) k_d2t_first(const TS *__restrict__ src, const double *__restrict__ scale,
+                                                   const T *__restrict__ dg, T omega, T *__restrict__ dst,
+                                                   T *__restrict__ xf, int64_t n) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const T bv = (T)((double)src[p] * scale[p]);
+    const T d = dg[p];
+    dst[p] = bv;
+    xf[p] = d != (T)0 ? omega * bv / d : (T)0;
+}
+
 template <int C, int NC, typename T, typename P>
 __device__ __forceinline__ void resid_comp(const P &pv, const LevelDims &d, int64_t p, int i, int j, int k,
                                            const T *__restrict__ x, const T *__restrict__ b, T *__restrict__ r) {
@@ -429,21 +476,16 @@ struct Prolong {
     }
 };
 
-template <int C, int NC, int MODE, bool PRO, typename T, typename P>
-__device__ __forceinline__ void cop_comp(const P &pv, const LevelDims &d, int64_t p, int i, int j, int k,
-                                         const T *__restrict__ x, const T *__restrict__ b, T *__restrict__ out,
-                                         T omega, const Prolong<T> &pr) {
-    Row<T> r;
-    const bool act = pv.template row_all<C>(i, j, k, r);
-    const int64_t q = (int64_t)C * d.n1 + p;
+// The output of component C's row from the loaded row r (act: the row is
+// active), the right-hand side entry bq and X(c, i, j, k), the value of
+// component c at a node, which clamps the node into the box.  Shared by the
+// one-launch-per-sweep kernels and the one-workgroup tail of the V-cycle.
+template <int C, int NC, int MODE, typename T, typename XF>
+__device__ __forceinline__ T cop_eval(const Row<T> &r, bool act, const XF &X, int i, int j, int k, T bq, T omega) {
     T v;
     if (MODE == 1) {
-        v = omega * b[q] / row_diag(r);
+        v = omega * bq / row_diag(r);
     } else {
-        auto X = [&](int c, int ni, int nj, int nk) {
-            ni = max(0, min(ni, d.nx - 1)); nj = max(0, min(nj, d.ny - 1)); nk = max(0, min(nk, d.nz - 1));
-            return pr.template at<PRO>(x, (int64_t)c * d.n1 + lidx(d, ni, nj, nk), c, ni, nj, nk);
-        };
         const T x0 = X(C, i, j, k);
         T y = r.exc * x0;
         T diag = r.exc;
@@ -456,9 +498,25 @@ __device__ __forceinline__ void cop_comp(const P &pv, const LevelDims &d, int64_
 #pragma unroll
             for (int s = 0; s < 8; ++s) y += r.cross[s] * X(xo_c(C, s), i + xo_i(C, s), j + xo_j(C, s), k + xo_k(C, s));
         }
-        v = MODE == 0 ? x0 + omega * (b[q] - y) / diag : b[q] - y;
+        v = MODE == 0 ? x0 + omega * (bq - y) / diag : bq - y;
     }
-    out[q] = act ? v : (T)0;
+    return act ? v : (T)0;
+}
+
+// Component C's output at a node of a coarse level (stored by the caller, so
+// that every component's loads are issued before the first store, which the
+// compiler cannot move loads of the non-restrict class table across).
+template <int C, int NC, int MODE, bool PRO, typename T, typename P>
+__device__ __forceinline__ T cop_comp(const P &pv, const LevelDims &d, int64_t p, int i, int j, int k,
+                                      const T *__restrict__ x, const T *__restrict__ b, T omega,
+                                      const Prolong<T> &pr) {
+    Row<T> r;
+    const bool act = pv.template row_all<C>(i, j, k, r);
+    auto X = [&](int c, int ni, int nj, int nk) {
+        ni = max(0, min(ni, d.nx - 1)); nj = max(0, min(nj, d.ny - 1)); nk = max(0, min(nk, d.nz - 1));
+        return pr.template at<PRO>(x, (int64_t)c * d.n1 + lidx(d, ni, nj, nk), c, ni, nj, nk);
+    };
+    return cop_eval<C, NC, MODE, T>(r, act, X, i, j, k, b[(int64_t)C * d.n1 + p], omega);
 }
 
 template <typename T, typename P, int NC, int MODE, bool PRO>
@@ -468,11 +526,14 @@ __global__ void __launch_bounds__(256) k_mg_cop(P pv, const T *__restrict__ x, c
     int i, j, k;
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (!node_of(d, p, i, j, k)) return;
-    cop_comp<0, NC, MODE, PRO, T>(pv, d, p, i, j, k, x, b, out, omega, pr);
-    if (NC == 3) {
-        cop_comp<1, NC, MODE, PRO, T>(pv, d, p, i, j, k, x, b, out, omega, pr);
-        cop_comp<2, NC, MODE, PRO, T>(pv, d, p, i, j, k, x, b, out, omega, pr);
+    const T v0 = cop_comp<0, NC, MODE, PRO, T>(pv, d, p, i, j, k, x, b, omega, pr);
+    if constexpr (NC == 3) {
+        const T v1 = cop_comp<1, NC, MODE, PRO, T>(pv, d, p, i, j, k, x, b, omega, pr);
+        const T v2 = cop_comp<2, NC, MODE, PRO, T>(pv, d, p, i, j, k, x, b, omega, pr);
+        out[d.n1 + p] = v1;
+        out[2 * d.n1 + p] = v2;
     }
+    out[p] = v0;
 }
 
 // ---------------------------------------------------------------------------
@@ -657,23 +718,22 @@ __global__ void __launch_bounds__(256) k_mg_restrict(LevelDims fd, LevelDims cd,
 // the same expression as k_mg_restrict followed by k_mg_cop's first sweep,
 // in one launch.
 template <int C, typename T, typename P>
-__device__ __forceinline__ void restrict_first_comp(const P &pv, const LevelDims &fd, const LevelDims &cd, int I,
-                                                    int J, int K, int64_t pc, const T *__restrict__ r,
-                                                    T *__restrict__ bc, T *__restrict__ xo, T omega) {
+__device__ __forceinline__ void restrict_first_comp(const P &pv, const LevelDims &fd, int I, int J, int K,
+                                                    const T *__restrict__ r, T omega, T &sum, T &first) {
     T s = (T)0;
 #pragma unroll
     for (int ch = 0; ch < 8; ++ch) {
         const int i = 2 * I + (ch & 1), j = 2 * J + ((ch >> 1) & 1), k = 2 * K + (ch >> 2);
         if (inside(fd, i, j, k)) s += r[(int64_t)C * fd.n1 + lidx(fd, i, j, k)];
     }
-    const int64_t q = (int64_t)C * cd.n1 + pc;
-    bc[q] = s;
     Row<T> row;
     const bool act = pv.template row_all<C>(I, J, K, row);
     const T v = omega * s / row_diag(row);
-    xo[q] = act ? v : (T)0;
+    sum = s;
+    first = act ? v : (T)0;
 }
 
+// Every component's loads are issued before the first store (see k_mg_cop).
 template <typename T, typename P, int NC>
 __global__ void __launch_bounds__(256) k_mg_restrict_first(P pv, LevelDims fd, const T *__restrict__ r,
                                                            T *__restrict__ bc, T *__restrict__ xo, T omega) {
@@ -681,10 +741,16 @@ __global__ void __launch_bounds__(256) k_mg_restrict_first(P pv, LevelDims fd, c
     int I, J, K;
     const int64_t pc = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (!node_of(cd, pc, I, J, K)) return;
-    restrict_first_comp<0, T>(pv, fd, cd, I, J, K, pc, r, bc, xo, omega);
-    if (NC == 3) {
-        restrict_first_comp<1, T>(pv, fd, cd, I, J, K, pc, r, bc, xo, omega);
-        restrict_first_comp<2, T>(pv, fd, cd, I, J, K, pc, r, bc, xo, omega);
+    T s[NC], f[NC];
+    restrict_first_comp<0, T>(pv, fd, I, J, K, r, omega, s[0], f[0]);
+    if constexpr (NC == 3) {
+        restrict_first_comp<1, T>(pv, fd, I, J, K, r, omega, s[1], f[1]);
+        restrict_first_comp<2, T>(pv, fd, I, J, K, r, omega, s[2], f[2]);
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        bc[(int64_t)c * cd.n1 + pc] = s[c];
+        xo[(int64_t)c * cd.n1 + pc] = f[c];
     }
 }
 
@@ -695,6 +761,57 @@ __global__ void __launch_bounds__(256) k_mg_restrict_first(P pv, LevelDims fd, c
 // column-major, so the threads of a wave read consecutive entries of a
 // column, and the G partial sums are added in order.
 constexpr int MG_COARSE_MAX = 1024, MG_COARSE_THREADS = 1024;
+
+// The solve proper, on bs[0..n) (gathered b, FP64) with partial sums in
+// ps[0..MG_COARSE_THREADS); put(r, v) stores row r's result.  Every thread of
+// the workgroup calls it; the partition is that of MG_COARSE_THREADS threads
+// whatever the workgroup size (a smaller workgroup takes several of those
+// slots per thread), so the sums are the same bit for bit.
+template <typename F>
+__device__ __forceinline__ void coarse_core(int n, const double *__restrict__ invT, const double *bs, double *ps,
+                                            F put) {
+    constexpr int VT = MG_COARSE_THREADS;
+    const int G = n > 0 ? max(1, min(8, VT / n)) : 1;
+    const int RP = VT / G;
+    for (int r0 = 0; r0 < n; r0 += RP) {
+        for (int t = (int)threadIdx.x; t < VT; t += (int)blockDim.x) {
+            const int r = r0 + t % RP, g = t / RP;
+            double s = 0.0;
+            if (r < n && g < G) {
+                const int c0 = g * n / G, c1 = (g + 1) * n / G;
+                double a[4] = {0.0, 0.0, 0.0, 0.0};
+                int c = c0;
+                // 16 loads in flight, then the products in the order of the
+                // 4-wide loop below
+                for (; c + 16 <= c1; c += 16) {
+                    double v[16];
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) v[u] = invT[(int64_t)(c + u) * n + r];
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) a[u & 3] += v[u] * bs[c + u];
+                }
+                for (; c + 4 <= c1; c += 4) {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) a[u] += invT[(int64_t)(c + u) * n + r] * bs[c + u];
+                }
+                for (; c < c1; ++c) a[0] += invT[(int64_t)c * n + r] * bs[c];
+                s = (a[0] + a[1]) + (a[2] + a[3]);
+            }
+            ps[t] = s;
+        }
+        __syncthreads();
+        for (int t = (int)threadIdx.x; t < VT; t += (int)blockDim.x) {
+            const int r = r0 + t % RP, g = t / RP;
+            if (r < n && g == 0) {
+                double tot = 0.0;
+                for (int u = 0; u < G; ++u) tot += ps[t + u * RP];
+                put(r, tot);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(MG_COARSE_THREADS) k_mg_coarse(int n, int64_t nv, const int *__restrict__ idx,
                                                                  const double *__restrict__ invT,
@@ -704,30 +821,145 @@ __global__ void __launch_bounds__(MG_COARSE_THREADS) k_mg_coarse(int n, int64_t 
     for (int c = threadIdx.x; c < n; c += blockDim.x) bs[c] = (double)b[idx[c]];
     for (int64_t q = threadIdx.x; q < nv; q += blockDim.x) x[q] = (T)0;
     __syncthreads();
-    const int G = n > 0 ? max(1, min(8, (int)blockDim.x / n)) : 1;
-    for (int r0 = 0; r0 < n; r0 += (int)blockDim.x / G) {
-        const int t = (int)threadIdx.x;
-        const int r = r0 + t % ((int)blockDim.x / G), g = t / ((int)blockDim.x / G);
-        double s = 0.0;
-        if (r < n && g < G) {
-            const int c0 = g * n / G, c1 = (g + 1) * n / G;
-            double a[4] = {0.0, 0.0, 0.0, 0.0};
-            int c = c0;
-            for (; c + 4 <= c1; c += 4) {
+    coarse_core(n, invT, bs, ps, [&](int r, double v) { x[idx[r]] = (T)v; });
+}
+
+// ---------------------------------------------------------------------------
+// The tail of the V-cycle in one workgroup: the level above the coarsest
+// (at most mg_tail_threads nodes, one thread per node, its rows held in
+// registers and its vectors in LDS) and the coarsest solve, in one launch
+// instead of 2 nu + 4.  Input: the residual rf of the level below (fd);
+// output: xout, the tail level's x after its post-smoothing.  The steps and
+// their arithmetic are those of the one-launch-per-step path (the restriction
+// fused with the first sweep, nu - 1 sweeps, the residual, the restriction to
+// the coarsest level, its dense solve, the prolongation fused into the first
+// post-sweep, nu - 1 sweeps), with barriers between them.
+// ---------------------------------------------------------------------------
+constexpr int MG_TAIL_LDS = 49152;
+template <typename T>
+constexpr int mg_tail_rows() { return MG_TAIL_LDS / (4 * (int)sizeof(T)); }
+// Threads (= nodes at most) of the tail: the three FP64 rows of an A-block
+// node take about 90 registers, so that case runs 512 threads, which leaves
+// 256 registers per lane and no spills.
+template <typename T, int NC>
+constexpr int mg_tail_threads() { return (sizeof(T) == 8 && NC == 3) ? 512 : 1024; }
+
+template <typename T, typename P, int NC>
+__global__ void __launch_bounds__((mg_tail_threads<T, NC>())) k_mg_tail(P pv, LevelDims fd, const T *__restrict__ rf,
+                                                             LevelDims cd, int n, const int *__restrict__ idx,
+                                                             const double *__restrict__ invT, T *__restrict__ xout,
+                                                             T omega, T alpha, int nu) {
+    constexpr int R = mg_tail_rows<T>();
+    static_assert(R * sizeof(T) >= sizeof(double) * MG_COARSE_THREADS && R * sizeof(T) >= sizeof(double) * MG_COARSE_MAX,
+                  "the coarse solve's gathered b and partial sums live in two of the tail level's vectors");
+    // four vectors of the tail level (x ping-pong, b, r), then the coarsest
+    // level's b and x in node layout; the gathered coarse b and the partial
+    // sums of its solve reuse the r vector and the free x buffer
+    __shared__ double raw[MG_TAIL_LDS / 8 + 2 * MG_COARSE_MAX * sizeof(T) / 8];
+    T *const v0 = (T *)raw, *const v1 = v0 + R, *const bl = v1 + R, *const rl = bl + R;
+    T *const bcs = rl + R, *const xcs = bcs + MG_COARSE_MAX;
+    const LevelDims d = pv.dims();
+    const int64_t nvc = (int64_t)NC * cd.n1;
+    int i = 0, j = 0, k = 0;
+    const int p = (int)threadIdx.x;
+    const bool live = node_of(d, p, i, j, k);
+    Row<T> rw[NC];
+    bool act[NC];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) a[u] += invT[(int64_t)(c + u) * n + r] * bs[c + u];
+    for (int c = 0; c < NC; ++c) act[c] = false;
+    if (live) {
+        act[0] = pv.template row_all<0>(i, j, k, rw[0]);
+        if constexpr (NC == 3) {
+            act[1] = pv.template row_all<1>(i, j, k, rw[1]);
+            act[2] = pv.template row_all<2>(i, j, k, rw[2]);
+        }
+    }
+    auto Xof = [&](const T *xs) {
+        return [=, &d](int c, int ni, int nj, int nk) {
+            ni = max(0, min(ni, d.nx - 1)); nj = max(0, min(nj, d.ny - 1)); nk = max(0, min(nk, d.nz - 1));
+            return xs[c * d.n1 + lidx(d, ni, nj, nk)];
+        };
+    };
+    // one sweep (MODE 0) or the residual (MODE 2) of every row of the node
+    auto pass = [&](auto mode, const auto &X, T *dst) {
+        constexpr int MODE = decltype(mode)::value;
+        if (live) {
+            T o[NC];
+            o[0] = cop_eval<0, NC, MODE, T>(rw[0], act[0], X, i, j, k, bl[p], omega);
+            if constexpr (NC == 3) {
+                o[1] = cop_eval<1, NC, MODE, T>(rw[1], act[1], X, i, j, k, bl[d.n1 + p], omega);
+                o[2] = cop_eval<2, NC, MODE, T>(rw[2], act[2], X, i, j, k, bl[2 * d.n1 + p], omega);
             }
-            for (; c < c1; ++c) a[0] += invT[(int64_t)c * n + r] * bs[c];
-            s = (a[0] + a[1]) + (a[2] + a[3]);
-        }
-        ps[t] = s;
-        __syncthreads();
-        if (r < n && g == 0) {
-            double tot = 0.0;
-            for (int u = 0; u < G; ++u) tot += ps[t + u * ((int)blockDim.x / G)];
-            x[idx[r]] = (T)tot;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) dst[c * d.n1 + p] = o[c];
         }
         __syncthreads();
+    };
+    using M0 = std::integral_constant<int, 0>;
+    using M2 = std::integral_constant<int, 2>;
+    // restriction of rf fused with the first sweep from zero
+    T *cur = v0, *oth = v1;
+    if (live) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            T s = (T)0;
+#pragma unroll
+            for (int ch = 0; ch < 8; ++ch) {
+                const int fi = 2 * i + (ch & 1), fj = 2 * j + ((ch >> 1) & 1), fk = 2 * k + (ch >> 2);
+                if (inside(fd, fi, fj, fk)) s += rf[(int64_t)c * fd.n1 + lidx(fd, fi, fj, fk)];
+            }
+            const T v = omega * s / row_diag(rw[c]);
+            bl[c * d.n1 + p] = s;
+            cur[c * d.n1 + p] = act[c] ? v : (T)0;
+        }
+    }
+    __syncthreads();
+    for (int it = 1; it < nu; ++it) {
+        pass(M0(), Xof(cur), oth);
+        T *t = cur; cur = oth; oth = t;
+    }
+    pass(M2(), Xof(cur), rl);
+    // restriction to the coarsest level
+    if (p < cd.n1) {
+        int ci, cj, ck;
+        node_of(cd, p, ci, cj, ck);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            T s = (T)0;
+#pragma unroll
+            for (int ch = 0; ch < 8; ++ch) {
+                const int fi = 2 * ci + (ch & 1), fj = 2 * cj + ((ch >> 1) & 1), fk = 2 * ck + (ch >> 2);
+                if (inside(d, fi, fj, fk)) s += rl[c * d.n1 + lidx(d, fi, fj, fk)];
+            }
+            bcs[c * cd.n1 + p] = s;
+        }
+    }
+    __syncthreads();
+    // coarsest solve (gathered b in the r vector, the MG_COARSE_THREADS
+    // partial sums in the free x buffer)
+    double *bs = (double *)rl, *ps = (double *)oth;
+    for (int c = p; c < n; c += blockDim.x) bs[c] = (double)bcs[idx[c]];
+    for (int64_t q = p; q < nvc; q += blockDim.x) xcs[q] = (T)0;
+    __syncthreads();
+    coarse_core(n, invT, bs, ps, [&](int r, double v) { xcs[idx[r]] = (T)v; });
+    // post-smoothing, the first sweep reading x + alpha * (the coarse x of
+    // the parent node)
+    {
+        const T *xs = cur;
+        auto XP = [=, &d, &cd](int c, int ni, int nj, int nk) {
+            ni = max(0, min(ni, d.nx - 1)); nj = max(0, min(nj, d.ny - 1)); nk = max(0, min(nk, d.nz - 1));
+            return xs[c * d.n1 + lidx(d, ni, nj, nk)] + alpha * xcs[c * cd.n1 + lidx(cd, ni >> 1, nj >> 1, nk >> 1)];
+        };
+        pass(M0(), XP, oth);
+        T *t = cur; cur = oth; oth = t;
+    }
+    for (int it = 1; it < nu; ++it) {
+        pass(M0(), Xof(cur), oth);
+        T *t = cur; cur = oth; oth = t;
+    }
+    if (live) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) xout[(int64_t)c * d.n1 + p] = cur[c * d.n1 + p];
     }
 }
 
@@ -960,6 +1192,10 @@ FineConst<double> fine_const(const Grid &g, double dt) {
     return k;
 }
 
+static inline dim3 g1(int64_t n);
+template <typename T, typename F>
+static void on_level(const Hierarchy<T> &h, int l, F f);
+
 template <typename T>
 void mg_setup(const Grid &g, const HostCoef &hc, const CoeffsT<T> &dc, double dt, int block, const MgAlloc &al,
               Hierarchy<T> &h) {
@@ -973,8 +1209,14 @@ void mg_setup(const Grid &g, const HostCoef &hc, const CoeffsT<T> &dc, double dt
     h.fk.idt = (T)fkd.idt; h.fk.sigfloor = (T)fkd.sigfloor;
     h.g = g;
     for (int a = 0; a < 3; ++a) { h.im[a] = dc.im[a]; h.sg[a] = dc.sg[a]; }
-    h.cls = dc.cls;
-    h.ctab = dc.ctab;
+    // Level 0 reads its coefficients through the class table (2 bytes per
+    // node) in FP64 and in the V block.  The FP32 A block reads the six FP32
+    // arrays instead: measured faster at 128^3 (26.2 against 28.6 us per
+    // sweep), where the dependent loads class -> tuple cost more time than
+    // the 24 bytes per node they save.
+    const bool direct = std::is_same<T, float>::value && block == 0;
+    h.cls = direct ? nullptr : dc.cls;
+    h.ctab = direct ? nullptr : dc.ctab;
     {
         // level-0 z-march: 32 x 8 columns, the deepest z-chunk of 1..16 planes
         // that leaves at least 768 workgroups (three per CU) for the A block
@@ -993,6 +1235,12 @@ void mg_setup(const Grid &g, const HostCoef &hc, const CoeffsT<T> &dc, double dt
     h.lev[0].ncomp = nc;
     h.lev[0].t = (T *)al.alloc(sizeof(T) * (size_t)(nc * g.n1));
     h.lev[0].r = (T *)al.alloc(sizeof(T) * (size_t)(nc * g.n1));
+    h.lev[0].dg = (T *)al.alloc(sizeof(T) * (size_t)(nc * g.n1));
+    on_level(h, 0, [&](auto pv, auto ncc) {
+        using P = decltype(pv);
+        constexpr int NC = decltype(ncc)::value;
+        hipLaunchKernelGGL((k_mg_diag<T, P, NC>), g1(g.n1), dim3(256), 0, al.stream, pv, h.lev[0].dg);
+    });
     // coarse levels
     std::vector<HostLevel> hl;
     auto fine_row = [&](int c, int i, int j, int k, Row<double> &r) {
@@ -1222,6 +1470,18 @@ static void resid(const Hierarchy<T> &h, int l, const T *x, const T *b, T *r, hi
 // Smoothing sweeps before and after the coarse correction on level l.
 static inline int level_nu(const MGParams &mp, int l) { return l == 0 ? mp.nu : mp.nu_coarse; }
 
+// Whether level l (above the coarsest) and the coarsest solve run as one
+// k_mg_tail launch: l is a coarse level whose nodes, vectors and coarse
+// level fit one workgroup.
+template <typename T>
+static bool tail_fits(const Hierarchy<T> &h, const MGParams &mp, int l) {
+    if (!mp.tail || l < 1 || l != h.nlev - 2) return false;
+    const MgLevel<T> &L = h.lev[l], &K = h.lev[l + 1];
+    const int threads = L.ncomp == 3 ? mg_tail_threads<T, 3>() : mg_tail_threads<T, 1>();
+    return L.d.n1 <= threads && (int64_t)L.ncomp * L.d.n1 <= mg_tail_rows<T>() &&
+           (int64_t)K.ncomp * K.d.n1 <= MG_COARSE_MAX && h.ncoarse <= MG_COARSE_MAX;
+}
+
 // Where level l's first sweep from zero writes: pre-smoothing ping-pongs
 // between x and L.t and must end in x.
 template <typename T>
@@ -1253,7 +1513,18 @@ static void vcycle_rec(Hierarchy<T> &h, const MGParams &mp, int l, T *x, const T
     }
     resid<T>(h, l, x, b, L.r, s);
     MgLevel<T> &C = h.lev[l + 1];
-    if (l + 1 < h.nlev - 1) {
+    if (tail_fits(h, mp, l + 1)) {
+        // levels l+1 .. nlev-1 in one workgroup, leaving level l+1's x
+        const MgLevel<T> &K = h.lev[h.nlev - 1];
+        on_level(h, l + 1, [&](auto pv, auto ncc) {
+            using P = decltype(pv);
+            constexpr int NC = decltype(ncc)::value;
+            if constexpr (is_coarse_prov<P>::value)
+                hipLaunchKernelGGL((k_mg_tail<T, P, NC>), dim3(1), dim3(mg_tail_threads<T, NC>()), 0, s, pv, L.d,
+                                   (const T *)L.r, K.d, h.ncoarse, (const int *)h.coarse_idx,
+                                   (const double *)h.coarse_inv, C.x, w, (T)mp.alpha, level_nu(mp, l + 1));
+        });
+    } else if (l + 1 < h.nlev - 1) {
         // restriction fused with the next level's first sweep
         T *xf = first_target(h, mp, l + 1);
         on_level(h, l + 1, [&](auto pv, auto ncc) {
@@ -1280,8 +1551,27 @@ static void vcycle_rec(Hierarchy<T> &h, const MGParams &mp, int l, T *x, const T
 }
 
 template <typename T>
-void mg_vcycle(Hierarchy<T> &h, const MGParams &mp, T *x0, const T *b0, hipStream_t s) {
-    vcycle_rec<T>(h, mp, 0, x0, b0, s);
+void mg_vcycle(Hierarchy<T> &h, const MGParams &mp, T *x0, const T *b0, hipStream_t s, bool first_done) {
+    vcycle_rec<T>(h, mp, 0, x0, b0, s, first_done);
+}
+
+template <typename T>
+T *mg_first_target(Hierarchy<T> &h, const MGParams &mp, T *x0) {
+    return (level_nu(mp, 0) % 2) ? x0 : h.lev[0].t;
+}
+
+template <typename T>
+void launch_first_sweep(const Hierarchy<T> &h, const MGParams &mp, const T *b, T *xf, hipStream_t s) {
+    const int64_t n = (int64_t)h.lev[0].ncomp * h.lev[0].d.n1;
+    hipLaunchKernelGGL(k_first_sweep<T>, g1(n), dim3(256), 0, s, b, (const T *)h.lev[0].dg, (T)mp.omega, xf, n);
+}
+
+template <typename TS, typename T>
+void launch_d2t_first(const Hierarchy<T> &h, const MGParams &mp, const TS *src, const double *scale, T *dst, T *xf,
+                      hipStream_t s) {
+    const int64_t n = (int64_t)h.lev[0].ncomp * h.lev[0].d.n1;
+    hipLaunchKernelGGL((k_d2t_first<TS, T>), g1(n), dim3(256), 0, s, src, scale, (const T *)h.lev[0].dg,
+                       (T)mp.omega, dst, xf, n);
 }
 
 template <typename T>
@@ -1298,8 +1588,18 @@ template void mg_setup<float>(const Grid &, const HostCoef &, const CoeffsT<floa
                               Hierarchy<float> &);
 template void mg_setup<double>(const Grid &, const HostCoef &, const CoeffsT<double> &, double, int, const MgAlloc &,
                                Hierarchy<double> &);
-template void mg_vcycle<float>(Hierarchy<float> &, const MGParams &, float *, const float *, hipStream_t);
-template void mg_vcycle<double>(Hierarchy<double> &, const MGParams &, double *, const double *, hipStream_t);
+template void mg_vcycle<float>(Hierarchy<float> &, const MGParams &, float *, const float *, hipStream_t, bool);
+template void mg_vcycle<double>(Hierarchy<double> &, const MGParams &, double *, const double *, hipStream_t, bool);
+template float *mg_first_target<float>(Hierarchy<float> &, const MGParams &, float *);
+template double *mg_first_target<double>(Hierarchy<double> &, const MGParams &, double *);
+template void launch_first_sweep<float>(const Hierarchy<float> &, const MGParams &, const float *, float *,
+                                        hipStream_t);
+template void launch_first_sweep<double>(const Hierarchy<double> &, const MGParams &, const double *, double *,
+                                         hipStream_t);
+template void launch_d2t_first<float, float>(const Hierarchy<float> &, const MGParams &, const float *,
+                                             const double *, float *, float *, hipStream_t);
+template void launch_d2t_first<double, double>(const Hierarchy<double> &, const MGParams &, const double *,
+                                               const double *, double *, double *, hipStream_t);
 template void mg_fine_sweep<float>(Hierarchy<float> &, const MGParams &, const float *, const float *, float *,
                                    hipStream_t);
 template void mg_fine_sweep<double>(Hierarchy<double> &, const MGParams &, const double *, const double *, double *,
@@ -1310,24 +1610,9 @@ template void launch_fill<double>(double *, int64_t, double, hipStream_t);
 // ---------------------------------------------------------------------------
 // Vector conversions at the Krylov / multigrid boundary
 // ---------------------------------------------------------------------------
-template <typename TD, typename TS = double>
-__global__ void __launch_bounds__(256) k_d2t(const TS *__restrict__ src, const double *__restrict__ scale,
-                                             TD *__restrict__ dst, int64_t n) {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < n) dst[p] = (TD)((double)src[p] * scale[p]);
-}
 __global__ void __launch_bounds__(256) k_cvt(const double *__restrict__ src, float *__restrict__ dst, int64_t n) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p < n) dst[p] = (float)src[p];
-}
-void launch_d2t(const double *src, const double *scale, float *dst, int64_t n, hipStream_t s) {
-    hipLaunchKernelGGL(k_d2t<float>, g1(n), dim3(256), 0, s, src, scale, dst, n);
-}
-void launch_d2t(const double *src, const double *scale, double *dst, int64_t n, hipStream_t s) {
-    hipLaunchKernelGGL(k_d2t<double>, g1(n), dim3(256), 0, s, src, scale, dst, n);
-}
-void launch_d2t(const float *src, const double *scale, float *dst, int64_t n, hipStream_t s) {
-    hipLaunchKernelGGL((k_d2t<float, float>), g1(n), dim3(256), 0, s, src, scale, dst, n);
 }
 void launch_cvt(const double *src, float *dst, int64_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_cvt, g1(n), dim3(256), 0, s, src, dst, n);

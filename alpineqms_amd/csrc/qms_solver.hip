@@ -279,6 +279,7 @@ MGParams mgp(const qms_handle *h) {
     m.nu_coarse = h->cfg.mg_nu_coarse > 0 ? h->cfg.mg_nu_coarse : h->cfg.mg_nu;
     m.omega = h->cfg.mg_omega;
     m.alpha = h->cfg.mg_alpha;
+    m.tail = h->cfg.mg_fused_tail;
     return m;
 }
 
@@ -370,18 +371,18 @@ void precond(qms_handle *h, const OpParams &op, int tr, const double *D, const T
     PrecondState<TA> &ps = pstate<TA>(h);
     const int64_t n1 = h->g.n1;
     const MGParams mp = mgp(h);
-    launch_d2t(v, D, ps.rT, 3 * n1, h->s);
-    mg_vcycle<TA>(ps.hA[tr], mp, z, ps.rT, h->s);
+    // each block's level-0 first sweep from zero is fused into (A) or
+    // follows (V) the kernel that forms its right-hand side
+    launch_d2t_first(ps.hA[tr], mp, v, D, ps.rT, mg_first_target(ps.hA[tr], mp, z), h->s);
+    mg_vcycle<TA>(ps.hA[tr], mp, z, ps.rT, h->s, true);
     if constexpr (std::is_same<TA, double>::value)
         launch_vrhs(h->g, op_coeffs(h), op, z, v, ps.rV, h->s);
     else
         launch_vrhs_seg(h->g, op_coeffs(h), op, z, (const double *)((const char *)v + offv), ps.rV, h->s);
-    if constexpr (std::is_same<TA, double>::value) {
-        mg_vcycle<double>(ps.hV, mp, z + 3 * n1, ps.rV, h->s);
-    } else {
-        mg_vcycle<double>(ps.hV, mp, ps.zV, ps.rV, h->s);
-        launch_cvt(ps.zV, z + 3 * n1, n1, h->s);
-    }
+    double *zv = std::is_same<TA, double>::value ? (double *)(void *)(z + 3 * n1) : ps.zV;
+    launch_first_sweep(ps.hV, mp, (const double *)ps.rV, mg_first_target(ps.hV, mp, zv), h->s);
+    mg_vcycle<double>(ps.hV, mp, zv, ps.rV, h->s, true);
+    if constexpr (!std::is_same<TA, double>::value) launch_cvt(ps.zV, z + 3 * n1, n1, h->s);
 }
 
 // ||w||_2, read back on the host (one synchronisation).
@@ -626,6 +627,7 @@ int qms_config_default(qms_config *cfg) {
     cfg->inner_tol = 1e-6;
     cfg->mg_nu_coarse = 6;
     cfg->floor_tol = 0.0;
+    cfg->mg_fused_tail = 1;
     return QMS_OK;
 }
 
@@ -822,6 +824,7 @@ int64_t qms_estimate_bytes(const qms_config *cfg) {
     auto hier = [&](int nc_, int64_t s) {
         int64_t x = nx, y = ny, z = nz;
         add(s * nc_ * x * y * z); add(s * nc_ * x * y * z);     // level 0: t, r
+        add(s * nc_ * x * y * z);                               // level 0: diagonal
         int nlev = 1;
         int64_t cn = 0;
         while (nlev < MG_MAX_LEVELS) {

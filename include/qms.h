@@ -66,6 +66,9 @@ typedef struct {
                             than 20% over four iterations, or a cycle does not halve the residual)
                             is accepted when its FP64 scaled residual is at most this, and counted
                             in qms_stats.stalled_solves (<= 0: 100 x tol) */
+    int mg_fused_tail;   /* 1: the multigrid level above the coarsest (when it has at most 1024
+                            nodes) and the coarsest solve run as one single-workgroup launch per
+                            V-cycle; 0: one launch per sweep.  Same arithmetic either way. */
 } qms_config;
 
 typedef struct {
@@ -88,7 +91,7 @@ typedef struct {
 
 /* Fills defaults: FP64, tol 1e-12, restart 30, 3000 iterations, nu 4 on
  * level 0 and 6 on the coarse levels, omega 0.7, alpha 1.8, graphs on,
- * inner_tol 1e-6. */
+ * inner_tol 1e-6, fused multigrid tail on. */
 int qms_config_default(qms_config *cfg);
 
 /* Setup and material coefficients: replaces QMS_3D.f90:172-269 (sizes,

@@ -77,6 +77,33 @@ def test_solve_matches_direct(cpu_lib, precision):
     assert "ok" in out
 
 
+def test_fused_multigrid_tail_matches_launches(cpu_lib):
+    """The V-cycle's last two levels in one single-workgroup launch
+    (mg_fused_tail, 1024 fibers with the tail's LDS) against one launch per
+    sweep, both precisions: bit-identical fields after the steady state and
+    two ramp steps."""
+    out = run_child(cpu_lib, """
+        from alpineqms_amd import configs, program
+        from tests.test_gpu_parity import solver_for
+        d = configs.small_box_design(10, 3, nbox=2, v=3.0, ramp=True, gradient=True, steps=2)
+        for prec in (native.QMS_FP64, native.QMS_MIXED):
+            runs = []
+            for tail in (True, False):
+                states = []
+                with solver_for(d, precision=prec, mg_fused_tail=tail) as s:
+                    s.steady(program.steady_velocity(d.inputs.traj, d.inputs.dt))
+                    states.append(s.get_fields())
+                    for n in (1, 2):
+                        s.step(program.step_velocity(d.inputs.traj, d.inputs.dt, n))
+                        states.append(s.get_fields())
+                runs.append(states)
+            for n, (a, b) in enumerate(zip(*runs)):
+                assert np.array_equal(a, b), (prec, n, float(np.max(np.abs(a - b))))
+        print("ok")
+    """)
+    assert "ok" in out
+
+
 def test_async_runs_and_served_mix_match_runs_alone(cpu_lib):
     """The asynchronous run and the serving loop with three designs in flight
     on the CPU build (library threads, one stream per handle): each design's

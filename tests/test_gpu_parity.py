@@ -282,3 +282,25 @@ def test_graph_replays_match_launches(precision):
         runs.append(states)
     for n, (a, b) in enumerate(zip(*runs)):
         assert np.array_equal(a, b), n
+
+
+@pytest.mark.parametrize("precision", [native.QMS_FP64, native.QMS_MIXED], ids=["fp64", "mixed"])
+@pytest.mark.parametrize("cells", [(15, 15, 15), (20, 12, 27)], ids=["N16", "ragged"])
+def test_fused_multigrid_tail_matches_launches(precision, cells):
+    """The V-cycle's last two levels (the 8^3 and 4^3 levels of a 16^3 design;
+    6x4x7 and 3x2x4 of the ragged one) in one single-workgroup launch
+    (mg_fused_tail) against one launch per sweep: the same arithmetic, so
+    bit-identical states after the steady state and every step."""
+    d = configs.small_box_design(0, 5, nbox=3, v=4.0, ramp=True, gradient=True, steps=3, cells=cells)
+    runs = []
+    for tail in (True, False):
+        states = []
+        with solver_for(d, precision=precision, mg_fused_tail=tail) as s:
+            s.steady(program.steady_velocity(d.inputs.traj, d.inputs.dt))
+            states.append(s.get_fields())
+            for n in range(1, 4):
+                s.step(program.step_velocity(d.inputs.traj, d.inputs.dt, n))
+                states.append(s.get_fields())
+        runs.append(states)
+    for n, (a, b) in enumerate(zip(*runs)):
+        assert np.array_equal(a, b), (n, float(np.max(np.abs(a - b))))
