@@ -826,7 +826,7 @@ __global__ void __launch_bounds__(MG_COARSE_THREADS) k_mg_coarse(int n, int64_t 
 
 // ---------------------------------------------------------------------------
 // The tail of the V-cycle in one workgroup: the level above the coarsest
-// (at most mg_tail_threads nodes, one thread per node, its rows held in
+// (at most MG_TAIL_THREADS nodes, one thread per node, its rows held in
 // registers and its vectors in LDS) and the coarsest solve, in one launch
 // instead of 2 nu + 4.  Input: the residual rf of the level below (fd);
 // output: xout, the tail level's x after its post-smoothing.  The steps and
@@ -838,14 +838,14 @@ __global__ void __launch_bounds__(MG_COARSE_THREADS) k_mg_coarse(int n, int64_t 
 constexpr int MG_TAIL_LDS = 49152;
 template <typename T>
 constexpr int mg_tail_rows() { return MG_TAIL_LDS / (4 * (int)sizeof(T)); }
-// Threads (= nodes at most) of the tail: the three FP64 rows of an A-block
-// node take about 90 registers, so that case runs 512 threads, which leaves
-// 256 registers per lane and no spills.
-template <typename T, int NC>
-constexpr int mg_tail_threads() { return (sizeof(T) == 8 && NC == 3) ? 512 : 1024; }
+// Threads of the tail, one per node: the level above the coarsest has at
+// most 8 nodes per axis (the coarsest has at most 4), so at most 512 nodes.
+// 512 threads leave 256 registers per lane for the rows held in registers
+// (about 90 for the three FP64 rows of an A-block node).
+constexpr int MG_TAIL_THREADS = 512;
 
 template <typename T, typename P, int NC>
-__global__ void __launch_bounds__((mg_tail_threads<T, NC>())) k_mg_tail(P pv, LevelDims fd, const T *__restrict__ rf,
+__global__ void __launch_bounds__(MG_TAIL_THREADS) k_mg_tail(P pv, LevelDims fd, const T *__restrict__ rf,
                                                              LevelDims cd, int n, const int *__restrict__ idx,
                                                              const double *__restrict__ invT, T *__restrict__ xout,
                                                              T omega, T alpha, int nu) {
@@ -874,10 +874,20 @@ __global__ void __launch_bounds__((mg_tail_threads<T, NC>())) k_mg_tail(P pv, Le
             act[2] = pv.template row_all<2>(i, j, k, rw[2]);
         }
     }
+    // LDS index of the node's 27 neighbours (offsets -1..1 per axis), each
+    // clamped into the box, computed once: the rows only ever read these
+    int nb[27];
+#pragma unroll
+    for (int o = 0; o < 27; ++o) {
+        const int ni = max(0, min(i + o % 3 - 1, d.nx - 1)), nj = max(0, min(j + (o / 3) % 3 - 1, d.ny - 1)),
+                  nk = max(0, min(k + o / 9 - 1, d.nz - 1));
+        nb[o] = (int)lidx(d, ni, nj, nk);
+    }
+    // X(c, ni, nj, nk) of the rows: (ni - i, nj - j, nk - k) are compile-time
+    // offsets once the row code is inlined
     auto Xof = [&](const T *xs) {
-        return [=, &d](int c, int ni, int nj, int nk) {
-            ni = max(0, min(ni, d.nx - 1)); nj = max(0, min(nj, d.ny - 1)); nk = max(0, min(nk, d.nz - 1));
-            return xs[c * d.n1 + lidx(d, ni, nj, nk)];
+        return [=, &nb, &d](int c, int ni, int nj, int nk) {
+            return xs[c * d.n1 + nb[(ni - i + 1) + 3 * (nj - j + 1) + 9 * (nk - k + 1)]];
         };
     };
     // one sweep (MODE 0) or the residual (MODE 2) of every row of the node
@@ -1477,8 +1487,7 @@ template <typename T>
 static bool tail_fits(const Hierarchy<T> &h, const MGParams &mp, int l) {
     if (!mp.tail || l < 1 || l != h.nlev - 2) return false;
     const MgLevel<T> &L = h.lev[l], &K = h.lev[l + 1];
-    const int threads = L.ncomp == 3 ? mg_tail_threads<T, 3>() : mg_tail_threads<T, 1>();
-    return L.d.n1 <= threads && (int64_t)L.ncomp * L.d.n1 <= mg_tail_rows<T>() &&
+    return L.d.n1 <= MG_TAIL_THREADS && (int64_t)L.ncomp * L.d.n1 <= mg_tail_rows<T>() &&
            (int64_t)K.ncomp * K.d.n1 <= MG_COARSE_MAX && h.ncoarse <= MG_COARSE_MAX;
 }
 
@@ -1520,7 +1529,7 @@ static void vcycle_rec(Hierarchy<T> &h, const MGParams &mp, int l, T *x, const T
             using P = decltype(pv);
             constexpr int NC = decltype(ncc)::value;
             if constexpr (is_coarse_prov<P>::value)
-                hipLaunchKernelGGL((k_mg_tail<T, P, NC>), dim3(1), dim3(mg_tail_threads<T, NC>()), 0, s, pv, L.d,
+                hipLaunchKernelGGL((k_mg_tail<T, P, NC>), dim3(1), dim3(MG_TAIL_THREADS), 0, s, pv, L.d,
                                    (const T *)L.r, K.d, h.ncoarse, (const int *)h.coarse_idx,
                                    (const double *)h.coarse_inv, C.x, w, (T)mp.alpha, level_nu(mp, l + 1));
         });

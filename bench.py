@@ -309,6 +309,7 @@ def main():
     for name in runs:
         r = runs[name]
         r["stalled_solves"] = int(solvers[name].stats().stalled_solves)
+        r["device_bytes"] = int(solvers[name].stats().device_bytes)
         r["total_s"] = sum(r["step_ms"]) / 1e3
         r["ms_per_step"] = sum(r["step_ms"]) / len(r["step_ms"])
         r["steps_per_s"] = 1e3 / r["ms_per_step"]

@@ -66,9 +66,9 @@ typedef struct {
                             than 20% over four iterations, or a cycle does not halve the residual)
                             is accepted when its FP64 scaled residual is at most this, and counted
                             in qms_stats.stalled_solves (<= 0: 100 x tol) */
-    int mg_fused_tail;   /* 1: the multigrid level above the coarsest (when it has at most 1024
-                            nodes) and the coarsest solve run as one single-workgroup launch per
-                            V-cycle; 0: one launch per sweep.  Same arithmetic either way. */
+    int mg_fused_tail;   /* 1: the multigrid level above the coarsest (at most 8 nodes per axis)
+                            and the coarsest solve run as one single-workgroup launch per
+                            V-cycle; 0: one launch per sweep.  Same results bit for bit. */
 } qms_config;
 
 typedef struct {
