@@ -273,13 +273,18 @@ OpParams make_op(const qms_handle *h, const double v[3], double dt, int tr) {
     return op;
 }
 
-MGParams mgp(const qms_handle *h) {
+// Multigrid parameters of the A block (block 0) and of the V block (1).
+MGParams mgp(const qms_handle *h, int block = 0) {
     MGParams m;
     m.nu = h->cfg.mg_nu;
     m.nu_coarse = h->cfg.mg_nu_coarse > 0 ? h->cfg.mg_nu_coarse : h->cfg.mg_nu;
     m.omega = h->cfg.mg_omega;
     m.alpha = h->cfg.mg_alpha;
     m.tail = h->cfg.mg_fused_tail;
+    if (block == 1) {
+        if (h->cfg.mg_nu_v > 0) m.nu = h->cfg.mg_nu_v;
+        if (h->cfg.mg_omega_v > 0) m.omega = h->cfg.mg_omega_v;
+    }
     return m;
 }
 
@@ -380,8 +385,9 @@ void precond(qms_handle *h, const OpParams &op, int tr, const double *D, const T
     else
         launch_vrhs_seg(h->g, op_coeffs(h), op, z, (const double *)((const char *)v + offv), ps.rV, h->s);
     double *zv = std::is_same<TA, double>::value ? (double *)(void *)(z + 3 * n1) : ps.zV;
-    launch_first_sweep(ps.hV, mp, (const double *)ps.rV, mg_first_target(ps.hV, mp, zv), h->s);
-    mg_vcycle<double>(ps.hV, mp, zv, ps.rV, h->s, true);
+    const MGParams mpv = mgp(h, 1);
+    launch_first_sweep(ps.hV, mpv, (const double *)ps.rV, mg_first_target(ps.hV, mpv, zv), h->s);
+    mg_vcycle<double>(ps.hV, mpv, zv, ps.rV, h->s, true);
     if constexpr (!std::is_same<TA, double>::value) launch_cvt(ps.zV, z + 3 * n1, n1, h->s);
 }
 
@@ -625,9 +631,11 @@ int qms_config_default(qms_config *cfg) {
     cfg->mg_alpha = 1.8;
     cfg->use_graphs = 1;
     cfg->inner_tol = 1e-6;
-    cfg->mg_nu_coarse = 6;
+    cfg->mg_nu_coarse = 4;
     cfg->floor_tol = 0.0;
     cfg->mg_fused_tail = 1;
+    cfg->mg_omega_v = 0.85;
+    cfg->mg_nu_v = 6;
     return QMS_OK;
 }
 
@@ -645,7 +653,8 @@ int qms_create(const qms_config *cfg, const uint16_t *cell_slot, const double *s
         }
         if (!(cfg->h[a] > 0)) { g_last_error = "qms_create: grid steps must be positive"; return QMS_ERR_ARG; }
     }
-    if (cfg->restart < 1 || cfg->restart > 64 || cfg->mg_nu < 1 || cfg->mg_nu_coarse > 64 || !(cfg->tol > 0) ||
+    if (cfg->restart < 1 || cfg->restart > 64 || cfg->mg_nu < 1 || cfg->mg_nu > 64 || cfg->mg_nu_coarse > 64 ||
+        cfg->mg_nu_v > 64 || !(cfg->mg_omega_v <= 1.0) || !(cfg->tol > 0) ||
         cfg->max_iters < 1 ||
         (cfg->precision != QMS_FP64 && cfg->precision != QMS_MIXED)) {
         g_last_error = "qms_create: invalid solver parameters";
@@ -1008,7 +1017,7 @@ int qms_time_kernel(qms_handle *h, int which, const double v[3], int reps, doubl
             auto part = [&](auto &ps, auto tag) {
                 using T = decltype(tag);
                 if (which == 8) mg_vcycle<T>(ps.hA[tr], mp, ps.Z[0], ps.rT, h->s);
-                else if (which == 9) mg_vcycle<double>(ps.hV, mp, ps.zV, ps.rV, h->s);
+                else if (which == 9) mg_vcycle<double>(ps.hV, mgp(h, 1), ps.zV, ps.rV, h->s);
                 else fgmres_iteration<T>(h, opd, tr, D, j);
             };
             if (mixed) part(h->p32, 0.0f);
@@ -1017,7 +1026,7 @@ int qms_time_kernel(qms_handle *h, int which, const double v[3], int reps, doubl
             const MGParams mp = mgp(h);
             auto sweep = [&](auto &ps) {
                 if (which == 1) mg_fine_sweep(ps.hA[tr], mp, ps.Z[0], ps.rT, ps.hA[tr].lev[0].t, h->s);
-                else mg_fine_sweep(ps.hV, mp, ps.zV, ps.rV, ps.hV.lev[0].t, h->s);
+                else mg_fine_sweep(ps.hV, mgp(h, 1), ps.zV, ps.rV, ps.hV.lev[0].t, h->s);
             };
             if (mixed) sweep(h->p32);
             else sweep(h->p64);

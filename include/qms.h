@@ -53,8 +53,9 @@ typedef struct {
     double tol;          /* ||D^-1 (b - M x)|| <= tol ||D^-1 b||, D = diag(M) */
     int restart;         /* FGMRES restart length */
     int max_iters;       /* Krylov iteration cap per solve */
-    int mg_nu;           /* Jacobi sweeps before and after each coarse correction */
-    double mg_omega;     /* Jacobi damping */
+    int mg_nu;           /* Jacobi sweeps before and after the coarse correction on level 0 of
+                            the A block */
+    double mg_omega;     /* Jacobi damping of the A block */
     double mg_alpha;     /* coarse-correction scaling */
     int use_graphs;      /* 1: each FGMRES iteration replays a captured hipGraph (default);
                             0: its kernels are launched one by one (same results bit for bit) */
@@ -69,6 +70,9 @@ typedef struct {
     int mg_fused_tail;   /* 1: the multigrid level above the coarsest (at most 8 nodes per axis)
                             and the coarsest solve run as one single-workgroup launch per
                             V-cycle; 0: one launch per sweep.  Same results bit for bit. */
+    double mg_omega_v;   /* Jacobi damping of the V block (<= 0: mg_omega) */
+    int mg_nu_v;         /* Jacobi sweeps before and after the coarse correction on level 0 of
+                            the V block (<= 0: mg_nu) */
 } qms_config;
 
 typedef struct {
@@ -89,9 +93,10 @@ typedef struct {
                                 above it fail the solve with QMS_ERR_NOCONV) */
 } qms_stats;
 
-/* Fills defaults: FP64, tol 1e-12, restart 30, 3000 iterations, nu 4 on
- * level 0 and 6 on the coarse levels, omega 0.7, alpha 1.8, graphs on,
- * inner_tol 1e-6, fused multigrid tail on. */
+/* Fills defaults: FP64, tol 1e-12, restart 30, 3000 iterations; multigrid:
+ * on level 0 nu 4 (A block) and 6 (V block), 4 on the coarse levels, omega
+ * 0.7 (A block) and 0.85 (V block), alpha 1.8, fused tail on; graphs on,
+ * inner_tol 1e-6. */
 int qms_config_default(qms_config *cfg);
 
 /* Setup and material coefficients: replaces QMS_3D.f90:172-269 (sizes,

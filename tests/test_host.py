@@ -36,6 +36,20 @@ def test_config_defaults_without_gpu():
     c = native.default_config()
     assert (c.precision, c.restart, c.mg_nu) == (native.QMS_FP64, 30, 4)
     assert c.tol == 1e-12 and c.mg_omega == 0.7 and c.mg_alpha == 1.8
+    assert (c.mg_nu_coarse, c.mg_nu_v, c.mg_omega_v, c.mg_fused_tail) == (4, 6, 0.85, 1)
+
+
+def test_python_config_defaults_match_the_library():
+    """make_config (which every Solver goes through) starts from the C
+    defaults and sets every field: its own defaults must be the library's."""
+    if not os.path.exists(native.LIB_PATH):
+        pytest.skip("libqms.so not built")
+    c = native.default_config()
+    m = native.make_config((8, 8, 8), (1.0, 1.0, 1.0), 1.0, np.zeros(12))
+    for f, _ in native.Config._fields_:
+        if f in ("cells", "h", "dt", "bc"):
+            continue
+        assert getattr(c, f) == getattr(m, f), f
 
 
 @pytest.mark.parametrize("cells,h,restart", [((4, 5, 5), 5e-4, 30), ((5, 5, 5), 0.0, 30), ((5, 5, 5), 5e-4, 0),
