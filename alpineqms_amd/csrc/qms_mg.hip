@@ -38,6 +38,11 @@
 #include <vector>
 #include "qms_kernels.h"
 
+// Multiply-adds are fused only within one source expression, never across
+// statements: the same row code then rounds identically in every kernel that
+// inlines it (the one-workgroup tail is bit for bit its launch-per-step form).
+#pragma clang fp contract(on)
+
 namespace qms {
 
 // ---------------------------------------------------------------------------
