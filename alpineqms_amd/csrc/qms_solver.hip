@@ -1040,8 +1040,11 @@ int qms_time_kernel(qms_handle *h, int which, const double v[3], int reps, doubl
         // exceeds the L2s and the 256 MB Infinity Cache together.  Launches
         // run back to back and are timed as a whole, like the warm figure.
         // (The Gram-Schmidt step streams 4 GB and is cold as it is; the
-        // smoother sweeps are timed as they are.)
-        const int64_t per = 16 * n;                              // x read, w written, bytes
+        // smoother sweeps are timed as they are.)  Arrays every set shares
+        // -- the coefficient classes and table of the operator (4 MB at
+        // 128^3), the three sigma arrays the RHS reads (50 MB) -- are read
+        // by every launch and stay cached: those parts are warm.
+        const int64_t per = (which == 6 ? 24 : 16) * n;          // x (and b) read, the output written, bytes
         const int nsets = (int)std::min<int64_t>(64, 1 + ((int64_t)640 << 20) / per + 1);
         struct Sets {
             hipStream_t s;
