@@ -935,9 +935,9 @@ __global__ void __launch_bounds__(MG_TAIL_THREADS) k_mg_tail(P pv, LevelDims fd,
     }
     pass(M2(), Xof(cur), rl);
     // restriction to the coarsest level
-    if (p < cd.n1) {
+    for (int pc = p; pc < cd.n1; pc += (int)blockDim.x) {
         int ci, cj, ck;
-        node_of(cd, p, ci, cj, ck);
+        node_of(cd, pc, ci, cj, ck);
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
             T s = (T)0;
@@ -946,7 +946,7 @@ __global__ void __launch_bounds__(MG_TAIL_THREADS) k_mg_tail(P pv, LevelDims fd,
                 const int fi = 2 * ci + (ch & 1), fj = 2 * cj + ((ch >> 1) & 1), fk = 2 * ck + (ch >> 2);
                 if (inside(d, fi, fj, fk)) s += rl[c * d.n1 + lidx(d, fi, fj, fk)];
             }
-            bcs[c * cd.n1 + p] = s;
+            bcs[c * cd.n1 + pc] = s;
         }
     }
     __syncthreads();
