@@ -284,6 +284,7 @@ MGParams mgp(const qms_handle *h, int block = 0) {
     if (block == 1) {
         if (h->cfg.mg_nu_v > 0) m.nu = h->cfg.mg_nu_v;
         if (h->cfg.mg_omega_v > 0) m.omega = h->cfg.mg_omega_v;
+        if (h->cfg.mg_alpha_v > 0) m.alpha = h->cfg.mg_alpha_v;
     }
     return m;
 }
@@ -628,7 +629,7 @@ int qms_config_default(qms_config *cfg) {
     cfg->max_iters = 3000;
     cfg->mg_nu = 4;
     cfg->mg_omega = 0.7;
-    cfg->mg_alpha = 1.8;
+    cfg->mg_alpha = 1.5;
     cfg->use_graphs = 1;
     cfg->inner_tol = 1e-6;
     cfg->mg_nu_coarse = 4;
@@ -636,6 +637,7 @@ int qms_config_default(qms_config *cfg) {
     cfg->mg_fused_tail = 1;
     cfg->mg_omega_v = 0.85;
     cfg->mg_nu_v = 6;
+    cfg->mg_alpha_v = 1.8;
     return QMS_OK;
 }
 

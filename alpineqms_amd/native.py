@@ -71,6 +71,7 @@ class Config(ctypes.Structure):
         ("mg_fused_tail", ctypes.c_int),
         ("mg_omega_v", ctypes.c_double),
         ("mg_nu_v", ctypes.c_int),
+        ("mg_alpha_v", ctypes.c_double),
     ]
 
 
@@ -172,9 +173,10 @@ def default_config() -> Config:
 
 def make_config(cells: Sequence[int], h: Sequence[float], dt: float, bc: Sequence[float],
                 precision: int = QMS_FP64, tol: float = 1e-12, restart: int = 30, max_iters: int = 3000,
-                mg_nu: int = 4, mg_omega: float = 0.7, mg_alpha: float = 1.8, use_graphs: bool = True,
+                mg_nu: int = 4, mg_omega: float = 0.7, mg_alpha: float = 1.5, use_graphs: bool = True,
                 inner_tol: float = 1e-6, mg_nu_coarse: int = 4, floor_tol: float = 0.0,
-                mg_fused_tail: bool = True, mg_omega_v: float = 0.85, mg_nu_v: int = 6) -> Config:
+                mg_fused_tail: bool = True, mg_omega_v: float = 0.85, mg_nu_v: int = 6,
+                mg_alpha_v: float = 1.8) -> Config:
     cfg = default_config()
     for a in range(3):
         cfg.cells[a] = int(cells[a])
@@ -199,6 +201,7 @@ def make_config(cells: Sequence[int], h: Sequence[float], dt: float, bc: Sequenc
     cfg.mg_fused_tail = 1 if mg_fused_tail else 0
     cfg.mg_omega_v = float(mg_omega_v)
     cfg.mg_nu_v = int(mg_nu_v)
+    cfg.mg_alpha_v = float(mg_alpha_v)
     return cfg
 
 
@@ -231,12 +234,13 @@ class Solver:
     def __init__(self, cells: Sequence[int], h: Sequence[float], dt: float, bc: Sequence[float],
                  cell_slot: np.ndarray, slot_props: np.ndarray, precision: int = QMS_FP64,
                  tol: float = 1e-12, restart: int = 30, max_iters: int = 3000, mg_nu: int = 4,
-                 mg_omega: float = 0.7, mg_alpha: float = 1.8, use_graphs: bool = True, inner_tol: float = 1e-6,
+                 mg_omega: float = 0.7, mg_alpha: float = 1.5, use_graphs: bool = True, inner_tol: float = 1e-6,
                  mg_nu_coarse: int = 4, floor_tol: float = 0.0, mg_fused_tail: bool = True,
-                 mg_omega_v: float = 0.85, mg_nu_v: int = 6):
+                 mg_omega_v: float = 0.85, mg_nu_v: int = 6, mg_alpha_v: float = 1.8):
         L = load()
         cfg = make_config(cells, h, dt, bc, precision, tol, restart, max_iters, mg_nu, mg_omega, mg_alpha,
-                          use_graphs, inner_tol, mg_nu_coarse, floor_tol, mg_fused_tail, mg_omega_v, mg_nu_v)
+                          use_graphs, inner_tol, mg_nu_coarse, floor_tol, mg_fused_tail, mg_omega_v, mg_nu_v,
+                          mg_alpha_v)
         self.cells = tuple(int(c) for c in cells)
         self.nodes = tuple(c + 1 for c in self.cells)
         self.dt = float(dt)

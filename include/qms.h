@@ -56,7 +56,7 @@ typedef struct {
     int mg_nu;           /* Jacobi sweeps before and after the coarse correction on level 0 of
                             the A block */
     double mg_omega;     /* Jacobi damping of the A block */
-    double mg_alpha;     /* coarse-correction scaling */
+    double mg_alpha;     /* coarse-correction scaling of the A block */
     int use_graphs;      /* 1: each FGMRES iteration replays a captured hipGraph (default);
                             0: its kernels are launched one by one (same results bit for bit) */
     double inner_tol;    /* QMS_MIXED: each FP32 inner FGMRES cycle stops once it has reduced the
@@ -73,6 +73,7 @@ typedef struct {
     double mg_omega_v;   /* Jacobi damping of the V block (<= 0: mg_omega) */
     int mg_nu_v;         /* Jacobi sweeps before and after the coarse correction on level 0 of
                             the V block (<= 0: mg_nu) */
+    double mg_alpha_v;   /* coarse-correction scaling of the V block (<= 0: mg_alpha) */
 } qms_config;
 
 typedef struct {
@@ -95,8 +96,8 @@ typedef struct {
 
 /* Fills defaults: FP64, tol 1e-12, restart 30, 3000 iterations; multigrid:
  * on level 0 nu 4 (A block) and 6 (V block), 4 on the coarse levels, omega
- * 0.7 (A block) and 0.85 (V block), alpha 1.8, fused tail on; graphs on,
- * inner_tol 1e-6. */
+ * 0.7 (A block) and 0.85 (V block), alpha 1.5 (A block) and 1.8 (V block),
+ * fused tail on; graphs on, inner_tol 1e-6. */
 int qms_config_default(qms_config *cfg);
 
 /* Setup and material coefficients: replaces QMS_3D.f90:172-269 (sizes,
