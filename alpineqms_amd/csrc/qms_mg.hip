@@ -560,6 +560,8 @@ struct A0Z {
     const uint16_t *cls;
     const double *ctab;
     FineConst<T> fk;
+    const uint16_t *rcls;      // V block: row class per node (null: rows from the coefficient tuples)
+    const T *rtab;             // and the row table (MG_CLS_RS_V values per class)
 };
 
 // Coefficient tuples of a node and of its -x, -y and -z neighbours.
@@ -637,9 +639,10 @@ __device__ __forceinline__ void a0_row(const Grid &g, const FineConst<T> &fk, co
 // Level 0 of either block (NC = 3: A block, NC = 1: V block) on the z-march.
 // MODE 0: damped Jacobi sweep out = x + w (b - A x)/diag; MODE 1: residual
 // out = b - A x.  Inactive rows get 0.
-template <typename T, int NC, int MODE, bool CLS, bool PRO>
+template <typename T, int NC, int MODE, bool CLS, bool PRO, bool RC = false>
 __global__ void __launch_bounds__(256) k_a0_zm(A0Z<T> z, const T *__restrict__ x, const T *__restrict__ b,
                                                T *__restrict__ out, T omega, Prolong<T> pr) {
+    static_assert(!RC || NC == 1, "row classes: V block only");
     __shared__ T lds[NC * 3 * A0_PL];
     Grid g = z.g;
     g.nx = __builtin_amdgcn_readfirstlane(z.g.nx);
@@ -673,14 +676,19 @@ __global__ void __launch_bounds__(256) k_a0_zm(A0Z<T> z, const T *__restrict__ x
     const int dxm = ic > 0 ? -1 : 0;
     const int64_t dym = jc > 0 ? -(int64_t)g.nx : 0;
     ZmTuple<T> cf;
-    load_tuple<T, CLS>(z, pc + plane * max(kb - 1, 0), cf.imz, cf.sgz);
+    if constexpr (!RC) load_tuple<T, CLS>(z, pc + plane * max(kb - 1, 0), cf.imz, cf.sgz);
     for (int kk = kb; kk < ke; ++kk) {
         const int r = kk - kb;
         const int sm = r % 3, s0 = (r + 1) % 3, sp = (r + 2) % 3;
         const int64_t p = pc + plane * kk;
-        load_tuple<T, CLS>(z, p, cf.im0, cf.sg0);
-        load_tuple<T, CLS>(z, p + dxm, cf.imx, cf.sgx);
-        load_tuple<T, CLS>(z, p + dym, cf.imy, cf.sgy);
+        const T *rt = nullptr;
+        if constexpr (RC) {
+            rt = z.rtab + MG_CLS_RS_V * (int)z.rcls[p];
+        } else {
+            load_tuple<T, CLS>(z, p, cf.im0, cf.sg0);
+            load_tuple<T, CLS>(z, p + dxm, cf.imx, cf.sgx);
+            load_tuple<T, CLS>(z, p + dym, cf.imy, cf.sgy);
+        }
         T bq[NC];
 #pragma unroll
         for (int c = 0; c < NC; ++c) bq[c] = live ? b[c * n1 + p] : (T)0;
@@ -688,14 +696,29 @@ __global__ void __launch_bounds__(256) k_a0_zm(A0Z<T> z, const T *__restrict__ x
         __syncthreads();
         if (live) {
             const A0View<T> xv{lds, {sm * A0_PL, s0 * A0_PL, sp * A0_PL}, ctr};
-            a0_row<0, NC, MODE, T>(g, z.fk, cf, xv, i, j, kk, p, bq, out, omega);
-            if constexpr (NC == 3) {
-                a0_row<1, NC, MODE, T>(g, z.fk, cf, xv, i, j, kk, p, bq, out, omega);
-                a0_row<2, NC, MODE, T>(g, z.fk, cf, xv, i, j, kk, p, bq, out, omega);
+            if constexpr (RC) {
+                // the node's row from the table: conductances, excess
+                // diagonal, 1/diag (0: inactive row)
+                Row<T> rw;
+#pragma unroll
+                for (int q = 0; q < 6; ++q) rw.cond[q] = rt[q];
+                rw.exc = rt[6];
+                T diag, x0;
+                const T y = apply_lds<0, 1, T>(xv, rw, diag, x0);
+                const T v = MODE == 0 ? x0 + omega * (bq[0] - y) / diag : bq[0] - y;
+                out[p] = rt[7] != (T)0 ? v : (T)0;
+            } else {
+                a0_row<0, NC, MODE, T>(g, z.fk, cf, xv, i, j, kk, p, bq, out, omega);
+                if constexpr (NC == 3) {
+                    a0_row<1, NC, MODE, T>(g, z.fk, cf, xv, i, j, kk, p, bq, out, omega);
+                    a0_row<2, NC, MODE, T>(g, z.fk, cf, xv, i, j, kk, p, bq, out, omega);
+                }
             }
         }
+        if constexpr (!RC) {
 #pragma unroll
-        for (int a = 0; a < 3; ++a) { cf.imz[a] = cf.im0[a]; cf.sgz[a] = cf.sg0[a]; }
+            for (int a = 0; a < 3; ++a) { cf.imz[a] = cf.im0[a]; cf.sgz[a] = cf.sg0[a]; }
+        }
         __syncthreads();
     }
 }
@@ -1251,6 +1274,40 @@ void mg_setup(const Grid &g, const HostCoef &hc, const CoeffsT<T> &dc, double dt
     h.lev[0].t = (T *)al.alloc(sizeof(T) * (size_t)(nc * g.n1));
     h.lev[0].r = (T *)al.alloc(sizeof(T) * (size_t)(nc * g.n1));
     h.lev[0].dg = (T *)al.alloc(sizeof(T) * (size_t)(nc * g.n1));
+    if (block == 1) {
+        // V block: level 0's rows as a 2-byte class per node and a table of
+        // the distinct rows (built in FP64 from the same row formulas), so
+        // that a sweep reads one class and one row per node instead of four
+        // coefficient tuples and forming the row
+        HostLevel H0;
+        H0.d = d0;
+        H0.nc = 1;
+        H0.cond.assign((size_t)(6 * g.n1), 0.0);
+        H0.exc.assign((size_t)g.n1, 0.0);
+        H0.dinv.assign((size_t)g.n1, 0.0);
+        Row<double> rw;
+        for (int k = 0; k < g.nz; ++k)
+            for (int j = 0; j < g.ny; ++j)
+                for (int i = 0; i < g.nx; ++i) {
+                    if (!host_fine_row<0, 1>(g, hc, fk, i, j, k, rw)) continue;
+                    const int64_t p = gidx(g, i, j, k);
+                    double dg = rw.exc;
+                    for (int q = 0; q < 6; ++q) { H0.cond[(size_t)(q * g.n1 + p)] = rw.cond[q]; dg += rw.cond[q]; }
+                    H0.exc[(size_t)p] = rw.exc;
+                    H0.dinv[(size_t)p] = 1.0 / dg;
+                }
+        std::vector<uint16_t> cls0;
+        std::vector<double> tab0;
+        if (class_compress(H0, cls0, tab0)) {
+            std::vector<T> stage0;
+            h.lev[0].ncls = (int)(tab0.size() / MG_CLS_RS_V);
+            h.lev[0].tab = upload<T>(tab0, stage0, al);
+            h.lev[0].cls = (uint16_t *)al.alloc(sizeof(uint16_t) * cls0.size());
+            hip_check(hipMemcpyAsync(h.lev[0].cls, cls0.data(), sizeof(uint16_t) * cls0.size(), hipMemcpyHostToDevice,
+                                     al.stream), "h2d");
+            hip_check(hipStreamSynchronize(al.stream), "sync");
+        }
+    }
     on_level(h, 0, [&](auto pv, auto ncc) {
         using P = decltype(pv);
         constexpr int NC = decltype(ncc)::value;
@@ -1394,7 +1451,9 @@ template <typename T, typename F>
 static void on_level(const Hierarchy<T> &h, int l, F f) {
     using I3 = std::integral_constant<int, 3>;
     using I1 = std::integral_constant<int, 1>;
-    if (l == 0) {
+    if (l == 0 && h.block == 1 && h.lev[0].cls) {
+        f(cls_prov<T, 1>(h.lev[0]), I1());
+    } else if (l == 0) {
         if (h.cls) {
             if (h.block == 0) f(fine_prov<T, 3, true>(h), I3());
             else f(fine_prov<T, 1, true>(h), I1());
@@ -1420,6 +1479,8 @@ static A0Z<T> a0z(const Hierarchy<T> &h) {
     z.cls = h.cls;
     z.ctab = h.ctab;
     z.fk = h.fk;
+    z.rcls = h.block == 1 ? h.lev[0].cls : nullptr;
+    z.rtab = h.block == 1 ? h.lev[0].tab : nullptr;
     return z;
 }
 
@@ -1438,6 +1499,11 @@ static void a0_launch(const Hierarchy<T> &h, const T *x, const T *b, T *out, T w
     using Y = std::true_type;
     using N = std::false_type;
     const bool pro = MODE == 0 && pr.xc != nullptr;
+    if (h.block == 1 && h.lev[0].cls) {
+        if (pro) hipLaunchKernelGGL((k_a0_zm<T, 1, MODE, false, true, true>), grid, block, 0, s, a0z(h), x, b, out, w, pr);
+        else hipLaunchKernelGGL((k_a0_zm<T, 1, MODE, false, false, true>), grid, block, 0, s, a0z(h), x, b, out, w, pr);
+        return;
+    }
     if (h.block == 0) {
         if (h.cls) { if (pro) go(I3(), Y(), Y()); else go(I3(), Y(), N()); }
         else { if (pro) go(I3(), N(), Y()); else go(I3(), N(), N()); }

@@ -854,6 +854,7 @@ int64_t qms_estimate_bytes(const qms_config *cfg) {
     hier(3, ta);
     hier(3, ta);
     hier(1, 8);
+    add(2 * n1); add(8 * 8 * 8192);                             // V block: level-0 row classes and table
     return b + allocs * 256;       // allocation granularity
 }
 
