@@ -978,8 +978,29 @@ int qms_time_kernel(qms_handle *h, int which, const double v[3], int reps, doubl
     if (busy(h)) return QMS_ERR_STATE;
     const bool cold = (which & QMS_TIME_COLD) != 0;
     which &= ~QMS_TIME_COLD;
-    if (which < 0 || which > 10) return QMS_ERR_ARG;
+    if (which < 0 || which > 11) return QMS_ERR_ARG;
     QMS_TRY(h)
+    if (which == 11) {
+        // The copy kernel on two arrays of 2^28 doubles (2 GiB each), far
+        // beyond every cache: the sustained HBM rate of a plain stream, with
+        // launch and ramp a negligible share of each launch.
+        const int64_t big = (int64_t)1 << 28;
+        struct Pair {
+            hipStream_t s;
+            double *p[2] = {nullptr, nullptr};
+            ~Pair() {
+                hipStreamSynchronize(s);          // no launch may still use them (also on an error exit)
+                for (double *q : p) hipFree(q);
+            }
+        } pr{h->s};
+        for (double *&q : pr.p) check(hipMalloc(&q, sizeof(double) * big), "hipMalloc");
+        check(hipMemsetAsync(pr.p[1], 0, sizeof(double) * big, h->s), "memset");
+        auto launch = [&]() { launch_copy(pr.p[0], pr.p[1], big, h->s); };
+        launch();
+        *ms_per_launch = time_on(h->s, reps, launch);
+        *bytes_per_launch = 16.0 * (double)big;
+        return QMS_OK;
+    }
     const int tr = h->cfg.dt > 0 ? 1 : 0;
     const double v0[3] = {0, 0, 0};
     const OpParams op = make_op(h, v ? v : v0, h->cfg.dt, tr);

@@ -36,7 +36,7 @@ EXPORTS = (
 
 # qms_time_kernel selectors
 K_APPLY, K_SMOOTH_A, K_SMOOTH_V, K_COPY, K_RHS, K_HFIELD, K_RESID, K_GRAM_SCHMIDT = range(8)
-K_VCYCLE_A, K_VCYCLE_V, K_ITERATION = 8, 9, 10
+K_VCYCLE_A, K_VCYCLE_V, K_ITERATION, K_COPY_SUSTAINED = 8, 9, 10, 11
 # or-ed into a selector: the kernel timed with cold caches
 K_COLD = 256
 

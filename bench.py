@@ -338,6 +338,12 @@ def main():
                 gbs = nb / (ms * 1e-3) / 1e9
                 kernels[kname] = {"ms_per_launch": ms, "algorithmic_bytes": nb, "achieved_GBs": gbs,
                                   "frac": gbs / HBM_PEAK_GBS, "cold_caches": bool(sel & native.K_COLD)}
+            # the same copy kernel on 2 GiB arrays: the sustained rate of a plain stream
+            s64.time_kernel(native.K_COPY_SUSTAINED, 1)
+            ms, nb = s64.time_kernel(native.K_COPY_SUSTAINED, 10)
+            gbs = nb / (ms * 1e-3) / 1e9
+            kernels["copy_sustained_4GiB"] = {"ms_per_launch": ms, "algorithmic_bytes": nb, "achieved_GBs": gbs,
+                                              "frac": gbs / HBM_PEAK_GBS, "cold_caches": True}
     for s in solvers.values():
         s.close()
     head = runs[main_name]

@@ -205,7 +205,9 @@ int qms_set_colwidth(qms_handle *h, int nodes);
  * largest of a restart cycle: multi-dot, reduction, update, norm),
  * 8 one A-block V-cycle, 9 one V-block V-cycle, 10 one FGMRES iteration of
  * the handle's precision (preconditioner, operator, Gram-Schmidt against 11
- * basis vectors, Givens update); 8-10 are composite and report 0 bytes.
+ * basis vectors, Givens update); 8-10 are composite and report 0 bytes;
+ * 11 the copy kernel of 3 on two 2 GiB arrays allocated for the call (4 GiB
+ * moved per launch: the sustained HBM rate of a plain stream).
  * which | QMS_TIME_COLD: the same kernel with cold caches, as inside a solve
  * where the Krylov basis streams through the caches between two launches:
  * the launches rotate through copies of the kernel's arrays, enough that at
