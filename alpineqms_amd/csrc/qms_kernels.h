@@ -218,7 +218,18 @@ struct MGParams {
     double omega;    // Jacobi damping
     double alpha;    // coarse-correction scaling
     int tail;        // run the level above the coarsest and the coarsest solve as one launch
+    double cheb_lmax;   // > 0: Chebyshev-Richardson weights on [cheb_lmax / cheb_ratio, cheb_lmax]
+    double cheb_ratio;  //   instead of omega
 };
+
+// Weight of sweep it (0-based) of a level with nu sweeps, before (post =
+// false) or after the coarse correction: damped Jacobi's omega, or 1 /
+// theta_k with theta_k = (hi + lo) / 2 + (hi - lo) / 2 cos((2k + 1) pi / 2nu),
+// the roots of the degree-nu Chebyshev polynomial on [lo, hi], k = it before
+// the correction (largest root first) and nu - 1 - it after it.  nu sweeps
+// then apply the Chebyshev residual polynomial of D^-1 A, the one of degree
+// nu that is smallest on [lo, hi] among those equal to 1 at 0.
+double sweep_weight(const MGParams &mp, int nu, int it, bool post);
 
 // Device allocations of the hierarchy go through the owner (which frees them).
 struct MgAlloc {

@@ -871,12 +871,18 @@ constexpr int mg_tail_rows() { return MG_TAIL_LDS / (4 * (int)sizeof(T)); }
 // 512 threads leave 256 registers per lane for the rows held in registers
 // (about 90 for the three FP64 rows of an A-block node).
 constexpr int MG_TAIL_THREADS = 512;
+// Sweeps per side of the tail level that the tail takes weights for.
+constexpr int MG_TAIL_MAX_NU = 8;
+template <typename T>
+struct TailWeights {
+    T pre[MG_TAIL_MAX_NU], post[MG_TAIL_MAX_NU];   // sweep weights before and after the coarse solve
+};
 
 template <typename T, typename P, int NC>
 __global__ void __launch_bounds__(MG_TAIL_THREADS) k_mg_tail(P pv, LevelDims fd, const T *__restrict__ rf,
                                                              LevelDims cd, int n, const int *__restrict__ idx,
                                                              const double *__restrict__ invT, T *__restrict__ xout,
-                                                             T omega, T alpha, int nu) {
+                                                             TailWeights<T> wt, T alpha, int nu) {
     constexpr int R = mg_tail_rows<T>();
     static_assert(R * sizeof(T) >= sizeof(double) * MG_COARSE_THREADS && R * sizeof(T) >= sizeof(double) * MG_COARSE_MAX,
                   "the coarse solve's gathered b and partial sums live in two of the tail level's vectors");
@@ -918,8 +924,9 @@ __global__ void __launch_bounds__(MG_TAIL_THREADS) k_mg_tail(P pv, LevelDims fd,
             return xs[c * d.n1 + nb[(ni - i + 1) + 3 * (nj - j + 1) + 9 * (nk - k + 1)]];
         };
     };
-    // one sweep (MODE 0) or the residual (MODE 2) of every row of the node
-    auto pass = [&](auto mode, const auto &X, T *dst) {
+    // one sweep with weight omega (MODE 0) or the residual (MODE 2) of every
+    // row of the node
+    auto pass = [&](auto mode, const auto &X, T *dst, T omega) {
         constexpr int MODE = decltype(mode)::value;
         if (live) {
             T o[NC];
@@ -946,17 +953,17 @@ __global__ void __launch_bounds__(MG_TAIL_THREADS) k_mg_tail(P pv, LevelDims fd,
                 const int fi = 2 * i + (ch & 1), fj = 2 * j + ((ch >> 1) & 1), fk = 2 * k + (ch >> 2);
                 if (inside(fd, fi, fj, fk)) s += rf[(int64_t)c * fd.n1 + lidx(fd, fi, fj, fk)];
             }
-            const T v = omega * s / row_diag(rw[c]);
+            const T v = wt.pre[0] * s / row_diag(rw[c]);
             bl[c * d.n1 + p] = s;
             cur[c * d.n1 + p] = act[c] ? v : (T)0;
         }
     }
     __syncthreads();
     for (int it = 1; it < nu; ++it) {
-        pass(M0(), Xof(cur), oth);
+        pass(M0(), Xof(cur), oth, wt.pre[it]);
         T *t = cur; cur = oth; oth = t;
     }
-    pass(M2(), Xof(cur), rl);
+    pass(M2(), Xof(cur), rl, (T)0);
     // restriction to the coarsest level
     for (int pc = p; pc < cd.n1; pc += (int)blockDim.x) {
         int ci, cj, ck;
@@ -988,11 +995,11 @@ __global__ void __launch_bounds__(MG_TAIL_THREADS) k_mg_tail(P pv, LevelDims fd,
             ni = max(0, min(ni, d.nx - 1)); nj = max(0, min(nj, d.ny - 1)); nk = max(0, min(nk, d.nz - 1));
             return xs[c * d.n1 + lidx(d, ni, nj, nk)] + alpha * xcs[c * cd.n1 + lidx(cd, ni >> 1, nj >> 1, nk >> 1)];
         };
-        pass(M0(), XP, oth);
+        pass(M0(), XP, oth, wt.post[0]);
         T *t = cur; cur = oth; oth = t;
     }
     for (int it = 1; it < nu; ++it) {
-        pass(M0(), Xof(cur), oth);
+        pass(M0(), Xof(cur), oth, wt.post[it]);
         T *t = cur; cur = oth; oth = t;
     }
     if (live) {
@@ -1548,6 +1555,13 @@ static void resid(const Hierarchy<T> &h, int l, const T *x, const T *b, T *r, hi
             hipLaunchKernelGGL((k_mg_resid<T, P, NC>), g1(n1), dim3(256), 0, s, pv, x, b, r);
     });
 }
+double sweep_weight(const MGParams &mp, int nu, int it, bool post) {
+    if (!(mp.cheb_lmax > 0.0)) return mp.omega;
+    const double hi = mp.cheb_lmax, lo = hi / mp.cheb_ratio;
+    const int k = post ? nu - 1 - it : it;
+    return 1.0 / (0.5 * (hi + lo) + 0.5 * (hi - lo) * cos((2.0 * k + 1.0) * M_PI / (2.0 * nu)));
+}
+
 // Smoothing sweeps before and after the coarse correction on level l.
 static inline int level_nu(const MGParams &mp, int l) { return l == 0 ? mp.nu : mp.nu_coarse; }
 
@@ -1556,7 +1570,7 @@ static inline int level_nu(const MGParams &mp, int l) { return l == 0 ? mp.nu : 
 // level fit one workgroup.
 template <typename T>
 static bool tail_fits(const Hierarchy<T> &h, const MGParams &mp, int l) {
-    if (!mp.tail || l < 1 || l != h.nlev - 2) return false;
+    if (!mp.tail || l < 1 || l != h.nlev - 2 || level_nu(mp, l) > MG_TAIL_MAX_NU) return false;
     const MgLevel<T> &L = h.lev[l], &K = h.lev[l + 1];
     return L.d.n1 <= MG_TAIL_THREADS && (int64_t)L.ncomp * L.d.n1 <= mg_tail_rows<T>() &&
            (int64_t)K.ncomp * K.d.n1 <= MG_COARSE_MAX && h.ncoarse <= MG_COARSE_MAX;
@@ -1582,13 +1596,12 @@ static void vcycle_rec(Hierarchy<T> &h, const MGParams &mp, int l, T *x, const T
                            (const double *)h.coarse_inv, b, x);
         return;
     }
-    const T w = (T)mp.omega;
     const int nu = level_nu(mp, l);
     // pre-smoothing from zero, ping-pong between x and L.t, ending in x
     T *cur = (nu % 2) ? x : L.t, *oth = (nu % 2) ? L.t : x;
-    if (!first_done) smooth<T>(h, l, nullptr, b, cur, w, true, s);
+    if (!first_done) smooth<T>(h, l, nullptr, b, cur, (T)sweep_weight(mp, nu, 0, false), true, s);
     for (int it = 1; it < nu; ++it) {
-        smooth<T>(h, l, cur, b, oth, w, false, s);
+        smooth<T>(h, l, cur, b, oth, (T)sweep_weight(mp, nu, it, false), false, s);
         std::swap(cur, oth);
     }
     resid<T>(h, l, x, b, L.r, s);
@@ -1596,13 +1609,19 @@ static void vcycle_rec(Hierarchy<T> &h, const MGParams &mp, int l, T *x, const T
     if (tail_fits(h, mp, l + 1)) {
         // levels l+1 .. nlev-1 in one workgroup, leaving level l+1's x
         const MgLevel<T> &K = h.lev[h.nlev - 1];
+        const int nt = level_nu(mp, l + 1);
+        TailWeights<T> tw;
+        for (int it = 0; it < MG_TAIL_MAX_NU; ++it) {
+            tw.pre[it] = it < nt ? (T)sweep_weight(mp, nt, it, false) : (T)0;
+            tw.post[it] = it < nt ? (T)sweep_weight(mp, nt, it, true) : (T)0;
+        }
         on_level(h, l + 1, [&](auto pv, auto ncc) {
             using P = decltype(pv);
             constexpr int NC = decltype(ncc)::value;
             if constexpr (is_coarse_prov<P>::value)
                 hipLaunchKernelGGL((k_mg_tail<T, P, NC>), dim3(1), dim3(MG_TAIL_THREADS), 0, s, pv, L.d,
                                    (const T *)L.r, K.d, h.ncoarse, (const int *)h.coarse_idx,
-                                   (const double *)h.coarse_inv, C.x, w, (T)mp.alpha, level_nu(mp, l + 1));
+                                   (const double *)h.coarse_inv, C.x, tw, (T)mp.alpha, nt);
         });
     } else if (l + 1 < h.nlev - 1) {
         // restriction fused with the next level's first sweep
@@ -1612,7 +1631,7 @@ static void vcycle_rec(Hierarchy<T> &h, const MGParams &mp, int l, T *x, const T
             constexpr int NC = decltype(ncc)::value;
             if constexpr (is_coarse_prov<P>::value)
                 hipLaunchKernelGGL((k_mg_restrict_first<T, P, NC>), g1(C.d.n1), dim3(256), 0, s, pv, L.d,
-                                   (const T *)L.r, C.b, xf, w);
+                                   (const T *)L.r, C.b, xf, (T)sweep_weight(mp, level_nu(mp, l + 1), 0, false));
         });
         vcycle_rec<T>(h, mp, l + 1, C.x, C.b, s, true);
     } else {
@@ -1623,6 +1642,7 @@ static void vcycle_rec(Hierarchy<T> &h, const MGParams &mp, int l, T *x, const T
     cur = x;
     oth = L.t;
     for (int it = 0; it < nu; ++it) {
+        const T w = (T)sweep_weight(mp, nu, it, true);
         if (it == 0) smooth<T>(h, l, cur, b, oth, w, false, s, Prolong<T>{C.x, C.d, (T)mp.alpha});
         else smooth<T>(h, l, cur, b, oth, w, false, s);
         std::swap(cur, oth);
@@ -1643,7 +1663,8 @@ T *mg_first_target(Hierarchy<T> &h, const MGParams &mp, T *x0) {
 template <typename T>
 void launch_first_sweep(const Hierarchy<T> &h, const MGParams &mp, const T *b, T *xf, hipStream_t s) {
     const int64_t n = (int64_t)h.lev[0].ncomp * h.lev[0].d.n1;
-    hipLaunchKernelGGL(k_first_sweep<T>, g1(n), dim3(256), 0, s, b, (const T *)h.lev[0].dg, (T)mp.omega, xf, n);
+    hipLaunchKernelGGL(k_first_sweep<T>, g1(n), dim3(256), 0, s, b, (const T *)h.lev[0].dg,
+                       (T)sweep_weight(mp, level_nu(mp, 0), 0, false), xf, n);
 }
 
 template <typename TS, typename T>
@@ -1651,7 +1672,7 @@ void launch_d2t_first(const Hierarchy<T> &h, const MGParams &mp, const TS *src, 
                       hipStream_t s) {
     const int64_t n = (int64_t)h.lev[0].ncomp * h.lev[0].d.n1;
     hipLaunchKernelGGL((k_d2t_first<TS, T>), g1(n), dim3(256), 0, s, src, scale, (const T *)h.lev[0].dg,
-                       (T)mp.omega, dst, xf, n);
+                       (T)sweep_weight(mp, level_nu(mp, 0), 0, false), dst, xf, n);
 }
 
 template <typename T>

@@ -281,10 +281,14 @@ MGParams mgp(const qms_handle *h, int block = 0) {
     m.omega = h->cfg.mg_omega;
     m.alpha = h->cfg.mg_alpha;
     m.tail = h->cfg.mg_fused_tail;
+    m.cheb_lmax = h->cfg.mg_cheb_lmax;
+    m.cheb_ratio = h->cfg.mg_cheb_ratio > 1.0 ? h->cfg.mg_cheb_ratio : 4.0;
     if (block == 1) {
         if (h->cfg.mg_nu_v > 0) m.nu = h->cfg.mg_nu_v;
         if (h->cfg.mg_omega_v > 0) m.omega = h->cfg.mg_omega_v;
         if (h->cfg.mg_alpha_v > 0) m.alpha = h->cfg.mg_alpha_v;
+        m.cheb_lmax = h->cfg.mg_cheb_lmax_v;
+        m.cheb_ratio = h->cfg.mg_cheb_ratio_v > 1.0 ? h->cfg.mg_cheb_ratio_v : 4.0;
     }
     return m;
 }
@@ -638,6 +642,10 @@ int qms_config_default(qms_config *cfg) {
     cfg->mg_omega_v = 0.85;
     cfg->mg_nu_v = 6;
     cfg->mg_alpha_v = 1.8;
+    cfg->mg_cheb_lmax = 2.4;
+    cfg->mg_cheb_ratio = 4.0;
+    cfg->mg_cheb_lmax_v = 2.0;
+    cfg->mg_cheb_ratio_v = 8.0;
     return QMS_OK;
 }
 
@@ -657,6 +665,8 @@ int qms_create(const qms_config *cfg, const uint16_t *cell_slot, const double *s
     }
     if (cfg->restart < 1 || cfg->restart > 64 || cfg->mg_nu < 1 || cfg->mg_nu > 64 || cfg->mg_nu_coarse > 64 ||
         cfg->mg_nu_v > 64 || !(cfg->mg_omega_v <= 1.0) || !(cfg->tol > 0) ||
+        !(cfg->mg_cheb_lmax < 100.0) || !(cfg->mg_cheb_lmax_v < 100.0) || !(cfg->mg_cheb_ratio < 1e6) ||
+        !(cfg->mg_cheb_ratio_v < 1e6) ||
         cfg->max_iters < 1 ||
         (cfg->precision != QMS_FP64 && cfg->precision != QMS_MIXED)) {
         g_last_error = "qms_create: invalid solver parameters";

@@ -72,6 +72,10 @@ class Config(ctypes.Structure):
         ("mg_omega_v", ctypes.c_double),
         ("mg_nu_v", ctypes.c_int),
         ("mg_alpha_v", ctypes.c_double),
+        ("mg_cheb_lmax", ctypes.c_double),
+        ("mg_cheb_ratio", ctypes.c_double),
+        ("mg_cheb_lmax_v", ctypes.c_double),
+        ("mg_cheb_ratio_v", ctypes.c_double),
     ]
 
 
@@ -176,7 +180,8 @@ def make_config(cells: Sequence[int], h: Sequence[float], dt: float, bc: Sequenc
                 mg_nu: int = 4, mg_omega: float = 0.7, mg_alpha: float = 1.5, use_graphs: bool = True,
                 inner_tol: float = 1e-6, mg_nu_coarse: int = 4, floor_tol: float = 0.0,
                 mg_fused_tail: bool = True, mg_omega_v: float = 0.85, mg_nu_v: int = 6,
-                mg_alpha_v: float = 1.8) -> Config:
+                mg_alpha_v: float = 1.8, mg_cheb_lmax: float = 2.4, mg_cheb_ratio: float = 4.0,
+                mg_cheb_lmax_v: float = 2.0, mg_cheb_ratio_v: float = 8.0) -> Config:
     cfg = default_config()
     for a in range(3):
         cfg.cells[a] = int(cells[a])
@@ -202,6 +207,10 @@ def make_config(cells: Sequence[int], h: Sequence[float], dt: float, bc: Sequenc
     cfg.mg_omega_v = float(mg_omega_v)
     cfg.mg_nu_v = int(mg_nu_v)
     cfg.mg_alpha_v = float(mg_alpha_v)
+    cfg.mg_cheb_lmax = float(mg_cheb_lmax)
+    cfg.mg_cheb_ratio = float(mg_cheb_ratio)
+    cfg.mg_cheb_lmax_v = float(mg_cheb_lmax_v)
+    cfg.mg_cheb_ratio_v = float(mg_cheb_ratio_v)
     return cfg
 
 
@@ -236,11 +245,13 @@ class Solver:
                  tol: float = 1e-12, restart: int = 30, max_iters: int = 3000, mg_nu: int = 4,
                  mg_omega: float = 0.7, mg_alpha: float = 1.5, use_graphs: bool = True, inner_tol: float = 1e-6,
                  mg_nu_coarse: int = 4, floor_tol: float = 0.0, mg_fused_tail: bool = True,
-                 mg_omega_v: float = 0.85, mg_nu_v: int = 6, mg_alpha_v: float = 1.8):
+                 mg_omega_v: float = 0.85, mg_nu_v: int = 6, mg_alpha_v: float = 1.8,
+                 mg_cheb_lmax: float = 2.4, mg_cheb_ratio: float = 4.0, mg_cheb_lmax_v: float = 2.0,
+                 mg_cheb_ratio_v: float = 8.0):
         L = load()
         cfg = make_config(cells, h, dt, bc, precision, tol, restart, max_iters, mg_nu, mg_omega, mg_alpha,
                           use_graphs, inner_tol, mg_nu_coarse, floor_tol, mg_fused_tail, mg_omega_v, mg_nu_v,
-                          mg_alpha_v)
+                          mg_alpha_v, mg_cheb_lmax, mg_cheb_ratio, mg_cheb_lmax_v, mg_cheb_ratio_v)
         self.cells = tuple(int(c) for c in cells)
         self.nodes = tuple(c + 1 for c in self.cells)
         self.dt = float(dt)

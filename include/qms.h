@@ -74,6 +74,16 @@ typedef struct {
     int mg_nu_v;         /* Jacobi sweeps before and after the coarse correction on level 0 of
                             the V block (<= 0: mg_nu) */
     double mg_alpha_v;   /* coarse-correction scaling of the V block (<= 0: mg_alpha) */
+    double mg_cheb_lmax; /* > 0: Chebyshev smoothing of the A block: the nu sweeps of a level are
+                            Richardson sweeps x += w_k D^-1 (b - A x) with w_k = 1 / theta_k,
+                            theta_k the roots of the degree-nu Chebyshev polynomial on
+                            [mg_cheb_lmax / mg_cheb_ratio, mg_cheb_lmax] (mg_cheb_lmax: a bound on
+                            the spectrum of D^-1 A), the largest root first before the coarse
+                            correction and in reverse order after it; <= 0: damped Jacobi with
+                            mg_omega */
+    double mg_cheb_ratio;   /* ratio of that interval (<= 1: 4) */
+    double mg_cheb_lmax_v;  /* the same for the V block (<= 0: damped Jacobi with mg_omega_v) */
+    double mg_cheb_ratio_v; /* ratio of the V block's interval (<= 1: 4) */
 } qms_config;
 
 typedef struct {
@@ -95,9 +105,11 @@ typedef struct {
 } qms_stats;
 
 /* Fills defaults: FP64, tol 1e-12, restart 30, 3000 iterations; multigrid:
- * on level 0 nu 4 (A block) and 6 (V block), 4 on the coarse levels, omega
- * 0.7 (A block) and 0.85 (V block), alpha 1.5 (A block) and 1.8 (V block),
- * fused tail on; graphs on, inner_tol 1e-6. */
+ * on level 0 nu 4 (A block) and 6 (V block), 4 on the coarse levels,
+ * Chebyshev smoothing on [0.6, 2.4] (A block) and [0.25, 2.0] (V block)
+ * (the damped-Jacobi omegas 0.7 and 0.85 apply when it is switched off),
+ * alpha 1.5 (A block) and 1.8 (V block), fused tail on; graphs on,
+ * inner_tol 1e-6. */
 int qms_config_default(qms_config *cfg);
 
 /* Setup and material coefficients: replaces QMS_3D.f90:172-269 (sizes,

@@ -181,8 +181,11 @@ def test_time_kernel_selectors():
             # cold caches (rotating copies of the arrays): the same bytes
             ms_c, nbytes_c = s.time_kernel(which | native.K_COLD, 2, v=[1.0, 0.0, 0.0])
             assert ms_c > 0 and nbytes_c == nbytes, which
+        # the copy kernel on two 2 GiB arrays of its own: 4 GiB per launch
+        ms, nbytes = s.time_kernel(native.K_COPY_SUSTAINED, 1)
+        assert ms > 0 and nbytes == 2.0 ** 32
         with pytest.raises(native.QmsError):
-            s.time_kernel(11, 1)
+            s.time_kernel(12, 1)
     # composite selectors (whole V-cycles, one FGMRES iteration): timed, no byte count
     for prec in (native.QMS_FP64, native.QMS_MIXED):
         with solver_for(d, precision=prec) as s:
