@@ -636,13 +636,13 @@ int qms_config_default(qms_config *cfg) {
     cfg->mg_alpha = 1.5;
     cfg->use_graphs = 1;
     cfg->inner_tol = 1e-6;
-    cfg->mg_nu_coarse = 4;
+    cfg->mg_nu_coarse = 3;
     cfg->floor_tol = 0.0;
     cfg->mg_fused_tail = 1;
     cfg->mg_omega_v = 0.85;
     cfg->mg_nu_v = 6;
-    cfg->mg_alpha_v = 1.8;
-    cfg->mg_cheb_lmax = 2.4;
+    cfg->mg_alpha_v = 1.9;
+    cfg->mg_cheb_lmax = 2.6;
     cfg->mg_cheb_ratio = 4.0;
     cfg->mg_cheb_lmax_v = 2.0;
     cfg->mg_cheb_ratio_v = 8.0;

@@ -178,9 +178,9 @@ def default_config() -> Config:
 def make_config(cells: Sequence[int], h: Sequence[float], dt: float, bc: Sequence[float],
                 precision: int = QMS_FP64, tol: float = 1e-12, restart: int = 30, max_iters: int = 3000,
                 mg_nu: int = 4, mg_omega: float = 0.7, mg_alpha: float = 1.5, use_graphs: bool = True,
-                inner_tol: float = 1e-6, mg_nu_coarse: int = 4, floor_tol: float = 0.0,
+                inner_tol: float = 1e-6, mg_nu_coarse: int = 3, floor_tol: float = 0.0,
                 mg_fused_tail: bool = True, mg_omega_v: float = 0.85, mg_nu_v: int = 6,
-                mg_alpha_v: float = 1.8, mg_cheb_lmax: float = 2.4, mg_cheb_ratio: float = 4.0,
+                mg_alpha_v: float = 1.9, mg_cheb_lmax: float = 2.6, mg_cheb_ratio: float = 4.0,
                 mg_cheb_lmax_v: float = 2.0, mg_cheb_ratio_v: float = 8.0) -> Config:
     cfg = default_config()
     for a in range(3):
@@ -244,9 +244,9 @@ class Solver:
                  cell_slot: np.ndarray, slot_props: np.ndarray, precision: int = QMS_FP64,
                  tol: float = 1e-12, restart: int = 30, max_iters: int = 3000, mg_nu: int = 4,
                  mg_omega: float = 0.7, mg_alpha: float = 1.5, use_graphs: bool = True, inner_tol: float = 1e-6,
-                 mg_nu_coarse: int = 4, floor_tol: float = 0.0, mg_fused_tail: bool = True,
-                 mg_omega_v: float = 0.85, mg_nu_v: int = 6, mg_alpha_v: float = 1.8,
-                 mg_cheb_lmax: float = 2.4, mg_cheb_ratio: float = 4.0, mg_cheb_lmax_v: float = 2.0,
+                 mg_nu_coarse: int = 3, floor_tol: float = 0.0, mg_fused_tail: bool = True,
+                 mg_omega_v: float = 0.85, mg_nu_v: int = 6, mg_alpha_v: float = 1.9,
+                 mg_cheb_lmax: float = 2.6, mg_cheb_ratio: float = 4.0, mg_cheb_lmax_v: float = 2.0,
                  mg_cheb_ratio_v: float = 8.0):
         L = load()
         cfg = make_config(cells, h, dt, bc, precision, tol, restart, max_iters, mg_nu, mg_omega, mg_alpha,

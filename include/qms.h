@@ -105,10 +105,10 @@ typedef struct {
 } qms_stats;
 
 /* Fills defaults: FP64, tol 1e-12, restart 30, 3000 iterations; multigrid:
- * on level 0 nu 4 (A block) and 6 (V block), 4 on the coarse levels,
- * Chebyshev smoothing on [0.6, 2.4] (A block) and [0.25, 2.0] (V block)
+ * on level 0 nu 4 (A block) and 6 (V block), 3 on the coarse levels,
+ * Chebyshev smoothing on [0.65, 2.6] (A block) and [0.25, 2.0] (V block)
  * (the damped-Jacobi omegas 0.7 and 0.85 apply when it is switched off),
- * alpha 1.5 (A block) and 1.8 (V block), fused tail on; graphs on,
+ * alpha 1.5 (A block) and 1.9 (V block), fused tail on; graphs on,
  * inner_tol 1e-6. */
 int qms_config_default(qms_config *cfg);
 

@@ -36,8 +36,8 @@ def test_config_defaults_without_gpu():
     c = native.default_config()
     assert (c.precision, c.restart, c.mg_nu) == (native.QMS_FP64, 30, 4)
     assert c.tol == 1e-12 and c.mg_omega == 0.7 and c.mg_alpha == 1.5
-    assert (c.mg_nu_coarse, c.mg_nu_v, c.mg_omega_v, c.mg_alpha_v, c.mg_fused_tail) == (4, 6, 0.85, 1.8, 1)
-    assert (c.mg_cheb_lmax, c.mg_cheb_ratio, c.mg_cheb_lmax_v, c.mg_cheb_ratio_v) == (2.4, 4.0, 2.0, 8.0)
+    assert (c.mg_nu_coarse, c.mg_nu_v, c.mg_omega_v, c.mg_alpha_v, c.mg_fused_tail) == (3, 6, 0.85, 1.9, 1)
+    assert (c.mg_cheb_lmax, c.mg_cheb_ratio, c.mg_cheb_lmax_v, c.mg_cheb_ratio_v) == (2.6, 4.0, 2.0, 8.0)
 
 
 def test_python_config_defaults_match_the_library():
