@@ -343,7 +343,8 @@ __device__ __forceinline__ bool node_of(const LevelDims &d, int64_t p, int &i, i
 // ---------------------------------------------------------------------------
 // Kernels (one thread per node, all components of the block)
 // ---------------------------------------------------------------------------
-// Damped Jacobi: xo = x + w (b - A x) / diag; FIRST: x = 0, xo = w b / diag.
+// Weighted Jacobi (Richardson) sweep: xo = x + w (b - A x) / diag, w the sweep's weight
+// (fixed damping or a Chebyshev weight); FIRST: x = 0, xo = w b / diag.
 // Inactive rows get 0.
 template <int C, int NC, bool FIRST, typename T, typename P>
 __device__ __forceinline__ void smooth_comp(const P &pv, const LevelDims &d, int64_t p, int i, int j, int k,
@@ -456,7 +457,7 @@ __global__ void __launch_bounds__(256) k_mg_resid(P pv, const T *__restrict__ x,
 // row is selected to 0 at the end.  The loads of all components can then be
 // in flight together.  The arithmetic is that of k_mg_smooth / k_mg_resid:
 // a skipped zero term there adds a zero product here.
-// MODE 0: damped Jacobi sweep, 1: first sweep from x = 0, 2: residual.
+// MODE 0: weighted Jacobi sweep, 1: first sweep from x = 0, 2: residual.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int64_t lidx_clamped(const LevelDims &d, int i, int j, int k) {
     return lidx(d, max(0, min(i, d.nx - 1)), max(0, min(j, d.ny - 1)), max(0, min(k, d.nz - 1)));
@@ -637,7 +638,7 @@ __device__ __forceinline__ void a0_row(const Grid &g, const FineConst<T> &fk, co
 }
 
 // Level 0 of either block (NC = 3: A block, NC = 1: V block) on the z-march.
-// MODE 0: damped Jacobi sweep out = x + w (b - A x)/diag; MODE 1: residual
+// MODE 0: weighted Jacobi sweep out = x + w (b - A x)/diag; MODE 1: residual
 // out = b - A x.  Inactive rows get 0.
 template <typename T, int NC, int MODE, bool CLS, bool PRO, bool RC = false>
 __global__ void __launch_bounds__(256) k_a0_zm(A0Z<T> z, const T *__restrict__ x, const T *__restrict__ b,

@@ -374,7 +374,7 @@ def main():
         "dof_steps_per_s": head["dof_steps_per_s"],
         "max_rel_error_mixed_vs_fp64": errors,
         "rel_error_mixed_vs_fp64_by_step": err_steps,
-        "roofline": {"kernel": f"k_a0_zm (level-0 A-block damped Jacobi sweep, {sw}, 3 components per node, "
+        "roofline": {"kernel": f"k_a0_zm (level-0 A-block smoother sweep (weighted Jacobi), {sw}, 3 components per node, "
                                f"LDS z-march, coefficients "
                                f"{'from the six FP32 arrays' if main_name == 'mixed' else 'through the class table'})",
                      "bound": "hbm", "achieved": gbs_s, "peak": HBM_PEAK_GBS, "unit": "GB/s",
