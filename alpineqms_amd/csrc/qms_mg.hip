@@ -1577,11 +1577,13 @@ static bool tail_fits(const Hierarchy<T> &h, const MGParams &mp, int l) {
            (int64_t)K.ncomp * K.d.n1 <= MG_COARSE_MAX && h.ncoarse <= MG_COARSE_MAX;
 }
 
-// Where level l's first sweep from zero writes: pre-smoothing ping-pongs
-// between x and L.t and must end in x.
+// Where level l's first sweep from zero writes: the level's scratch vector
+// L.t.  The sweeps then ping-pong between L.t and x, so pre-smoothing ends
+// in L.t for odd nu and in x for even nu, and the nu post-smoothing sweeps
+// that start there end in x either way, with no copy.
 template <typename T>
-static T *first_target(Hierarchy<T> &h, const MGParams &mp, int l) {
-    return (level_nu(mp, l) % 2) ? h.lev[l].x : h.lev[l].t;
+static T *first_target(Hierarchy<T> &h, const MGParams &, int l) {
+    return h.lev[l].t;
 }
 
 // first_done: the caller's restriction already wrote the first sweep from
@@ -1598,14 +1600,14 @@ static void vcycle_rec(Hierarchy<T> &h, const MGParams &mp, int l, T *x, const T
         return;
     }
     const int nu = level_nu(mp, l);
-    // pre-smoothing from zero, ping-pong between x and L.t, ending in x
-    T *cur = (nu % 2) ? x : L.t, *oth = (nu % 2) ? L.t : x;
+    // pre-smoothing from zero, ping-pong between L.t and x
+    T *cur = L.t, *oth = x;
     if (!first_done) smooth<T>(h, l, nullptr, b, cur, (T)sweep_weight(mp, nu, 0, false), true, s);
     for (int it = 1; it < nu; ++it) {
         smooth<T>(h, l, cur, b, oth, (T)sweep_weight(mp, nu, it, false), false, s);
         std::swap(cur, oth);
     }
-    resid<T>(h, l, x, b, L.r, s);
+    resid<T>(h, l, cur, b, L.r, s);
     MgLevel<T> &C = h.lev[l + 1];
     if (tail_fits(h, mp, l + 1)) {
         // levels l+1 .. nlev-1 in one workgroup, leaving level l+1's x
@@ -1639,9 +1641,9 @@ static void vcycle_rec(Hierarchy<T> &h, const MGParams &mp, int l, T *x, const T
         hipLaunchKernelGGL(k_mg_restrict<T>, g1(C.d.n1), dim3(256), 0, s, L.d, C.d, nc, (const T *)L.r, C.b);
         vcycle_rec<T>(h, mp, l + 1, C.x, C.b, s);
     }
-    // post-smoothing; the first sweep reads x + alpha * P xc (the prolongation)
-    cur = x;
-    oth = L.t;
+    // post-smoothing from where the pre-smoothing ended (ending in x); the
+    // first sweep reads x + alpha * P xc (the prolongation)
+    oth = cur == x ? L.t : x;
     for (int it = 0; it < nu; ++it) {
         const T w = (T)sweep_weight(mp, nu, it, true);
         if (it == 0) smooth<T>(h, l, cur, b, oth, w, false, s, Prolong<T>{C.x, C.d, (T)mp.alpha});
@@ -1657,8 +1659,8 @@ void mg_vcycle(Hierarchy<T> &h, const MGParams &mp, T *x0, const T *b0, hipStrea
 }
 
 template <typename T>
-T *mg_first_target(Hierarchy<T> &h, const MGParams &mp, T *x0) {
-    return (level_nu(mp, 0) % 2) ? x0 : h.lev[0].t;
+T *mg_first_target(Hierarchy<T> &h, const MGParams &mp, T *) {
+    return first_target(h, mp, 0);
 }
 
 template <typename T>
