@@ -228,12 +228,12 @@ def test_hfield_matches_oracle(name, mk):
     assert rel(B, O.bfield(p, f)) < 1e-14
 
 
-def _run_both(d, precision, steps):
+def _run_both(d, precision, steps, **kw):
     p = problem(d)
     ref = O.run(p, steps=steps)
     coef = O.coeffs(p)
     out = []
-    with solver_for(d, precision=precision) as s:
+    with solver_for(d, precision=precision, **kw) as s:
         s.steady(program.steady_velocity(d.inputs.traj, d.inputs.dt))
         out.append((s.get_fields(), s.get_h(), s.get_h(unit_mu=True), s.stats().last_iters))
         for n in range(1, steps + 1):
@@ -242,10 +242,11 @@ def _run_both(d, precision, steps):
     return p, coef, ref, out
 
 
-def check_solve(d, precision, steps, tol=SOL_TOL):
-    """Steady state plus `steps` transient steps through the C ABI against the
-    oracle's direct solves: A, H, B and sigma*grad(V) within `tol`."""
-    p, coef, ref, out = _run_both(d, precision, steps)
+def check_solve(d, precision, steps, tol=SOL_TOL, **kw):
+    """Steady state plus `steps` transient steps through the C ABI (solver
+    options `kw`) against the oracle's direct solves: A, H, B and
+    sigma*grad(V) within `tol`."""
+    p, coef, ref, out = _run_both(d, precision, steps, **kw)
     for n, ((f, H, B, its), fr, Hr) in enumerate(zip(out, ref.fields, ref.H)):
         Br = O.bfield(p, fr)
         eA = rel(a_part(f), a_part(fr))
@@ -264,6 +265,16 @@ def check_solve(d, precision, steps, tol=SOL_TOL):
 ], ids=["N13_ramp", "N16_cfg1"])
 def test_solve_matches_direct(precision, name, mk, steps):
     check_solve(mk(), precision, steps)
+
+
+@pytest.mark.parametrize("precision", [native.QMS_FP64, native.QMS_MIXED], ids=["fp64", "mixed"])
+def test_solve_with_damped_jacobi_smoothing_matches_direct(precision):
+    """The multigrid with fixed Jacobi damping (mg_cheb_lmax = mg_cheb_lmax_v
+    = 0: every sweep weighted by mg_omega / mg_omega_v) instead of the
+    Chebyshev weights, with 4 coarse sweeps: the same parity with the
+    oracle's direct solves."""
+    d = configs.small_box_design(13, 2, nbox=3, v=4.0, ramp=True, gradient=True, steps=3)
+    check_solve(d, precision, 3, mg_cheb_lmax=0.0, mg_cheb_lmax_v=0.0, mg_nu_coarse=4, mg_alpha_v=1.8)
 
 
 @pytest.mark.parametrize("precision", [native.QMS_FP64, native.QMS_MIXED], ids=["fp64", "mixed"])
