@@ -220,6 +220,7 @@ struct MGParams {
     int tail;        // run the level above the coarsest and the coarsest solve as one launch
     double cheb_lmax;   // > 0: Chebyshev-Richardson weights on [cheb_lmax / cheb_ratio, cheb_lmax]
     double cheb_ratio;  //   instead of omega
+    int fuse_restrict;  // level 0's residual, restriction and level 1's first sweep in one launch
 };
 
 // Weight of sweep it (0-based) of a level with nu sweeps, before (post =

@@ -623,28 +623,52 @@ __device__ __forceinline__ T apply_lds(const A0View<T> &xv, const Row<T> &r, T &
 // Row of component C of an NC-component block (NC = 3: the A block's Ax, Ay,
 // Az rows; NC = 1: the V block's row) at a node of the z-march.
 template <int C, int NC, int MODE, typename T>
-__device__ __forceinline__ void a0_row(const Grid &g, const FineConst<T> &fk, const ZmTuple<T> &cf,
-                                       const A0View<T> &xv, int i, int j, int k, int64_t p, const T *bq,
-                                       T *__restrict__ out, T omega) {
-    const int64_t q = (int64_t)C * g.n1 + p;
-    if (!free_f(g, NC == 1 ? FV : C, i, j, k)) { out[q] = (T)0; return; }
+__device__ __forceinline__ T a0_value(const Grid &g, const FineConst<T> &fk, const ZmTuple<T> &cf,
+                                      const A0View<T> &xv, int i, int j, int k, const T *bq, T omega) {
+    if (!free_f(g, NC == 1 ? FV : C, i, j, k)) return (T)0;
     Row<T> r;
     if (NC == 3) row_a_raw<C, T>(cf, fk, r);
     else row_v_raw<T>(cf, fk, r);
     eliminate<C, NC, T>(g, i, j, k, r);
     T diag, x0;
     const T y = apply_lds<C, NC, T>(xv, r, diag, x0);
-    out[q] = MODE == 0 ? x0 + omega * (bq[C] - y) / diag : bq[C] - y;
+    return MODE == 0 ? x0 + omega * (bq[C] - y) / diag : bq[C] - y;
 }
+template <int C, int NC, int MODE, typename T>
+__device__ __forceinline__ void a0_row(const Grid &g, const FineConst<T> &fk, const ZmTuple<T> &cf,
+                                       const A0View<T> &xv, int i, int j, int k, int64_t p, const T *bq,
+                                       T *__restrict__ out, T omega) {
+    out[(int64_t)C * g.n1 + p] = a0_value<C, NC, MODE, T>(g, fk, cf, xv, i, j, k, bq, omega);
+}
+
+// MODE 3 of the z-march below: the residual r = b - A x of level 0
+// restricted to level 1, bc = P^T r, together with level 1's first sweep
+// from zero, xf = w1 bc / diag (0 on inactive rows), in place of the
+// residual launch and k_mg_restrict_first: the residuals of two planes stay
+// in LDS and each coarse node sums its (up to) 2 x 2 x 2 children there, in
+// the order and with the expression of k_mg_restrict_first.  Needs an even
+// z-chunk, so that both child planes of a coarse node are in one column.
+struct NoRestrict {};
+template <typename T, typename P>
+struct A0Restrict {
+    P pv;          // level 1's rows
+    T *bc, *xf;    // level 1's right-hand side and first sweep
+    T w1;          // level 1's first sweep weight
+};
 
 // Level 0 of either block (NC = 3: A block, NC = 1: V block) on the z-march.
 // MODE 0: weighted Jacobi sweep out = x + w (b - A x)/diag; MODE 1: residual
-// out = b - A x.  Inactive rows get 0.
-template <typename T, int NC, int MODE, bool CLS, bool PRO, bool RC = false>
+// out = b - A x; MODE 3: the residual restricted to level 1 (rr; out unused).
+// Inactive rows get 0.
+template <typename T, int NC, int MODE, bool CLS, bool PRO, bool RC = false, typename RR = NoRestrict>
 __global__ void __launch_bounds__(256) k_a0_zm(A0Z<T> z, const T *__restrict__ x, const T *__restrict__ b,
-                                               T *__restrict__ out, T omega, Prolong<T> pr) {
+                                               T *__restrict__ out, T omega, Prolong<T> pr, RR rr) {
     static_assert(!RC || NC == 1, "row classes: V block only");
+    static_assert((MODE == 3) != std::is_same<RR, NoRestrict>::value, "MODE 3 takes the restriction's outputs");
+    constexpr int RM = MODE == 3 ? 1 : MODE;     // the row arithmetic: MODE 3 forms residuals
     __shared__ T lds[NC * 3 * A0_PL];
+    // MODE 3: the residuals of the last two planes, [plane & 1][comp][node of the column]
+    __shared__ T rs[MODE == 3 ? 2 * NC * A0_TX * A0_TY : 1];
     Grid g = z.g;
     g.nx = __builtin_amdgcn_readfirstlane(z.g.nx);
     g.ny = __builtin_amdgcn_readfirstlane(z.g.ny);
@@ -706,13 +730,58 @@ __global__ void __launch_bounds__(256) k_a0_zm(A0Z<T> z, const T *__restrict__ x
                 rw.exc = rt[6];
                 T diag, x0;
                 const T y = apply_lds<0, 1, T>(xv, rw, diag, x0);
-                const T v = MODE == 0 ? x0 + omega * (bq[0] - y) / diag : bq[0] - y;
-                out[p] = rt[7] != (T)0 ? v : (T)0;
+                const T v = RM == 0 ? x0 + omega * (bq[0] - y) / diag : bq[0] - y;
+                const T o = rt[7] != (T)0 ? v : (T)0;
+                if constexpr (MODE == 3) rs[(kk & 1) * A0_TX * A0_TY + threadIdx.x] = o;
+                else out[p] = o;
+            } else if constexpr (MODE == 3) {
+                const int sl = (kk & 1) * NC * A0_TX * A0_TY + (int)threadIdx.x;
+                rs[sl] = a0_value<0, NC, 1, T>(g, z.fk, cf, xv, i, j, kk, bq, omega);
+                if constexpr (NC == 3) {
+                    rs[sl + A0_TX * A0_TY] = a0_value<1, NC, 1, T>(g, z.fk, cf, xv, i, j, kk, bq, omega);
+                    rs[sl + 2 * A0_TX * A0_TY] = a0_value<2, NC, 1, T>(g, z.fk, cf, xv, i, j, kk, bq, omega);
+                }
             } else {
                 a0_row<0, NC, MODE, T>(g, z.fk, cf, xv, i, j, kk, p, bq, out, omega);
                 if constexpr (NC == 3) {
                     a0_row<1, NC, MODE, T>(g, z.fk, cf, xv, i, j, kk, p, bq, out, omega);
                     a0_row<2, NC, MODE, T>(g, z.fk, cf, xv, i, j, kk, p, bq, out, omega);
+                }
+            }
+        } else if constexpr (MODE == 3) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) rs[((kk & 1) * NC + c) * A0_TX * A0_TY + threadIdx.x] = (T)0;
+        }
+        if constexpr (MODE == 3) {
+            if ((kk & 1) || kk == g.nz - 1) {
+                __syncthreads();
+                // coarse plane kk / 2: the 16 x 4 coarse nodes of this column
+                const LevelDims fd{g.nx, g.ny, g.nz, g.n1};
+                const LevelDims cd = rr.pv.dims();
+                const int I = i0 / 2 + (int)(threadIdx.x % (A0_TX / 2)), J = j0 / 2 + (int)(threadIdx.x / (A0_TX / 2));
+                const int K = kk >> 1;
+                if (threadIdx.x < (A0_TX / 2) * (A0_TY / 2) && I < cd.nx && J < cd.ny) {
+                    const int64_t pcn = lidx(cd, I, J, K);
+                    auto comp = [&](auto cc) {
+                        constexpr int C = decltype(cc)::value;
+                        T sum = (T)0;
+#pragma unroll
+                        for (int ch = 0; ch < 8; ++ch) {
+                            const int fi = 2 * I + (ch & 1), fj = 2 * J + ((ch >> 1) & 1), fk2 = 2 * K + (ch >> 2);
+                            if (inside(fd, fi, fj, fk2))
+                                sum += rs[((fk2 & 1) * NC + C) * A0_TX * A0_TY + (fj - j0) * A0_TX + (fi - i0)];
+                        }
+                        Row<T> row;
+                        const bool act = rr.pv.template row_all<C>(I, J, K, row);
+                        const T v = rr.w1 * sum / row_diag(row);
+                        rr.bc[(int64_t)C * cd.n1 + pcn] = sum;
+                        rr.xf[(int64_t)C * cd.n1 + pcn] = act ? v : (T)0;
+                    };
+                    comp(std::integral_constant<int, 0>());
+                    if constexpr (NC == 3) {
+                        comp(std::integral_constant<int, 1>());
+                        comp(std::integral_constant<int, 2>());
+                    }
                 }
             }
         }
@@ -1264,14 +1333,16 @@ void mg_setup(const Grid &g, const HostCoef &hc, const CoeffsT<T> &dc, double dt
     h.cls = direct ? nullptr : dc.cls;
     h.ctab = direct ? nullptr : dc.ctab;
     {
-        // level-0 z-march: 32 x 8 columns, the deepest z-chunk of 1..16 planes
+        // level-0 z-march: 32 x 8 columns, the deepest z-chunk of 2..16 planes
         // that leaves at least 768 workgroups (three per CU) for the A block
         // and 1536 for the one-component V block, whose workgroups are
         // lighter (measured fastest at 128^3: 8 and 4 planes)
         const int64_t cols2d = (int64_t)((g.nx + A0_TX - 1) / A0_TX) * ((g.ny + A0_TY - 1) / A0_TY);
         const int64_t want = block == 0 ? 768 : 1536;
-        int kz = 1;
-        for (int k : {2, 4, 8, 16})
+        // (at least 2, so that every coarse plane's two child planes lie in
+        // one column: the fused residual restriction needs it)
+        int kz = 2;
+        for (int k : {4, 8, 16})
             if (cols2d * ((g.nz + k - 1) / k) >= want) kz = k;
         h.zt = ztiling(g, kz, A0_TX, h.zcols);
     }
@@ -1500,7 +1571,8 @@ static void a0_launch(const Hierarchy<T> &h, const T *x, const T *b, T *out, T w
     auto go = [&](auto nc, auto cls, auto pro) {
         constexpr int NC = decltype(nc)::value;
         constexpr bool CLS = decltype(cls)::value, PRO = decltype(pro)::value;
-        hipLaunchKernelGGL((k_a0_zm<T, NC, MODE, CLS, PRO>), grid, block, 0, s, a0z(h), x, b, out, w, pr);
+        hipLaunchKernelGGL((k_a0_zm<T, NC, MODE, CLS, PRO>), grid, block, 0, s, a0z(h), x, b, out, w, pr,
+                           NoRestrict{});
     };
     using I3 = std::integral_constant<int, 3>;
     using I1 = std::integral_constant<int, 1>;
@@ -1508,8 +1580,12 @@ static void a0_launch(const Hierarchy<T> &h, const T *x, const T *b, T *out, T w
     using N = std::false_type;
     const bool pro = MODE == 0 && pr.xc != nullptr;
     if (h.block == 1 && h.lev[0].cls) {
-        if (pro) hipLaunchKernelGGL((k_a0_zm<T, 1, MODE, false, true, true>), grid, block, 0, s, a0z(h), x, b, out, w, pr);
-        else hipLaunchKernelGGL((k_a0_zm<T, 1, MODE, false, false, true>), grid, block, 0, s, a0z(h), x, b, out, w, pr);
+        if (pro)
+            hipLaunchKernelGGL((k_a0_zm<T, 1, MODE, false, true, true>), grid, block, 0, s, a0z(h), x, b, out, w, pr,
+                               NoRestrict{});
+        else
+            hipLaunchKernelGGL((k_a0_zm<T, 1, MODE, false, false, true>), grid, block, 0, s, a0z(h), x, b, out, w, pr,
+                               NoRestrict{});
         return;
     }
     if (h.block == 0) {
@@ -1519,6 +1595,42 @@ static void a0_launch(const Hierarchy<T> &h, const T *x, const T *b, T *out, T w
         if (h.cls) { if (pro) go(I1(), Y(), Y()); else go(I1(), Y(), N()); }
         else { if (pro) go(I1(), N(), Y()); else go(I1(), N(), N()); }
     }
+}
+
+// Level 0's residual restricted to level 1, with level 1's first sweep from
+// zero (MODE 3 of the z-march), when level 1 is class-compressed and the
+// z-chunk even; false: nothing launched.
+template <typename T>
+static bool a0_restrict_launch(const Hierarchy<T> &h, const T *x, const T *b, T *bc, T *xf, T w1, hipStream_t s) {
+    if (h.zt.kz % 2) return false;
+    bool done = false;
+    const dim3 grid(h.zcols), block(256);
+    const Prolong<T> none{nullptr, LevelDims{0, 0, 0, 0}, (T)0};
+    on_level(h, 1, [&](auto pv, auto ncc) {
+        using P = decltype(pv);
+        constexpr int NC = decltype(ncc)::value;
+        if constexpr (is_coarse_prov<P>::value) {
+            using RR = A0Restrict<T, P>;
+            const RR rr{pv, bc, xf, w1};
+            auto go = [&](auto cls, auto rc) {
+                constexpr bool CLS = decltype(cls)::value, RC = decltype(rc)::value;
+                hipLaunchKernelGGL((k_a0_zm<T, NC, 3, CLS, false, RC, RR>), grid, block, 0, s, a0z(h), x, b,
+                                   (T *)nullptr, (T)0, none, rr);
+            };
+            using Y = std::true_type;
+            using N = std::false_type;
+            if constexpr (NC == 1) {
+                if (h.lev[0].cls) go(N(), Y());
+                else if (h.cls) go(Y(), N());
+                else go(N(), N());
+            } else {
+                if (h.cls) go(Y(), N());
+                else go(N(), N());
+            }
+            done = true;
+        }
+    });
+    return done;
 }
 
 // One smoother sweep of level l: from x = 0 (first), or from x plus the
@@ -1607,8 +1719,13 @@ static void vcycle_rec(Hierarchy<T> &h, const MGParams &mp, int l, T *x, const T
         smooth<T>(h, l, cur, b, oth, (T)sweep_weight(mp, nu, it, false), false, s);
         std::swap(cur, oth);
     }
-    resid<T>(h, l, cur, b, L.r, s);
     MgLevel<T> &C = h.lev[l + 1];
+    // level 0: the residual, its restriction and level 1's first sweep in one
+    // launch where that applies (level 1 neither the tail nor the coarsest)
+    const bool restricted = l == 0 && mp.fuse_restrict && l + 1 < h.nlev - 1 && !tail_fits(h, mp, l + 1) &&
+                            a0_restrict_launch(h, (const T *)cur, b, C.b, first_target(h, mp, l + 1),
+                                               (T)sweep_weight(mp, level_nu(mp, l + 1), 0, false), s);
+    if (!restricted) resid<T>(h, l, cur, b, L.r, s);
     if (tail_fits(h, mp, l + 1)) {
         // levels l+1 .. nlev-1 in one workgroup, leaving level l+1's x
         const MgLevel<T> &K = h.lev[h.nlev - 1];
@@ -1629,13 +1746,14 @@ static void vcycle_rec(Hierarchy<T> &h, const MGParams &mp, int l, T *x, const T
     } else if (l + 1 < h.nlev - 1) {
         // restriction fused with the next level's first sweep
         T *xf = first_target(h, mp, l + 1);
-        on_level(h, l + 1, [&](auto pv, auto ncc) {
-            using P = decltype(pv);
-            constexpr int NC = decltype(ncc)::value;
-            if constexpr (is_coarse_prov<P>::value)
-                hipLaunchKernelGGL((k_mg_restrict_first<T, P, NC>), g1(C.d.n1), dim3(256), 0, s, pv, L.d,
-                                   (const T *)L.r, C.b, xf, (T)sweep_weight(mp, level_nu(mp, l + 1), 0, false));
-        });
+        if (!restricted)
+            on_level(h, l + 1, [&](auto pv, auto ncc) {
+                using P = decltype(pv);
+                constexpr int NC = decltype(ncc)::value;
+                if constexpr (is_coarse_prov<P>::value)
+                    hipLaunchKernelGGL((k_mg_restrict_first<T, P, NC>), g1(C.d.n1), dim3(256), 0, s, pv, L.d,
+                                       (const T *)L.r, C.b, xf, (T)sweep_weight(mp, level_nu(mp, l + 1), 0, false));
+            });
         vcycle_rec<T>(h, mp, l + 1, C.x, C.b, s, true);
     } else {
         hipLaunchKernelGGL(k_mg_restrict<T>, g1(C.d.n1), dim3(256), 0, s, L.d, C.d, nc, (const T *)L.r, C.b);

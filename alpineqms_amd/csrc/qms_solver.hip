@@ -281,6 +281,7 @@ MGParams mgp(const qms_handle *h, int block = 0) {
     m.omega = h->cfg.mg_omega;
     m.alpha = h->cfg.mg_alpha;
     m.tail = h->cfg.mg_fused_tail;
+    m.fuse_restrict = h->cfg.mg_fused_restrict;
     m.cheb_lmax = h->cfg.mg_cheb_lmax;
     m.cheb_ratio = h->cfg.mg_cheb_ratio > 1.0 ? h->cfg.mg_cheb_ratio : 4.0;
     if (block == 1) {
@@ -646,6 +647,7 @@ int qms_config_default(qms_config *cfg) {
     cfg->mg_cheb_ratio = 4.0;
     cfg->mg_cheb_lmax_v = 2.0;
     cfg->mg_cheb_ratio_v = 8.0;
+    cfg->mg_fused_restrict = 1;
     return QMS_OK;
 }
 

@@ -76,6 +76,7 @@ class Config(ctypes.Structure):
         ("mg_cheb_ratio", ctypes.c_double),
         ("mg_cheb_lmax_v", ctypes.c_double),
         ("mg_cheb_ratio_v", ctypes.c_double),
+        ("mg_fused_restrict", ctypes.c_int),
     ]
 
 
@@ -181,7 +182,8 @@ def make_config(cells: Sequence[int], h: Sequence[float], dt: float, bc: Sequenc
                 inner_tol: float = 1e-6, mg_nu_coarse: int = 3, floor_tol: float = 0.0,
                 mg_fused_tail: bool = True, mg_omega_v: float = 0.85, mg_nu_v: int = 6,
                 mg_alpha_v: float = 1.9, mg_cheb_lmax: float = 2.6, mg_cheb_ratio: float = 4.0,
-                mg_cheb_lmax_v: float = 2.0, mg_cheb_ratio_v: float = 8.0) -> Config:
+                mg_cheb_lmax_v: float = 2.0, mg_cheb_ratio_v: float = 8.0,
+                mg_fused_restrict: bool = True) -> Config:
     cfg = default_config()
     for a in range(3):
         cfg.cells[a] = int(cells[a])
@@ -211,6 +213,7 @@ def make_config(cells: Sequence[int], h: Sequence[float], dt: float, bc: Sequenc
     cfg.mg_cheb_ratio = float(mg_cheb_ratio)
     cfg.mg_cheb_lmax_v = float(mg_cheb_lmax_v)
     cfg.mg_cheb_ratio_v = float(mg_cheb_ratio_v)
+    cfg.mg_fused_restrict = 1 if mg_fused_restrict else 0
     return cfg
 
 
@@ -247,11 +250,12 @@ class Solver:
                  mg_nu_coarse: int = 3, floor_tol: float = 0.0, mg_fused_tail: bool = True,
                  mg_omega_v: float = 0.85, mg_nu_v: int = 6, mg_alpha_v: float = 1.9,
                  mg_cheb_lmax: float = 2.6, mg_cheb_ratio: float = 4.0, mg_cheb_lmax_v: float = 2.0,
-                 mg_cheb_ratio_v: float = 8.0):
+                 mg_cheb_ratio_v: float = 8.0, mg_fused_restrict: bool = True):
         L = load()
         cfg = make_config(cells, h, dt, bc, precision, tol, restart, max_iters, mg_nu, mg_omega, mg_alpha,
                           use_graphs, inner_tol, mg_nu_coarse, floor_tol, mg_fused_tail, mg_omega_v, mg_nu_v,
-                          mg_alpha_v, mg_cheb_lmax, mg_cheb_ratio, mg_cheb_lmax_v, mg_cheb_ratio_v)
+                          mg_alpha_v, mg_cheb_lmax, mg_cheb_ratio, mg_cheb_lmax_v, mg_cheb_ratio_v,
+                          mg_fused_restrict)
         self.cells = tuple(int(c) for c in cells)
         self.nodes = tuple(c + 1 for c in self.cells)
         self.dt = float(dt)

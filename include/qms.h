@@ -84,6 +84,9 @@ typedef struct {
     double mg_cheb_ratio;   /* ratio of that interval (<= 1: 4) */
     double mg_cheb_lmax_v;  /* the same for the V block (<= 0: damped Jacobi with mg_omega_v) */
     double mg_cheb_ratio_v; /* ratio of the V block's interval (<= 1: 4) */
+    int mg_fused_restrict;  /* 1: level 0's residual, its restriction to level 1 and level 1's
+                               first sweep run as one launch (where level 1 is not the tail's);
+                               0: two launches.  Same results bit for bit. */
 } qms_config;
 
 typedef struct {
@@ -108,7 +111,8 @@ typedef struct {
  * on level 0 nu 4 (A block) and 6 (V block), 3 on the coarse levels,
  * Chebyshev smoothing on [0.65, 2.6] (A block) and [0.25, 2.0] (V block)
  * (the damped-Jacobi omegas 0.7 and 0.85 apply when it is switched off),
- * alpha 1.5 (A block) and 1.9 (V block), fused tail on; graphs on,
+ * alpha 1.5 (A block) and 1.9 (V block), fused tail and fused level-0
+ * restriction on; graphs on,
  * inner_tol 1e-6. */
 int qms_config_default(qms_config *cfg);
 

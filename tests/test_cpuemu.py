@@ -104,6 +104,34 @@ def test_fused_multigrid_tail_matches_launches(cpu_lib):
     assert "ok" in out
 
 
+def test_fused_level0_restriction_matches_launches(cpu_lib):
+    """Level 0's residual, restriction and level 1's first sweep in one
+    z-march launch (mg_fused_restrict) against two launches, both precisions,
+    on a ragged design with four levels (level 1 is not the tail's):
+    bit-identical fields after the steady state and two ramp steps."""
+    out = run_child(cpu_lib, """
+        from alpineqms_amd import configs, program
+        from tests.test_gpu_parity import solver_for
+        d = configs.small_box_design(0, 3, nbox=2, v=3.0, ramp=True, gradient=True, steps=2, cells=(17, 14, 16))
+        for prec in (native.QMS_FP64, native.QMS_MIXED):
+            runs = []
+            for fused in (True, False):
+                states = []
+                with solver_for(d, precision=prec, mg_fused_restrict=fused) as s:
+                    s.steady(program.steady_velocity(d.inputs.traj, d.inputs.dt))
+                    states.append(s.get_fields())
+                    for n in (1, 2):
+                        s.step(program.step_velocity(d.inputs.traj, d.inputs.dt, n))
+                        states.append(s.get_fields())
+                    assert s.stats().n_levels_a >= 4 and s.stats().n_levels_v >= 4, s.stats().n_levels_a
+                runs.append(states)
+            for n, (a, b) in enumerate(zip(*runs)):
+                assert np.array_equal(a, b), (prec, n, float(np.max(np.abs(a - b))))
+        print("ok")
+    """)
+    assert "ok" in out
+
+
 def test_async_runs_and_served_mix_match_runs_alone(cpu_lib):
     """The asynchronous run and the serving loop with three designs in flight
     on the CPU build (library threads, one stream per handle): each design's

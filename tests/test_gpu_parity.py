@@ -299,6 +299,30 @@ def test_graph_replays_match_launches(precision):
 
 
 @pytest.mark.parametrize("precision", [native.QMS_FP64, native.QMS_MIXED], ids=["fp64", "mixed"])
+@pytest.mark.parametrize("cells", [(29, 22, 24), (40, 33, 35)], ids=["ragged30", "ragged41"])
+def test_fused_level0_restriction_matches_launches(precision, cells):
+    """Level 0's residual, its restriction to level 1 and level 1's first
+    sweep in one z-march launch (mg_fused_restrict) against the residual
+    launch and k_mg_restrict_first: the same sums in the same order, so
+    bit-identical states after the steady state and every step.  Odd node
+    counts give coarse nodes with one child along an axis, the odd nz a last
+    plane of its own."""
+    d = configs.small_box_design(0, 5, nbox=3, v=4.0, ramp=True, gradient=True, steps=3, cells=cells)
+    runs = []
+    for fused in (True, False):
+        states = []
+        with solver_for(d, precision=precision, mg_fused_restrict=fused) as s:
+            s.steady(program.steady_velocity(d.inputs.traj, d.inputs.dt))
+            states.append(s.get_fields())
+            for n in range(1, 4):
+                s.step(program.step_velocity(d.inputs.traj, d.inputs.dt, n))
+                states.append(s.get_fields())
+        runs.append(states)
+    for n, (a, b) in enumerate(zip(*runs)):
+        assert np.array_equal(a, b), (n, float(np.max(np.abs(a - b))))
+
+
+@pytest.mark.parametrize("precision", [native.QMS_FP64, native.QMS_MIXED], ids=["fp64", "mixed"])
 @pytest.mark.parametrize("cells", [(15, 15, 15), (20, 12, 27)], ids=["N16", "ragged"])
 def test_fused_multigrid_tail_matches_launches(precision, cells):
     """The V-cycle's last two levels (the 8^3 and 4^3 levels of a 16^3 design;

@@ -38,6 +38,7 @@ def test_config_defaults_without_gpu():
     assert c.tol == 1e-12 and c.mg_omega == 0.7 and c.mg_alpha == 1.5
     assert (c.mg_nu_coarse, c.mg_nu_v, c.mg_omega_v, c.mg_alpha_v, c.mg_fused_tail) == (3, 6, 0.85, 1.9, 1)
     assert (c.mg_cheb_lmax, c.mg_cheb_ratio, c.mg_cheb_lmax_v, c.mg_cheb_ratio_v) == (2.6, 4.0, 2.0, 8.0)
+    assert c.mg_fused_restrict == 1
 
 
 def test_python_config_defaults_match_the_library():
