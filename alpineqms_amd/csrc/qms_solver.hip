@@ -529,7 +529,7 @@ int fgmres(qms_handle *h, const OpParams &op, int tr) {
         // ramp it rises from about 1e-13 to 1e-11 as the fields grow; near it
         // FGMRES stagnates, whatever the restart length.  A cycle that
         // stopped improving (its own estimate, checked on the device over the
-        // last four iterations, or one that ended above half the residual it
+        // last two iterations, or one that ended above half the residual it
         // started from) is accepted when the FP64 residual is within
         // floor_tol (and counted); two such cycles in a row above it fail at
         // once instead of running to the iteration cap.

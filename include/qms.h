@@ -64,7 +64,7 @@ typedef struct {
     int mg_nu_coarse;    /* Jacobi sweeps before and after the correction on the coarse levels
                             (<= 0: mg_nu) */
     double floor_tol;    /* a solve whose FGMRES stagnates (the residual estimate improves by less
-                            than 20% over four iterations, or a cycle does not halve the residual)
+                            than 10% over two iterations, or a cycle does not halve the residual)
                             is accepted when its FP64 scaled residual is at most this, and counted
                             in qms_stats.stalled_solves (<= 0: 100 x tol) */
     int mg_fused_tail;   /* 1: the multigrid level above the coarsest (at most 8 nodes per axis)
