@@ -157,6 +157,9 @@ struct qms_handle {
     // qms_run_async: the library thread running the steps and its status
     std::thread worker;
     std::atomic<int> running{0};
+    // the thread that runs the steps, set by that thread itself before its
+    // first call (h->worker is assigned only once the thread has started)
+    std::atomic<std::thread::id> owner{};
     int async_rc = QMS_OK;
     std::vector<double> async_v;
 
@@ -601,7 +604,7 @@ void prepare_solve(qms_handle *h, int tr) {
 // True (and the handle's message set) when another thread than the handle's
 // asynchronous run calls while that run is going.
 bool busy(qms_handle *h) {
-    if (h->running.load() && std::this_thread::get_id() != h->worker.get_id()) {
+    if (h->running.load() && std::this_thread::get_id() != h->owner.load()) {
         h->err = "handle busy: an asynchronous run is going (qms_wait first)";
         return true;
     }
@@ -794,6 +797,7 @@ int qms_run_async(qms_handle *h, int nsteps, const double *v) {
     h->running.store(1);
     try {
         h->worker = std::thread([h, nsteps]() {
+            h->owner.store(std::this_thread::get_id());
             h->async_rc = qms_run(h, nsteps, h->async_v.data());
             h->running.store(0);
         });
