@@ -10,10 +10,13 @@ steady state, so the ramp is the variant whose steps carry work).
 
 Per step (QMS_3D.f90:1424-2234 restated on the GPU): right-hand side,
 Dirichlet values, extrapolated initial guess, FGMRES to a scaled relative
-residual of 1e-12 with the block-triangular multigrid preconditioner, every
+residual of 3e-13 with the block-triangular multigrid preconditioner, every
 iteration a replayed hipGraph; a step whose FGMRES stagnates at the residual's
-rounding floor (which rises above 1e-12 as the fields grow over the run) is
-accepted within 100 x the tolerance and counted (`stalled_solves`).  Fields stay on the device between steps.
+rounding floor (which rises to about 1e-11 as the fields grow over the run)
+is accepted within 1e-10 and counted (`stalled_solves`).  3e-13 is what keeps
+the mixed path's A, B and sigma*grad(V) within 1e-8 of the FP64 path's at
+every checked step (at 1e-12 sigma*grad(V) differs by 2e-8 at step 13, where
+both solves still converge).  Fields stay on the device between steps.
 
 Timed: the steady state, `--warmup` untimed steps, then (by default) every
 remaining step of the 200-step run, each a blocking call; `ms_per_step` is
@@ -54,6 +57,10 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 TOL = 1e-12
+# the headline's solves: the tolerance that keeps mixed within 1e-8 of FP64 at
+# every checked step, and the residual a stagnating solve is accepted at
+HEADLINE_TOL = 3e-13
+HEADLINE_FLOOR_TOL = 1e-10
 
 
 def traffic_from_profile(name: str):
@@ -237,7 +244,9 @@ def parser() -> argparse.ArgumentParser:
     ap.add_argument("--check-every", type=int, default=10,
                     help="compare the paths' A, B and sigma*grad(V) every this many timed steps (and at the last)")
     ap.add_argument("--nodes", type=int, default=128)
-    ap.add_argument("--tol", type=float, default=TOL, help="scaled relative residual of every solve")
+    ap.add_argument("--tol", type=float, default=HEADLINE_TOL, help="scaled relative residual of every solve")
+    ap.add_argument("--floor-tol", type=float, default=HEADLINE_FLOOR_TOL,
+                    help="scaled residual at which a stagnating solve is accepted")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch each FGMRES iteration's kernels one by one instead of replaying captured graphs "
                          "(the same kernels and results; for profilers that do not follow graph launches)")
@@ -266,7 +275,7 @@ def main():
     runs = {}
     solvers = {}
     for name in names:
-        s = make_solver(d, precs[name], tol=a.tol, use_graphs=not a.no_graphs)
+        s = make_solver(d, precs[name], tol=a.tol, floor_tol=a.floor_tol, use_graphs=not a.no_graphs)
         t0 = time.perf_counter()
         s.steady(program.steady_velocity(inp.traj, inp.dt))
         st = s.stats()
@@ -367,7 +376,7 @@ def main():
         "config": {"workload": f"config 3: {nx}x{ny}x{nz} nodes, 0.25 mm, 24 boxes + 40x40x8 slab, frame velocity "
                                f"ramp 0 -> 5 m/s over 200 steps of 2 us, steps {a.warmup + 1}-"
                                f"{a.warmup + a.steps} timed, {main_name} path, FGMRES(30) to {a.tol:g} scaled "
-                               f"residual (accepted at stagnation within 100 x that)",
+                               f"residual (accepted at stagnation within {a.floor_tol:g})",
                    "dof": d.dof},
         "paths": dict(runs, **({"mixed_over_fp64": runs["mixed"]["steps_per_s"] / runs["fp64"]["steps_per_s"]}
                                if len(names) == 2 else {})),
