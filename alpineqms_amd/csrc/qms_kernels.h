@@ -210,6 +210,7 @@ struct Hierarchy {
     int ncoarse = 0;           // active unknowns of the coarsest level
     int *coarse_idx = nullptr; // [ncoarse] -> comp*n1 + p on the coarsest level
     double *coarse_inv = nullptr;   // [ncoarse*ncoarse] row-major FP64 inverse
+    bool fused_restrict = false;    // the last V-cycle restricted level 0's residual in the z-march
 };
 
 struct MGParams {

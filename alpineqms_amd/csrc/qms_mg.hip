@@ -1725,6 +1725,7 @@ static void vcycle_rec(Hierarchy<T> &h, const MGParams &mp, int l, T *x, const T
     const bool restricted = l == 0 && mp.fuse_restrict && l + 1 < h.nlev - 1 && !tail_fits(h, mp, l + 1) &&
                             a0_restrict_launch(h, (const T *)cur, b, C.b, first_target(h, mp, l + 1),
                                                (T)sweep_weight(mp, level_nu(mp, l + 1), 0, false), s);
+    if (l == 0) h.fused_restrict = restricted;
     if (!restricted) resid<T>(h, l, cur, b, L.r, s);
     if (tail_fits(h, mp, l + 1)) {
         // levels l+1 .. nlev-1 in one workgroup, leaving level l+1's x

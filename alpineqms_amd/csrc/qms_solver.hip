@@ -959,6 +959,12 @@ int qms_get_stats(const qms_handle *h, qms_stats *st) {
     *st = h->st;
     st->device_bytes = h->bytes;
     st->coef_classes = h->ncls;
+    auto fused = [&](const auto &ps) {
+        int nb = ps.builtV && ps.hV.fused_restrict ? 1 : 0;
+        if ((ps.builtA[0] && ps.hA[0].fused_restrict) || (ps.builtA[1] && ps.hA[1].fused_restrict)) ++nb;
+        return nb;
+    };
+    st->fused_restrict_blocks = h->cfg.precision == QMS_MIXED ? fused(h->p32) : fused(h->p64);
     return QMS_OK;
 }
 

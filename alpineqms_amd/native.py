@@ -96,6 +96,7 @@ class Stats(ctypes.Structure):
         ("mg_classes_a", ctypes.c_int),
         ("mg_classes_v", ctypes.c_int),
         ("stalled_solves", ctypes.c_int64),
+        ("fused_restrict_blocks", ctypes.c_int),
     ]
 
 

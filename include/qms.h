@@ -105,6 +105,9 @@ typedef struct {
                                          V blocks (0: that level stores explicit stencil arrays) */
     int64_t stalled_solves;  /* solves accepted on stagnation within floor_tol (two stagnant cycles
                                 above it fail the solve with QMS_ERR_NOCONV) */
+    int fused_restrict_blocks;  /* multigrid blocks (0..2) whose last V-cycle ran level 0's residual,
+                                   its restriction and level 1's first sweep as one launch
+                                   (mg_fused_restrict) */
 } qms_stats;
 
 /* Fills defaults: FP64, tol 1e-12, restart 30, 3000 iterations; multigrid:

@@ -124,6 +124,7 @@ def test_fused_level0_restriction_matches_launches(cpu_lib):
                         s.step(program.step_velocity(d.inputs.traj, d.inputs.dt, n))
                         states.append(s.get_fields())
                     assert s.stats().n_levels_a >= 4 and s.stats().n_levels_v >= 4, s.stats().n_levels_a
+                    assert s.stats().fused_restrict_blocks == (2 if fused else 0)
                 runs.append(states)
             for n, (a, b) in enumerate(zip(*runs)):
                 assert np.array_equal(a, b), (prec, n, float(np.max(np.abs(a - b))))

@@ -317,6 +317,8 @@ def test_fused_level0_restriction_matches_launches(precision, cells):
             for n in range(1, 4):
                 s.step(program.step_velocity(d.inputs.traj, d.inputs.dt, n))
                 states.append(s.get_fields())
+            # both blocks took the one-launch path (or neither)
+            assert s.stats().fused_restrict_blocks == (2 if fused else 0)
         runs.append(states)
     for n, (a, b) in enumerate(zip(*runs)):
         assert np.array_equal(a, b), (n, float(np.max(np.abs(a - b))))
