@@ -212,14 +212,24 @@ __global__ void k_scale(TE *__restrict__ dst, const double *__restrict__ src, do
     else ((double *)((char *)dst + offv))[e - 3 * n1] = src[e] * a;
 }
 
-// x += sum_i y_i Z_i (Z in the preconditioner's precision; FP64 sums)
+// x += sum_i y_i Z_i (Z in the preconditioner's precision; FP64 sums, added
+// in basis order).  The loads of eight vectors are issued before their sums,
+// so a thread has eight loads in flight instead of one.
 template <typename TZ>
 __global__ void __launch_bounds__(256) k_axpy_list(double *__restrict__ x, const TZ *const *__restrict__ Z, int nz,
                                                    const double *__restrict__ y, int64_t n) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
     double s = x[e];
-    for (int i = 0; i < nz; ++i) s += y[i] * (double)Z[i][e];
+    int i = 0;
+    for (; i + 8 <= nz; i += 8) {
+        TZ v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = Z[i + u][e];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += y[i + u] * (double)v[u];
+    }
+    for (; i < nz; ++i) s += y[i] * (double)Z[i][e];
     x[e] = s;
 }
 
