@@ -591,6 +591,16 @@ __device__ __forceinline__ void load_tuple(const A0Z<T> &z, int64_t q, T im[3], 
 
 constexpr int A0_TX = 32, A0_TY = 8, A0_SY = A0_TX + 2, A0_PL = A0_SY * (A0_TY + 2);
 
+// Workgroup barrier ordering LDS only: it waits for this wave's LDS
+// operations, not for its global loads, so loads issued before it (the next
+// plane's prefetch) stay in flight across it.  Used where no thread reads
+// global memory another thread of the workgroup wrote before the barrier.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 template <typename T>
 struct A0View {
     const T *lds;      // [comp][slot][A0_PL]
@@ -685,18 +695,40 @@ __global__ void __launch_bounds__(256) k_a0_zm(A0Z<T> z, const T *__restrict__ x
     const bool live = i < g.nx && j < g.ny;
     const int ctr = (lj + 1) * A0_SY + (li + 1);
     const int64_t n1 = g.n1, plane = (int64_t)g.nx * g.ny;
-    auto stage = [&](int kp, int slot) {
+    // staging of a plane (with its one-node x/y halo): each thread owns the
+    // ring positions threadIdx.x and threadIdx.x + 256 (< A0_PL), loaded into
+    // registers (xr) one plane ahead and stored to LDS the next iteration
+    static_assert(A0_PL <= 512, "two staging positions per thread");
+    T xr[2][NC];
+    auto stage_load = [&](int kp) {
         kp = max(0, min(kp, g.nz - 1));
-        for (int e = (int)threadIdx.x; e < A0_PL; e += 256) {
-            const int ei = e % A0_SY, ej = e / A0_SY;
-            const int gi = max(0, min(i0 - 1 + ei, g.nx - 1)), gj = max(0, min(j0 - 1 + ej, g.ny - 1));
-            const int64_t q = gi + (int64_t)g.nx * gj + plane * kp;
 #pragma unroll
-            for (int c = 0; c < NC; ++c) lds[(c * 3 + slot) * A0_PL + e] = pr.template at<PRO>(x, c * n1 + q, c, gi, gj, kp);
+        for (int h2 = 0; h2 < 2; ++h2) {
+            const int e = (int)threadIdx.x + 256 * h2;
+            if (e < A0_PL) {
+                const int ei = e % A0_SY, ej = e / A0_SY;
+                const int gi = max(0, min(i0 - 1 + ei, g.nx - 1)), gj = max(0, min(j0 - 1 + ej, g.ny - 1));
+                const int64_t q = gi + (int64_t)g.nx * gj + plane * kp;
+#pragma unroll
+                for (int c = 0; c < NC; ++c) xr[h2][c] = pr.template at<PRO>(x, c * n1 + q, c, gi, gj, kp);
+            }
         }
     };
-    stage(kb - 1, 0);
-    stage(kb, 1);
+    auto stage_store = [&](int slot) {
+#pragma unroll
+        for (int h2 = 0; h2 < 2; ++h2) {
+            const int e = (int)threadIdx.x + 256 * h2;
+            if (e < A0_PL) {
+#pragma unroll
+                for (int c = 0; c < NC; ++c) lds[(c * 3 + slot) * A0_PL + e] = xr[h2][c];
+            }
+        }
+    };
+    stage_load(kb - 1);
+    stage_store(0);
+    stage_load(kb);
+    stage_store(1);
+    stage_load(kb + 1);
     const int64_t pc = ic + (int64_t)g.nx * jc;
     const int dxm = ic > 0 ? -1 : 0;
     const int64_t dym = jc > 0 ? -(int64_t)g.nx : 0;
@@ -717,8 +749,9 @@ __global__ void __launch_bounds__(256) k_a0_zm(A0Z<T> z, const T *__restrict__ x
         T bq[NC];
 #pragma unroll
         for (int c = 0; c < NC; ++c) bq[c] = live ? b[c * n1 + p] : (T)0;
-        stage(kk + 1, sp);
-        __syncthreads();
+        stage_store(sp);                    // plane kk + 1, loaded the iteration before
+        if (kk + 1 < ke) stage_load(kk + 2);   // in flight across the barriers below
+        lds_barrier();
         if (live) {
             const A0View<T> xv{lds, {sm * A0_PL, s0 * A0_PL, sp * A0_PL}, ctr};
             if constexpr (RC) {
@@ -754,7 +787,7 @@ __global__ void __launch_bounds__(256) k_a0_zm(A0Z<T> z, const T *__restrict__ x
         }
         if constexpr (MODE == 3) {
             if ((kk & 1) || kk == g.nz - 1) {
-                __syncthreads();
+                lds_barrier();
                 // coarse plane kk / 2: the 16 x 4 coarse nodes of this column
                 const LevelDims fd{g.nx, g.ny, g.nz, g.n1};
                 const LevelDims cd = rr.pv.dims();
@@ -789,7 +822,7 @@ __global__ void __launch_bounds__(256) k_a0_zm(A0Z<T> z, const T *__restrict__ x
 #pragma unroll
             for (int a = 0; a < 3; ++a) { cf.imz[a] = cf.im0[a]; cf.sgz[a] = cf.sg0[a]; }
         }
-        __syncthreads();
+        lds_barrier();
     }
 }
 

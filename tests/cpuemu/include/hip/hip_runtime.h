@@ -330,6 +330,10 @@ inline unsigned flat_tid() {
 #define hipLaunchKernelGGL(K, G, B, SH, S, ...) qemu::launch(K, dim3(G), dim3(B), (size_t)(SH), (S), ##__VA_ARGS__)
 
 inline void __syncthreads() { qemu::barrier(); }
+// the workgroup barrier and fences as builtins (LDS-only barriers): every
+// fiber stops at the barrier, and fibers share memory, so the fences are void
+#define __builtin_amdgcn_s_barrier() qemu::barrier()
+#define __builtin_amdgcn_fence(...) ((void)0)
 
 template <typename T>
 inline T __shfl_xor(T v, int mask, int width = 64) {
